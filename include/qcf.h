@@ -1,0 +1,304 @@
+/*
+ * qcf.h -- C ABI of the MI355X (gfx950) Pauli-frame sampler and bit-packed
+ * Pauli product.  Everything here is plain C: device buffers are raw pointers
+ * obtained from hipMalloc (or from a ROCArray / torch tensor), streams are
+ * hipStream_t passed as void*, and every entry point returns a status code.
+ *
+ * Reference semantics are those of QuantumClifford.jl v0.11.7; each function
+ * names the reference routine it stands in for (paths relative to the
+ * reference checkout).
+ *
+ * Conventions
+ *   - Qubits and classical bits are 1-based in qcf_op, exactly as in the
+ *     reference operation structs (sCNOT(1,2), sMRZ(4,1), ...).  Bit 0 means
+ *     "no classical bit" (reference sMZ(q,0) / sMRZ(q,0)).
+ *   - Pauli operators use the reference layout: xz = [x words ; z words], each
+ *     half `nwords` UInt64 words, qubit q at word (q-1)/64, bit (q-1)%64
+ *     (src/QuantumClifford.jl:994-1001, src/pauli_operator.jl:165).  Phases are
+ *     one byte each, 0:+1 1:+i 2:-1 3:-i (src/QuantumClifford.jl:118-121).
+ *   - Pauli frames are shot-packed: plane-major, 64 shots per UInt64 word.
+ *     A frame buffer holds 2*nq planes (x planes of qubits 1..nq, then z
+ *     planes of qubits 1..nq), each `ld` words long; shot s of plane p is bit
+ *     s%64 of word p*ld + s/64.  The measurement record uses the same packing:
+ *     column b-1 of classical bit b is words [(b-1)*ld, (b-1)*ld + ld).
+ *   - Every launching call is stream ordered and asynchronous; the caller owns
+ *     all memory passed in.  The library never frees caller memory.
+ */
+#ifndef QCF_H
+#define QCF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mirroring the reference's exceptions) ---------------- */
+#define QCF_OK 0
+#define QCF_ERR_DIMENSION 1   /* DimensionMismatch (src/mul_leftright.jl:138)   */
+#define QCF_ERR_BOUNDS 2      /* BoundsError (qubit/bit index beyond the frame)  */
+#define QCF_ERR_ARGUMENT 3    /* ArgumentError / NoZeroQubit (src/throws.jl:1)   */
+#define QCF_ERR_UNSUPPORTED 4 /* MethodError: op has no PauliFrame method        */
+#define QCF_ERR_HIP 5         /* HIP runtime failure                             */
+#define QCF_ERR_ALLOC 6       /* host or device allocation failure               */
+
+/* Thread-local text of the last error set by any qcf_* call on this thread. */
+const char *qcf_last_error(void);
+
+/* Library version string and the offload arch it was built for. */
+const char *qcf_version(void);
+
+/* ======================================================================= */
+/*  Bit-packed Pauli product (src/mul_leftright.jl)                         */
+/* ======================================================================= */
+
+/*
+ * r := l * r on two Pauli operators of `nwords` words per half.
+ * Replaces mul_left!(r::PauliOperator, l::PauliOperator; phases)
+ * (src/mul_leftright.jl:137-142).  With phases != 0:
+ *     r_phase[0] = (s + r_phase[0] + l_phase[0]) & 3,
+ * where s is the product phase of the bit arrays (src/mul_leftright.jl:123-126).
+ * With phases == 0 only the bits are updated (plain XOR).  r and l may not
+ * partially overlap; r == l is allowed (the result is the identity with
+ * s = 0).  All pointers are device pointers.
+ */
+int qcf_mul_left(uint64_t *r_xz, uint8_t *r_phase, const uint64_t *l_xz,
+                 const uint8_t *l_phase, size_t nwords, int phases,
+                 void *stream);
+
+/*
+ * l := l * r.  Replaces mul_right!(l::PauliOperator, r::PauliOperator)
+ * (src/mul_leftright.jl:144-149): phase fold ((c1 ^ c2<<1) + 2*c1) & 3
+ * (src/mul_leftright.jl:128-131), then l_phase = (s + l_phase + r_phase) & 3.
+ */
+int qcf_mul_right(uint64_t *l_xz, uint8_t *l_phase, const uint64_t *r_xz,
+                  const uint8_t *r_phase, size_t nwords, int phases,
+                  void *stream);
+
+/*
+ * Vector form: r := l * r on raw [x;z] word arrays, writing the 2-bit product
+ * phase s to out_phase[0] (device byte).  Replaces
+ * mul_left!(r::AbstractVector, l::AbstractVector)::UInt8
+ * (src/mul_leftright.jl:123-126); right = 1 selects the mul_right! fold
+ * (src/mul_leftright.jl:128-131).  out_phase may be NULL when phases == 0.
+ */
+int qcf_mul_vec(uint64_t *r_xz, const uint64_t *l_xz, size_t nwords,
+                int phases, int right, uint8_t *out_phase, void *stream);
+
+/*
+ * Row sweep over a tableau stored row-contiguous (`fastrow`): row m occupies
+ * xzs[m*ld .. m*ld + 2*nwords).  For every row m in [0, nrows) with m != skip
+ * and (row_mask == NULL || row_mask[m] != 0):
+ *     row m := src * row m,   phases[m] = (s_m + phases[m] + src_phase) & 3.
+ * `src` is either another row of the same tableau (the pivot sweep of
+ * canonicalize!, src/canonicalization.jl:105-109 over
+ * mul_left!(s::Tableau, m, i), src/mul_leftright.jl:177-181; pass
+ * src = xzs + i*ld, src_phase = phases + i, skip = i) or an independent Pauli
+ * (mul_left!(s::Tableau, p::PauliOperator), src/mul_leftright.jl:221-227;
+ * pass skip = SIZE_MAX).  `src` must not alias any row that is updated.
+ * right = 1 gives the mul_right! variants (src/mul_leftright.jl:234-240).
+ */
+int qcf_mul_rows(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nwords,
+                 size_t nrows, const uint64_t *src, const uint8_t *src_phase,
+                 size_t skip, const uint8_t *row_mask, int phases_on,
+                 int right, void *stream);
+
+/* ======================================================================= */
+/*  Pauli frames (src/pauli_frames.jl)                                      */
+/* ======================================================================= */
+
+/* Operation kinds; each names the reference struct it represents. */
+enum qcf_op_kind {
+  QCF_OP_NOP = 0,
+  /* single-qubit symbolic gates, src/symbolic_cliffords.jl:105-118 */
+  QCF_OP_ID1 = 1, QCF_OP_HADAMARD, QCF_OP_HADAMARD_XY, QCF_OP_HADAMARD_YZ,
+  QCF_OP_PHASE, QCF_OP_INV_PHASE, QCF_OP_X, QCF_OP_Y, QCF_OP_Z,
+  QCF_OP_SQRTX, QCF_OP_INV_SQRTX, QCF_OP_SQRTY, QCF_OP_INV_SQRTY,
+  QCF_OP_CXYZ, QCF_OP_CZYX,
+  /* SingleQubitOperator(q, xx, xz, zx, zz, px, pz),
+     src/symbolic_cliffords.jl:168-206: arg[0..3] = xx, xz, zx, zz */
+  QCF_OP_SINGLE_QUBIT_OPERATOR = 20,
+  /* two-qubit symbolic gates, src/symbolic_cliffords.jl:359-395 */
+  QCF_OP_SWAP = 32, QCF_OP_SWAPCX, QCF_OP_INV_SWAPCX, QCF_OP_ISWAP,
+  QCF_OP_INV_ISWAP, QCF_OP_CZSWAP, QCF_OP_CXSWAP, QCF_OP_CNOT, QCF_OP_CPHASE,
+  QCF_OP_ZCX, QCF_OP_ZCY, QCF_OP_ZCZ, QCF_OP_XCX, QCF_OP_XCY, QCF_OP_XCZ,
+  QCF_OP_YCX, QCF_OP_YCY, QCF_OP_YCZ, QCF_OP_ZCRY, QCF_OP_INV_ZCRY,
+  QCF_OP_SQRTZZ, QCF_OP_INV_SQRTZZ, QCF_OP_SQRTXX, QCF_OP_INV_SQRTXX,
+  QCF_OP_SQRTYY, QCF_OP_INV_SQRTYY,
+  /* measurements, src/pauli_frames.jl:74-116 */
+  QCF_OP_MZ = 64,   /* sMZ(q, bit)                                     */
+  QCF_OP_MRZ,       /* sMRZ(q, bit)                                    */
+  QCF_OP_MX,        /* sMX(q, bit)  = H then sMZ, no-op when bit == 0  */
+  QCF_OP_MRX,       /* sMRX(q, bit) = H then sMRZ                      */
+  /* PauliMeasurement(pauli, bit), src/pauli_frames.jl:118-134:
+     arg_off/arg_len index `args` holding one code per qubit 1..len
+     (0=I 1=X 2=Z 3=Y) followed by the phase byte. */
+  QCF_OP_PAULI_MEASUREMENT,
+  /* noise, src/pauli_frames.jl:136-174 and src/noise.jl:26-72 */
+  QCF_OP_PAULI_ERROR = 80,  /* UnbiasedUncorrelatedNoise(p[0]) on q1      */
+  QCF_OP_PAULI_NOISE,       /* PauliNoise(px=p[0], py=p[1], pz=p[2]) on q1 */
+  /* Two-qubit depolarizing on (q1,q2): with probability p[0] one of the 15
+     non-identity two-qubit Paulis, uniformly.  No PauliFrame method exists in
+     the reference; the distribution is that of
+     DepolarizationNoise(two_qubit_error_probability=p[0]) applied jointly
+     (src/noise.jl:49-53, 99-129). */
+  QCF_OP_DEPOLARIZE2,
+  /* ClassicalXOR(bits, store), src/misc_ops.jl:198-205, applied as in
+     src/pauli_frames.jl:63-72: args[arg_off .. arg_off+arg_len) are the
+     1-based input bits, `bit` is the store bit. */
+  QCF_OP_CLASSICAL_XOR = 96,
+};
+
+typedef struct qcf_op {
+  uint32_t kind;    /* enum qcf_op_kind                              */
+  uint32_t q1, q2;  /* 1-based qubits (q2 only for two-qubit ops)    */
+  uint32_t bit;     /* 1-based classical bit, 0 = none               */
+  uint32_t arg_off; /* offset into the shared `args` array           */
+  uint32_t arg_len; /* number of entries of `args` used              */
+  double p[3];      /* noise parameters                              */
+} qcf_op;
+
+/* Opaque compiled circuit: the op list lowered to the device instruction
+ * stream (qubit x/z planes renamed through permutation gates, classical
+ * bits assigned to on-chip slots, noise sites grouped into rate classes). */
+typedef struct qcf_program qcf_program;
+
+typedef struct qcf_program_info {
+  uint32_t nqubits;      /* frame qubits incl. the hidden ancilla (reference n+1) */
+  uint32_t nbits;        /* classical bits (record columns)                      */
+  uint32_t nplanes;      /* on-chip planes per shot word (2*nqubits + slots)     */
+  uint32_t ninstr;       /* device instructions                                  */
+  uint32_t nsites;       /* noise sites                                          */
+  uint32_t nclasses;     /* rate classes                                         */
+  uint32_t nreplay;      /* replay mask planes (RNG draws in program order)      */
+  uint32_t lds_bytes;    /* LDS per 2048-shot block                              */
+  double bitops_per_shot;/* frame bit-ops per shot (shared accounting)           */
+} qcf_program_info;
+
+/*
+ * Compile a circuit.  `nqubits`/`nbits` size the frame exactly like
+ * PauliFrame(frames, qubits, measurements) (src/pauli_frames.jl:30-37): the
+ * frame has nqubits+1 qubits (the extra one serves PauliMeasurement) and
+ * nbits record columns.  Pass 0 for either to size from the circuit as
+ * _create_pauliframe does (src/pauli_frames.jl:197-201).
+ * Errors: QCF_ERR_ARGUMENT (qubit 0, equal qubits in a two-qubit gate, bad
+ * probabilities), QCF_ERR_BOUNDS (index beyond the sizes given),
+ * QCF_ERR_UNSUPPORTED (no PauliFrame method in the reference, e.g. sMY).
+ */
+int qcf_program_build(const qcf_op *ops, size_t nops, const uint32_t *args,
+                      size_t nargs, uint32_t nqubits, uint32_t nbits,
+                      qcf_program **out);
+int qcf_program_free(qcf_program *prog);
+int qcf_program_get_info(const qcf_program *prog, qcf_program_info *info);
+
+/* qcf_pf_sample flags */
+#define QCF_PF_FRESH 1u        /* start from PauliFrame(...) + initZ!: x=0, z random,
+                                  record zeroed (pftrajectories(circuit; trajectories)) */
+#define QCF_PF_EXPORT_MASKS 2u /* native RNG, and write every drawn mask to `masks`
+                                  (which must be zeroed by the caller)            */
+
+/*
+ * Run the circuit on `shots` frames: pftrajectories(state::PauliFrame, circuit)
+ * (src/pauli_frames.jl:234-239) and, with QCF_PF_FRESH,
+ * pftrajectories(circuit; trajectories=shots) (src/pauli_frames.jl:193-227).
+ *
+ *   frame  : 2*info.nqubits planes of `ld` words (in/out; may be NULL with
+ *            QCF_PF_FRESH when the final frame is not wanted)
+ *   meas   : info.nbits columns of `ld` words (in/out)
+ *   masks  : NULL for the native counter-based RNG (Philox4x32-10 keyed on
+ *            `seed` and the absolute shot word); otherwise info.nreplay planes
+ *            of `ld` words that replace every random draw in program order --
+ *            the replayed-mask mode.  Planes 0..nqubits-1 are the initZ! z
+ *            words of qubits 1..nqubits (read only with QCF_PF_FRESH); then
+ *            one plane per sMRZ z-flip, two (x flip, z flip) per
+ *            single-qubit noise site and four (x1,z1,x2,z2) per DEPOLARIZE2
+ *            site, in op order.  With QCF_PF_EXPORT_MASKS the native draws
+ *            are written there instead.
+ *   ld     : words per plane, >= ceil(shots/64)
+ * Results do not depend on launch shape: the RNG stream of a shot depends only
+ * on (seed, shot index).
+ */
+int qcf_pf_sample(const qcf_program *prog, uint64_t shots, uint64_t seed,
+                  uint64_t *frame, uint64_t *meas, uint64_t *masks, size_t ld,
+                  uint32_t flags, void *stream);
+
+/* Launch-shape knobs chosen by the autotuner (0 = library default). */
+typedef struct qcf_tune {
+  uint32_t waves_per_cu; /* resident single-wave workgroups per CU (grid size) */
+  uint32_t tile_blocks;  /* 2048-shot tiles per work item of the device queue  */
+} qcf_tune;
+
+/*
+ * Batched launch (no reference counterpart; the reference runs one circuit
+ * per pftrajectories call): `n` compiled circuits, circuit k runs shots[k]
+ * fresh frames with seed seeds[k] into meas[k] (its nbits columns of ld[k]
+ * words).  qcf_batch_create uploads the per-circuit parameters and the work
+ * queue once; qcf_batch_launch enqueues ONE persistent kernel whose
+ * workgroups pull (circuit, tile range) items from a device queue, and
+ * allocates nothing, so it can be captured in a hipGraph;
+ * qcf_batch_launch_graph captures that launch once and replays the graph.
+ * Results are bit-identical to n qcf_pf_sample(..., QCF_PF_FRESH) calls.
+ */
+typedef struct qcf_batch qcf_batch;
+int qcf_batch_create(const qcf_program *const *progs, size_t n,
+                     const uint64_t *shots, const uint64_t *seeds,
+                     uint64_t *const *meas, const size_t *ld,
+                     const qcf_tune *tune, qcf_batch **out);
+int qcf_batch_launch(qcf_batch *batch, void *stream);
+int qcf_batch_launch_graph(qcf_batch *batch, void *stream);
+int qcf_batch_free(qcf_batch *batch);
+
+/* Blocking convenience: create, launch on `stream`, wait, free. */
+int qcf_pf_sample_batched(const qcf_program *const *progs, size_t n,
+                          const uint64_t *shots, const uint64_t *seeds,
+                          uint64_t *const *meas, const size_t *ld,
+                          const qcf_tune *tune, void *stream);
+
+/* Pick qcf_tune for a set of circuits by timing candidate launch shapes on
+ * the current device.  Writes only to scratch copies of the records; the
+ * choice never changes results (the RNG is keyed on absolute shots). */
+int qcf_autotune(const qcf_program *const *progs, size_t n,
+                 const uint64_t *shots, qcf_tune *out, void *stream);
+
+/* Run Philox4x32-10 on the device for n (counter[4], key[2]) records of
+ * `in` (6 words each) into `out` (4 words each): exposes the sampler's RNG
+ * to known-answer tests. */
+int qcf_debug_philox(const uint32_t *in, uint32_t *out, size_t n,
+                     void *stream);
+
+/* ---- reference-layout converters (to_gpu / to_cpu counterparts,
+ *      ext/QuantumCliffordGPUExt/adapters.jl:7-41) --------------------- */
+
+/*
+ * Shot-packed record (nbits columns of ld words) -> Matrix{Bool}(shots, nbits)
+ * column-major bytes, as returned by measurements(::PauliFrame)
+ * (src/pauli_frames.jl:280).  And the inverse.
+ */
+int qcf_record_to_bool(const uint64_t *meas, size_t ld, uint64_t shots,
+                       uint32_t nbits, uint8_t *out, void *stream);
+int qcf_record_from_bool(const uint8_t *in, uint64_t shots, uint32_t nbits,
+                         uint64_t *meas, size_t ld, void *stream);
+
+/*
+ * Shot-packed frame (2*nq planes of ld words) <-> reference frame layout:
+ * the `fastcolumn` xzs of PauliFrame.frame, i.e. a shots x 2W column-major
+ * UInt64 matrix with W = ceil(nq/64) (src/pauli_frames.jl:32,
+ * src/fastmemlayout.jl:47).  Pad bits beyond nq are written as zero by
+ * qcf_frame_to_reference and ignored by qcf_frame_from_reference.
+ */
+int qcf_frame_to_reference(const uint64_t *frame, size_t ld, uint64_t shots,
+                           uint32_t nq, uint64_t *xzs, void *stream);
+int qcf_frame_from_reference(const uint64_t *xzs, uint64_t shots, uint32_t nq,
+                             uint64_t *frame, size_t ld, void *stream);
+
+/* ---- shared accounting ------------------------------------------------- */
+
+/* Frame bit-ops per shot of a compiled circuit (CNOT 2, H 2, S 1, SWAP 4,
+ * DEPOLARIZE1 2, DEPOLARIZE2 4, X-flip 1, sMZ 1, sMRZ 3, XOR of k bits k-1). */
+double qcf_program_bitops(const qcf_program *prog);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QCF_H */

@@ -1,0 +1,876 @@
+// Pauli-frame sampler kernel for gfx950.
+//
+// One 64-lane wave owns a tile of 2048 shots: lane l holds 32 shots as one
+// 32-bit word of every plane (x and z bit of every frame qubit, plus
+// classical bits that are read again).  The planes live in LDS for the whole
+// circuit, so HBM carries only the measurement record.  The wave walks the
+// device program (src/pauli_frames.jl:234-239, one apply! per op) streamed
+// 64 instructions at a time: each lane loads one instruction with a vector
+// load and the wave reads instruction k back with v_readlane, so the program
+// fetch never waits on the LDS counter.
+//
+// Randomness (native mode) -- keyed only on (seed, absolute shot position):
+//   * gauge words (initZ!, the sMRZ z-flip, src/pauli_frames.jl:47-56,110-113):
+//     a per-lane xoshiro128** stream seeded from Philox4x32-10(seed, word);
+//   * noise events: per rate class, event positions along the (site ordinal,
+//     shot) sequence of the wave's 2048-shot tile are drawn 64 at a time, one
+//     geometric gap per lane from Philox4x32-10(seed, tile, batch, class,
+//     lane) and a wave prefix sum.  At a site the lanes holding events that
+//     fall in this site flip the target shot's bits with LDS atomics.
+// Replay mode reads every draw from caller-supplied mask planes instead.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+
+#include "qcf_common.hpp"
+#include "qcf_program.hpp"
+
+namespace qcf {
+namespace {
+
+enum Mode : int { kNative = 0, kReplay = 1, kExport = 2 };
+
+struct KParams {
+  const Instr *prog;
+  const Site *sites;
+  uint32_t ninstr;
+  uint32_t nblocks;    // 2048-shot tiles
+  uint32_t nw32;       // valid 32-bit shot words
+  uint32_t lastmask;   // valid bits of word nw32-1
+  uint64_t ld32;       // 32-bit words per plane of frame / record / masks
+  uint64_t seed;
+  uint32_t *frame;
+  uint32_t *rec;
+  uint32_t *masks;
+  uint32_t body_rbase; // replay plane of the first body draw
+  uint32_t nclasses;
+  float cls_c[kMaxClasses];
+};
+
+struct EvState {
+  uint32_t pos;    // event position along the class sequence (per lane)
+  uint32_t upau;   // uniform for the Pauli choice
+  uint32_t uacc;   // 31-bit uniform for thinning; 0xFFFFFFFF = null event
+  uint32_t last;   // position held by lane 63 (wave uniform)
+  uint32_t batch;  // batches drawn so far (wave uniform)
+};
+
+__device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t k) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k);
+}
+
+template <int C>
+__device__ __forceinline__ void gen_batch(EvState &ev, const KParams &P,
+                                          uint32_t blk, uint32_t lane) {
+  const u32x4 r = philox4x32_10(
+      u32x4{blk, ev.batch, ((uint32_t)C << 8) | lane, kStreamEvent},
+      (uint32_t)P.seed, (uint32_t)(P.seed >> 32));
+  // geometric gap: failures before the next event, g = floor(log2(u)/log2(1-p))
+  const float uf = ((float)(r.x >> 8) + 0.5f) * (1.0f / 16777216.0f);
+  const float g = __log2f(uf) * P.cls_c[C];
+  uint32_t incr;
+  bool null_ev;
+  if (!(g < 16777216.0f)) {  // gap beyond 2^24 trials: advance without an event
+    incr = 1u << 24;
+    null_ev = true;
+  } else {
+    incr = (uint32_t)g + 1u;
+    null_ev = false;
+  }
+  uint32_t v = incr;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t t = (uint32_t)__shfl_up((int)v, d, 64);
+    if ((int)lane >= d) v += t;
+  }
+  ev.pos = ev.last + v;
+  ev.last = readlane(ev.pos, 63);
+  ev.upau = r.y;
+  ev.uacc = null_ev ? 0xFFFFFFFFu : (r.z >> 1);
+  ev.batch += 1;
+}
+
+template <int MODE>
+__device__ __forceinline__ void export_flip(const KParams &P, uint32_t plane,
+                                            uint32_t gword, uint32_t bit) {
+  if (MODE != kExport) return;
+  if (gword >= P.nw32) return;
+  if (gword == P.nw32 - 1 && !(bit & P.lastmask)) return;
+  atomicXor(&P.masks[(uint64_t)plane * P.ld32 + gword], bit);
+}
+
+// Apply the events of class C that fall into this site's 2048 trials.
+template <int C, int MODE>
+__device__ __forceinline__ void site_events(EvState &ev, const Site &s,
+                                            uint32_t pa, uint32_t pb,
+                                            uint32_t pc, uint32_t pd,
+                                            uint32_t *lds, const KParams &P,
+                                            uint32_t blk, uint32_t lane) {
+  const uint32_t lo = s.ordinal * kBlockShots;
+  const uint32_t hi = lo + kBlockShots;
+  while (true) {
+    const uint32_t rel = ev.pos - lo;
+    if (rel < kBlockShots && ev.uacc < s.accept) {
+      const uint32_t tl = rel >> 5;
+      const uint32_t bit = 1u << (rel & 31u);
+      const uint32_t u = ev.upau;
+      const uint32_t rb = P.body_rbase + s.rslot;
+      const uint32_t gword = blk * 64u + tl;
+      if (s.kind == 1) {
+        const bool fx = (uint64_t)u < s.t1 || (uint64_t)u >= s.t2;  // X or Y
+        const bool fz = (uint64_t)u >= s.t1;                        // Z or Y
+        if (fx) {
+          atomicXor(&lds[pa * 64u + tl], bit);
+          export_flip<MODE>(P, rb, gword, bit);
+        }
+        if (fz) {
+          atomicXor(&lds[pb * 64u + tl], bit);
+          export_flip<MODE>(P, rb + 1, gword, bit);
+        }
+      } else {
+        // uniform over the 15 non-identity Paulis; digit order as
+        // _apply_depolarization_pauli! (src/noise.jl:112-118): first qubit
+        // is the least significant base-4 digit, 1=X 2=Z 3=Y
+        const uint32_t code = __umulhi(u, 15u) + 1u;
+        const uint32_t d1 = code & 3u, d2 = code >> 2;
+        if (d1 & 1u) {
+          atomicXor(&lds[pa * 64u + tl], bit);
+          export_flip<MODE>(P, rb, gword, bit);
+        }
+        if (d1 & 2u) {
+          atomicXor(&lds[pb * 64u + tl], bit);
+          export_flip<MODE>(P, rb + 1, gword, bit);
+        }
+        if (d2 & 1u) {
+          atomicXor(&lds[pc * 64u + tl], bit);
+          export_flip<MODE>(P, rb + 2, gword, bit);
+        }
+        if (d2 & 2u) {
+          atomicXor(&lds[pd * 64u + tl], bit);
+          export_flip<MODE>(P, rb + 3, gword, bit);
+        }
+      }
+    }
+    if (ev.last >= hi) break;
+    gen_batch<C>(ev, P, blk, lane);
+  }
+}
+
+// Run the whole program on one 2048-shot tile.
+template <int MODE>
+__device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
+                                         uint32_t *lds, uint32_t lane) {
+  const uint4 *prog4 = reinterpret_cast<const uint4 *>(P.prog);
+  {
+    const uint32_t gw = blk * 64u + lane;
+    const bool live = gw < P.nw32;
+    const uint32_t lmask = !live ? 0u : (gw == P.nw32 - 1 ? P.lastmask : 0xFFFFFFFFu);
+    const uint64_t goff = gw;
+    Xoshiro128 gauge{0, 0, 0, 1};
+    if (MODE != kReplay) gauge = gauge_stream(P.seed, gw);
+    EvState ev0{0, 0, 0xFFFFFFFFu, 0xFFFFFFFFu, 0}, ev1 = ev0, ev2 = ev0, ev3 = ev0;
+    if (MODE != kReplay) {
+      if (P.nclasses > 0) gen_batch<0>(ev0, P, blk, lane);
+      if (P.nclasses > 1) gen_batch<1>(ev1, P, blk, lane);
+      if (P.nclasses > 2) gen_batch<2>(ev2, P, blk, lane);
+      if (P.nclasses > 3) gen_batch<3>(ev3, P, blk, lane);
+    }
+    auto draw = [&](uint32_t rplane) -> uint32_t {
+      if (MODE == kReplay) return live ? P.masks[(uint64_t)rplane * P.ld32 + goff] : 0u;
+      const uint32_t v = gauge.next();
+      if (MODE == kExport && live) P.masks[(uint64_t)rplane * P.ld32 + goff] = v & lmask;
+      return v;
+    };
+    uint32_t acc = 0;
+    uint4 cur = lane < P.ninstr ? prog4[lane] : make_uint4(0, 0, 0, 0);
+    for (uint32_t base = 0; base < P.ninstr; base += 64u) {
+      const uint32_t nb = base + 64u;
+      const uint4 nxt = nb + lane < P.ninstr ? prog4[nb + lane] : make_uint4(0, 0, 0, 0);
+      const uint32_t cnt = min(64u, P.ninstr - base);
+      for (uint32_t k = 0; k < cnt; ++k) {
+        const uint32_t op = readlane(cur.x, k);
+        const uint32_t a = readlane(cur.y, k);
+        const uint32_t b = readlane(cur.z, k);
+        const uint32_t c = readlane(cur.w, k);
+        switch (op) {
+          case I_XOR2: {
+            const uint32_t c0 = c & 0xFFFFu, c1 = c >> 16;
+            const uint32_t vb = lds[b * 64u + lane];
+            const uint32_t vd = lds[c1 * 64u + lane];
+            atomicXor(&lds[a * 64u + lane], vb);
+            atomicXor(&lds[c0 * 64u + lane], vd);
+            break;
+          }
+          case I_XOR:
+            atomicXor(&lds[a * 64u + lane], lds[b * 64u + lane]);
+            break;
+          case I_NOISE1:
+          case I_NOISE2: {
+            const Site s = P.sites[c];
+            uint32_t pa, pb, pc = 0, pd = 0;
+            if (op == I_NOISE1) {
+              pa = a;
+              pb = b;
+            } else {
+              pa = a & 0xFFFFu;
+              pb = a >> 16;
+              pc = b & 0xFFFFu;
+              pd = b >> 16;
+            }
+            if (MODE == kReplay) {
+              const uint32_t rb = P.body_rbase + s.rslot;
+              const uint32_t m0 = draw(rb), m1 = draw(rb + 1);
+              lds[pa * 64u + lane] ^= m0;
+              lds[pb * 64u + lane] ^= m1;
+              if (op == I_NOISE2) {
+                const uint32_t m2 = draw(rb + 2), m3 = draw(rb + 3);
+                lds[pc * 64u + lane] ^= m2;
+                lds[pd * 64u + lane] ^= m3;
+              }
+            } else {
+              switch (s.cls) {
+                case 0: site_events<0, MODE>(ev0, s, pa, pb, pc, pd, lds, P, blk, lane); break;
+                case 1: site_events<1, MODE>(ev1, s, pa, pb, pc, pd, lds, P, blk, lane); break;
+                case 2: site_events<2, MODE>(ev2, s, pa, pb, pc, pd, lds, P, blk, lane); break;
+                case 3: site_events<3, MODE>(ev3, s, pa, pb, pc, pd, lds, P, blk, lane); break;
+                default: break;  // zero-rate site
+              }
+            }
+            break;
+          }
+          case I_MEAS: {
+            const uint32_t v = lds[a * 64u + lane];
+            if (live) P.rec[(uint64_t)b * P.ld32 + goff] = v & lmask;
+            if (c != kNone) lds[c * 64u + lane] = v;
+            break;
+          }
+          case I_RESETZ: {
+            lds[a * 64u + lane] = 0u;
+            const uint32_t r = draw(P.body_rbase + c);
+            lds[b * 64u + lane] ^= r;
+            break;
+          }
+          case I_ACC_LDS: acc = lds[a * 64u + lane]; break;
+          case I_ACCX_LDS: acc ^= lds[a * 64u + lane]; break;
+          case I_ACC_REC: acc = live ? P.rec[(uint64_t)a * P.ld32 + goff] : 0u; break;
+          case I_ACCX_REC: acc ^= live ? P.rec[(uint64_t)a * P.ld32 + goff] : 0u; break;
+          case I_ACC_STORE:
+            if (live) P.rec[(uint64_t)a * P.ld32 + goff] = acc & lmask;
+            if (b != kNone) lds[b * 64u + lane] = acc;
+            break;
+          case I_REC_ZERO:
+            if (live) P.rec[(uint64_t)a * P.ld32 + goff] = 0u;
+            break;
+          case I_INIT: {
+            lds[a * 64u + lane] = 0u;
+            const uint32_t r = draw(c);
+            lds[b * 64u + lane] = r;
+            break;
+          }
+          case I_LOAD:
+            lds[a * 64u + lane] = live ? P.frame[(uint64_t)b * P.ld32 + goff] : 0u;
+            break;
+          case I_STORE:
+            if (live) P.frame[(uint64_t)b * P.ld32 + goff] = lds[a * 64u + lane] & lmask;
+            break;
+          case I_ZERO: lds[a * 64u + lane] = 0u; break;
+          case I_LIN1: {
+            const uint32_t x = lds[a * 64u + lane], z = lds[b * 64u + lane];
+            const uint32_t nx = ((c & 1u) ? x : 0u) ^ ((c & 2u) ? z : 0u);
+            const uint32_t nz = ((c & 4u) ? x : 0u) ^ ((c & 8u) ? z : 0u);
+            lds[a * 64u + lane] = nx;
+            lds[b * 64u + lane] = nz;
+            break;
+          }
+          default: break;
+        }
+      }
+      cur = nxt;
+    }
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(64) void pf_kernel(KParams P) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t lane = threadIdx.x;
+  for (uint32_t blk = blockIdx.x; blk < P.nblocks; blk += gridDim.x)
+    run_tile<MODE>(P, blk, lds, lane);
+}
+
+// Persistent batched kernel: workgroups pull (circuit, tile range) items from
+// a device queue until it is drained.  The last workgroup to leave resets the
+// queue counters, so consecutive launches (and graph replays) need no clearing.
+struct WorkItem {
+  uint32_t circuit, blk0, nblk, pad;
+};
+
+__global__ __launch_bounds__(64) void pf_batch_kernel(const KParams *params,
+                                                      const WorkItem *items,
+                                                      uint32_t nitems,
+                                                      uint32_t *queue) {
+  extern __shared__ uint32_t lds[];
+  const uint32_t lane = threadIdx.x;
+  while (true) {
+    uint32_t it = 0;
+    if (lane == 0) it = atomicAdd(&queue[0], 1u);
+    it = readlane(it, 0);
+    if (it >= nitems) break;
+    const WorkItem w = items[it];
+    const KParams P = params[w.circuit];
+    for (uint32_t b = 0; b < w.nblk; ++b) run_tile<kNative>(P, w.blk0 + b, lds, lane);
+  }
+  if (lane == 0) {
+    __threadfence();
+    const uint32_t done = atomicAdd(&queue[1], 1u);
+    if (done == gridDim.x - 1) {
+      atomicExch(&queue[0], 0u);
+      atomicExch(&queue[1], 0u);
+    }
+  }
+}
+
+// ---- Philox self-check kernel (exposes the device RNG to tests)
+__global__ void philox_kernel(const uint32_t *in, uint32_t *out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t *c = in + 6 * i;
+  const u32x4 r = philox4x32_10(u32x4{c[0], c[1], c[2], c[3]}, c[4], c[5]);
+  out[4 * i + 0] = r.x;
+  out[4 * i + 1] = r.y;
+  out[4 * i + 2] = r.z;
+  out[4 * i + 3] = r.w;
+}
+
+// ---- reference-layout converters
+__global__ void record_to_bool_kernel(const uint64_t *meas, size_t ld,
+                                      uint64_t shots, uint32_t nbits,
+                                      uint8_t *out) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t b = blockIdx.y;
+  if (s >= shots || b >= nbits) return;
+  out[(uint64_t)b * shots + s] = (uint8_t)((meas[(uint64_t)b * ld + s / 64] >> (s % 64)) & 1u);
+}
+
+__global__ void record_from_bool_kernel(const uint8_t *in, uint64_t shots,
+                                        uint32_t nbits, uint64_t *meas,
+                                        size_t ld) {
+  const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t b = blockIdx.y;
+  if (w >= ld || b >= nbits) return;
+  uint64_t v = 0;
+  for (int i = 0; i < 64; ++i) {
+    const uint64_t s = w * 64 + i;
+    if (s < shots && in[(uint64_t)b * shots + s]) v |= 1ull << i;
+  }
+  meas[(uint64_t)b * ld + w] = v;
+}
+
+// Reference frame layout: shots x 2W column-major UInt64, column j < W holds
+// x bits of qubits 64j+1..64j+64, column W+j the z bits.
+__global__ void frame_to_reference_kernel(const uint64_t *frame, size_t ld,
+                                          uint64_t shots, uint32_t nq,
+                                          uint64_t *xzs) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t j = blockIdx.y;  // 0 .. 2W-1
+  const uint32_t W = (nq + 63) / 64;
+  if (s >= shots) return;
+  const uint32_t half = j / W, jj = j % W;
+  uint64_t v = 0;
+  for (uint32_t i = 0; i < 64; ++i) {
+    const uint32_t q = jj * 64 + i;
+    if (q >= nq) break;
+    const uint64_t plane = (uint64_t)half * nq + q;
+    v |= ((frame[plane * ld + s / 64] >> (s % 64)) & 1ull) << i;
+  }
+  xzs[(uint64_t)j * shots + s] = v;
+}
+
+__global__ void frame_from_reference_kernel(const uint64_t *xzs, uint64_t shots,
+                                            uint32_t nq, uint64_t *frame,
+                                            size_t ld) {
+  const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t plane = blockIdx.y;  // 0 .. 2nq-1
+  if (w >= ld) return;
+  const uint32_t half = plane / nq, q = plane % nq;
+  const uint32_t W = (nq + 63) / 64;
+  const uint32_t j = half * W + q / 64;
+  uint64_t v = 0;
+  for (int i = 0; i < 64; ++i) {
+    const uint64_t s = w * 64 + i;
+    if (s < shots) v |= ((xzs[(uint64_t)j * shots + s] >> (q % 64)) & 1ull) << i;
+  }
+  frame[(uint64_t)plane * ld + w] = v;
+}
+
+// ---- host side
+int ensure_device_copy(qcf_program *P, int variant, const Instr **prog,
+                       uint32_t *ninstr, const Site **sites) {
+  int dev = 0;
+  QCF_HIP_CHECK(hipGetDevice(&dev));
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  qcf_program::DeviceCopy *dc = nullptr;
+  for (auto &d : P->dev)
+    if (d.device == dev) dc = &d;
+  if (!dc) {
+    P->dev.emplace_back();
+    dc = &P->dev.back();
+    dc->device = dev;
+  }
+  if (!dc->prog[variant]) {
+    // variant bit 0: fresh prologue (else in-place LOAD); bit 1: STORE epilogue
+    std::vector<Instr> code;
+    const auto &pro = (variant & 1) ? P->prologue_fresh : P->prologue_inplace;
+    code.insert(code.end(), pro.begin(), pro.end());
+    code.insert(code.end(), P->body.begin(), P->body.end());
+    if (variant & 2) code.insert(code.end(), P->epilogue_store.begin(), P->epilogue_store.end());
+    if (code.empty()) code.push_back(Instr{I_END, 0, 0, 0});
+    void *d = nullptr;
+    QCF_HIP_CHECK(hipMalloc(&d, code.size() * sizeof(Instr)));
+    QCF_HIP_CHECK(hipMemcpy(d, code.data(), code.size() * sizeof(Instr), hipMemcpyHostToDevice));
+    dc->prog[variant] = d;
+    dc->ninstr[variant] = (uint32_t)code.size();
+  }
+  if (!dc->sites) {
+    const size_t n = std::max<size_t>(1, P->sites.size());
+    QCF_HIP_CHECK(hipMalloc(&dc->sites, n * sizeof(Site)));
+    if (!P->sites.empty())
+      QCF_HIP_CHECK(hipMemcpy(dc->sites, P->sites.data(), P->sites.size() * sizeof(Site),
+                              hipMemcpyHostToDevice));
+  }
+  *prog = (const Instr *)dc->prog[variant];
+  *ninstr = dc->ninstr[variant];
+  *sites = (const Site *)dc->sites;
+  return QCF_OK;
+}
+
+int max_lds_per_block(int *out) {
+  int dev = 0;
+  QCF_HIP_CHECK(hipGetDevice(&dev));
+  int v = 0;
+  QCF_HIP_CHECK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  *out = v;
+  return QCF_OK;
+}
+
+int allow_lds(const void *fn, uint32_t lds_bytes) {
+  if (lds_bytes <= 64u * 1024u) return QCF_OK;
+  QCF_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds_bytes));
+  return QCF_OK;
+}
+
+template <int MODE>
+int launch_pf(const KParams &kp, uint32_t lds_bytes, uint32_t grid, hipStream_t s) {
+  QCF_TRY(allow_lds((const void *)pf_kernel<MODE>, lds_bytes));
+  hipLaunchKernelGGL(pf_kernel<MODE>, dim3(grid), dim3(64), lds_bytes, s, kp);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+}  // namespace
+
+// Shared with the batched launcher.
+int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
+                  uint64_t *frame, uint64_t *meas, uint64_t *masks, size_t ld,
+                  uint32_t flags, uint32_t waves_per_cu, hipStream_t stream) {
+  qcf_program *P = const_cast<qcf_program *>(prog);
+  if (shots == 0) return QCF_OK;
+  if (ld < (shots + 63) / 64)
+    return set_error(QCF_ERR_DIMENSION, "pftrajectories: plane stride %zu words < %llu shots",
+                     ld, (unsigned long long)shots);
+  const bool fresh = flags & QCF_PF_FRESH;
+  const bool exp = flags & QCF_PF_EXPORT_MASKS;
+  if (!fresh && !frame)
+    return set_error(QCF_ERR_ARGUMENT, "pftrajectories(state, circuit) needs the frame buffer");
+  if (!meas && P->nbits > 0)
+    return set_error(QCF_ERR_ARGUMENT, "pftrajectories: null measurement buffer");
+  if (exp && !masks)
+    return set_error(QCF_ERR_ARGUMENT, "pftrajectories: mask export needs a mask buffer");
+  if (shots > (uint64_t)0xFFFFFFFFu * 16)
+    return set_error(QCF_ERR_UNSUPPORTED, "pftrajectories: too many shots in one call");
+  int lds_max = 0;
+  QCF_TRY(max_lds_per_block(&lds_max));
+  const uint32_t lds_bytes = P->nplanes * 256u;
+  if ((int)lds_bytes > lds_max)
+    return set_error(QCF_ERR_UNSUPPORTED,
+                     "pftrajectories: circuit needs %u bytes of on-chip frame per tile (limit %d)",
+                     lds_bytes, lds_max);
+  const int variant = (fresh ? 1 : 0) | (frame ? 2 : 0);
+  KParams kp{};
+  QCF_TRY(ensure_device_copy(P, variant, &kp.prog, &kp.ninstr, &kp.sites));
+  const uint64_t nw32 = (shots + 31) / 32;
+  kp.nw32 = (uint32_t)nw32;
+  kp.nblocks = (uint32_t)((nw32 + 63) / 64);
+  const uint32_t tail = (uint32_t)(shots % 32);
+  kp.lastmask = tail ? ((1u << tail) - 1u) : 0xFFFFFFFFu;
+  kp.ld32 = 2 * (uint64_t)ld;
+  kp.seed = seed;
+  kp.frame = (uint32_t *)frame;
+  kp.rec = (uint32_t *)meas;
+  kp.masks = (uint32_t *)masks;
+  kp.body_rbase = P->ninit_rs;
+  kp.nclasses = (uint32_t)P->classes.size();
+  for (uint32_t c = 0; c < kMaxClasses; ++c)
+    kp.cls_c[c] = c < P->classes.size() ? P->classes[c].c : 0.f;
+  DeviceState *st;
+  QCF_TRY(device_state(&st));
+  uint32_t wpc = waves_per_cu;
+  if (wpc == 0) wpc = std::max<uint32_t>(1, std::min<uint32_t>(16, (160u * 1024u) / std::max(1u, lds_bytes)));
+  const uint64_t grid = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * wpc);
+  if (masks && !exp) return launch_pf<kReplay>(kp, lds_bytes, (uint32_t)grid, stream);
+  if (exp) return launch_pf<kExport>(kp, lds_bytes, (uint32_t)grid, stream);
+  return launch_pf<kNative>(kp, lds_bytes, (uint32_t)grid, stream);
+}
+
+}  // namespace qcf
+
+using namespace qcf;
+
+extern "C" int qcf_pf_sample(const qcf_program *prog, uint64_t shots,
+                             uint64_t seed, uint64_t *frame, uint64_t *meas,
+                             uint64_t *masks, size_t ld, uint32_t flags,
+                             void *stream) {
+  clear_error();
+  if (!prog) return set_error(QCF_ERR_ARGUMENT, "null program");
+  return pf_launch_one(prog, shots, seed, frame, meas, masks, ld, flags, 0,
+                       (hipStream_t)stream);
+}
+
+extern "C" int qcf_debug_philox(const uint32_t *in, uint32_t *out, size_t n,
+                                void *stream) {
+  clear_error();
+  if (n == 0) return QCF_OK;
+  hipLaunchKernelGGL(philox_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, in, out, n);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+extern "C" int qcf_record_to_bool(const uint64_t *meas, size_t ld, uint64_t shots,
+                                  uint32_t nbits, uint8_t *out, void *stream) {
+  clear_error();
+  if (shots == 0 || nbits == 0) return QCF_OK;
+  if (ld < (shots + 63) / 64) return set_error(QCF_ERR_DIMENSION, "record stride too short");
+  dim3 grid((unsigned)((shots + 255) / 256), nbits);
+  hipLaunchKernelGGL(record_to_bool_kernel, grid, dim3(256), 0, (hipStream_t)stream, meas, ld,
+                     shots, nbits, out);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+extern "C" int qcf_record_from_bool(const uint8_t *in, uint64_t shots, uint32_t nbits,
+                                    uint64_t *meas, size_t ld, void *stream) {
+  clear_error();
+  if (shots == 0 || nbits == 0) return QCF_OK;
+  if (ld < (shots + 63) / 64) return set_error(QCF_ERR_DIMENSION, "record stride too short");
+  dim3 grid((unsigned)((ld + 255) / 256), nbits);
+  hipLaunchKernelGGL(record_from_bool_kernel, grid, dim3(256), 0, (hipStream_t)stream, in,
+                     shots, nbits, meas, ld);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+extern "C" int qcf_frame_to_reference(const uint64_t *frame, size_t ld, uint64_t shots,
+                                      uint32_t nq, uint64_t *xzs, void *stream) {
+  clear_error();
+  if (shots == 0 || nq == 0) return QCF_OK;
+  if (ld < (shots + 63) / 64) return set_error(QCF_ERR_DIMENSION, "frame stride too short");
+  const uint32_t W = (nq + 63) / 64;
+  dim3 grid((unsigned)((shots + 255) / 256), 2 * W);
+  hipLaunchKernelGGL(frame_to_reference_kernel, grid, dim3(256), 0, (hipStream_t)stream,
+                     frame, ld, shots, nq, xzs);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+extern "C" int qcf_frame_from_reference(const uint64_t *xzs, uint64_t shots, uint32_t nq,
+                                        uint64_t *frame, size_t ld, void *stream) {
+  clear_error();
+  if (shots == 0 || nq == 0) return QCF_OK;
+  if (ld < (shots + 63) / 64) return set_error(QCF_ERR_DIMENSION, "frame stride too short");
+  dim3 grid((unsigned)((ld + 255) / 256), 2 * nq);
+  hipLaunchKernelGGL(frame_from_reference_kernel, grid, dim3(256), 0, (hipStream_t)stream,
+                     xzs, shots, nq, frame, ld);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+// ======================================================================
+// Batched persistent launch + autotuner
+// ======================================================================
+struct qcf_batch {
+  int device = -1;
+  size_t n = 0;
+  std::vector<qcf::KParams> params;
+  std::vector<qcf::WorkItem> items;
+  qcf::KParams *d_params = nullptr;
+  qcf::WorkItem *d_items = nullptr;
+  uint32_t *d_queue = nullptr;
+  uint32_t lds_bytes = 0;
+  uint32_t grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~qcf_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (d_params) (void)hipFree(d_params);
+    if (d_items) (void)hipFree(d_items);
+    if (d_queue) (void)hipFree(d_queue);
+  }
+};
+
+namespace qcf {
+namespace {
+
+int build_kparams(const qcf_program *prog, uint64_t shots, uint64_t seed,
+                  uint64_t *meas, size_t ld, KParams *kp) {
+  qcf_program *P = const_cast<qcf_program *>(prog);
+  if (ld < (shots + 63) / 64)
+    return set_error(QCF_ERR_DIMENSION, "batched pftrajectories: stride %zu < %llu shots", ld,
+                     (unsigned long long)shots);
+  if (!meas && P->nbits > 0)
+    return set_error(QCF_ERR_ARGUMENT, "batched pftrajectories: null record");
+  *kp = KParams{};
+  QCF_TRY(ensure_device_copy(P, 1, &kp->prog, &kp->ninstr, &kp->sites));
+  const uint64_t nw32 = (shots + 31) / 32;
+  kp->nw32 = (uint32_t)nw32;
+  kp->nblocks = (uint32_t)((nw32 + 63) / 64);
+  const uint32_t tail = (uint32_t)(shots % 32);
+  kp->lastmask = tail ? ((1u << tail) - 1u) : 0xFFFFFFFFu;
+  kp->ld32 = 2 * (uint64_t)ld;
+  kp->seed = seed;
+  kp->rec = (uint32_t *)meas;
+  kp->body_rbase = P->ninit_rs;
+  kp->nclasses = (uint32_t)P->classes.size();
+  for (uint32_t c = 0; c < kMaxClasses; ++c)
+    kp->cls_c[c] = c < P->classes.size() ? P->classes[c].c : 0.f;
+  return QCF_OK;
+}
+
+void make_items(qcf_batch *B, uint32_t tile_blocks) {
+  B->items.clear();
+  std::vector<std::pair<double, WorkItem>> tmp;
+  for (size_t k = 0; k < B->n; ++k) {
+    const KParams &kp = B->params[k];
+    for (uint32_t b0 = 0; b0 < kp.nblocks; b0 += tile_blocks) {
+      const uint32_t nb = std::min(tile_blocks, kp.nblocks - b0);
+      tmp.push_back({(double)kp.ninstr * nb, WorkItem{(uint32_t)k, b0, nb, 0}});
+    }
+  }
+  // longest items first so the queue drains evenly
+  std::stable_sort(tmp.begin(), tmp.end(),
+                   [](const auto &a, const auto &b) { return a.first > b.first; });
+  for (auto &t : tmp) B->items.push_back(t.second);
+}
+
+int upload_items(qcf_batch *B) {
+  if (B->d_items) QCF_HIP_CHECK(hipFree(B->d_items));
+  B->d_items = nullptr;
+  const size_t n = std::max<size_t>(1, B->items.size());
+  QCF_HIP_CHECK(hipMalloc(&B->d_items, n * sizeof(WorkItem)));
+  if (!B->items.empty())
+    QCF_HIP_CHECK(hipMemcpy(B->d_items, B->items.data(), B->items.size() * sizeof(WorkItem),
+                            hipMemcpyHostToDevice));
+  return QCF_OK;
+}
+
+int set_tune(qcf_batch *B, const qcf_tune *tune) {
+  DeviceState *st;
+  QCF_TRY(device_state(&st));
+  uint32_t wpc = tune ? tune->waves_per_cu : 0;
+  uint32_t tb = tune ? tune->tile_blocks : 0;
+  const uint32_t lds_cap = std::max<uint32_t>(1, (160u * 1024u) / std::max(1u, B->lds_bytes));
+  if (wpc == 0) wpc = std::min<uint32_t>(16, lds_cap);
+  wpc = std::max<uint32_t>(1, std::min(wpc, lds_cap));
+  if (tb == 0) tb = 1;
+  make_items(B, tb);
+  QCF_TRY(upload_items(B));
+  B->grid = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>((uint64_t)st->cu_count * wpc, B->items.size()));
+  if (B->exec) {
+    (void)hipGraphExecDestroy(B->exec);
+    B->exec = nullptr;
+  }
+  if (B->graph) {
+    (void)hipGraphDestroy(B->graph);
+    B->graph = nullptr;
+  }
+  return QCF_OK;
+}
+
+int batch_launch(qcf_batch *B, hipStream_t s) {
+  if (B->items.empty()) return QCF_OK;
+  QCF_TRY(allow_lds((const void *)pf_batch_kernel, B->lds_bytes));
+  hipLaunchKernelGGL(pf_batch_kernel, dim3(B->grid), dim3(64), B->lds_bytes, s, B->d_params,
+                     B->d_items, (uint32_t)B->items.size(), B->d_queue);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+}  // namespace
+}  // namespace qcf
+
+extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
+                                const uint64_t *shots, const uint64_t *seeds,
+                                uint64_t *const *meas, const size_t *ld,
+                                const qcf_tune *tune, qcf_batch **out) {
+  clear_error();
+  if (!out) return set_error(QCF_ERR_ARGUMENT, "null output pointer");
+  *out = nullptr;
+  if (n && (!progs || !shots || !seeds || !meas || !ld))
+    return set_error(QCF_ERR_ARGUMENT, "null batch argument");
+  auto *B = new (std::nothrow) qcf_batch();
+  if (!B) return set_error(QCF_ERR_ALLOC, "out of host memory");
+  B->n = n;
+  QCF_HIP_CHECK(hipGetDevice(&B->device));
+  int lds_max = 0;
+  if (int st = max_lds_per_block(&lds_max); st != QCF_OK) {
+    delete B;
+    return st;
+  }
+  B->params.resize(n);
+  for (size_t k = 0; k < n; ++k) {
+    if (!progs[k]) {
+      delete B;
+      return set_error(QCF_ERR_ARGUMENT, "null program %zu", k);
+    }
+    const uint32_t lb = progs[k]->nplanes * 256u;
+    if ((int)lb > lds_max) {
+      delete B;
+      return set_error(QCF_ERR_UNSUPPORTED,
+                       "circuit %zu needs %u bytes of on-chip frame per tile (limit %d)", k, lb,
+                       lds_max);
+    }
+    B->lds_bytes = std::max(B->lds_bytes, lb);
+    if (int st = build_kparams(progs[k], shots[k], seeds[k], meas[k], ld[k], &B->params[k]);
+        st != QCF_OK) {
+      delete B;
+      return st;
+    }
+  }
+  hipError_t e = hipMalloc(&B->d_params, std::max<size_t>(1, n) * sizeof(KParams));
+  if (e == hipSuccess && n)
+    e = hipMemcpy(B->d_params, B->params.data(), n * sizeof(KParams), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&B->d_queue, 2 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemset(B->d_queue, 0, 2 * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    delete B;
+    return set_error(QCF_ERR_HIP, "batch setup: %s", hipGetErrorString(e));
+  }
+  if (int st = set_tune(B, tune); st != QCF_OK) {
+    delete B;
+    return st;
+  }
+  *out = B;
+  return QCF_OK;
+}
+
+extern "C" int qcf_batch_launch(qcf_batch *B, void *stream) {
+  clear_error();
+  if (!B) return set_error(QCF_ERR_ARGUMENT, "null batch");
+  return batch_launch(B, (hipStream_t)stream);
+}
+
+extern "C" int qcf_batch_launch_graph(qcf_batch *B, void *stream) {
+  clear_error();
+  if (!B) return set_error(QCF_ERR_ARGUMENT, "null batch");
+  hipStream_t s = (hipStream_t)stream;
+  if (!B->exec) {
+    hipStream_t cap;
+    QCF_HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    QCF_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    const int st = batch_launch(B, cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(cap, &g);
+    (void)hipStreamDestroy(cap);
+    if (st != QCF_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (e != hipSuccess) return set_error(QCF_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    B->graph = g;
+    QCF_HIP_CHECK(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+  }
+  QCF_HIP_CHECK(hipGraphLaunch(B->exec, s));
+  return QCF_OK;
+}
+
+extern "C" int qcf_batch_free(qcf_batch *B) {
+  delete B;
+  return QCF_OK;
+}
+
+extern "C" int qcf_pf_sample_batched(const qcf_program *const *progs, size_t n,
+                                     const uint64_t *shots, const uint64_t *seeds,
+                                     uint64_t *const *meas, const size_t *ld,
+                                     const qcf_tune *tune, void *stream) {
+  qcf_batch *B = nullptr;
+  QCF_TRY(qcf_batch_create(progs, n, shots, seeds, meas, ld, tune, &B));
+  int st = qcf_batch_launch(B, stream);
+  if (st == QCF_OK) {
+    const hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) st = set_error(QCF_ERR_HIP, "batch sync: %s", hipGetErrorString(e));
+  }
+  qcf_batch_free(B);
+  return st;
+}
+
+extern "C" int qcf_autotune(const qcf_program *const *progs, size_t n,
+                            const uint64_t *shots, qcf_tune *out, void *stream) {
+  clear_error();
+  if (!out) return set_error(QCF_ERR_ARGUMENT, "null output pointer");
+  *out = qcf_tune{0, 0};
+  if (n == 0) return QCF_OK;
+  hipStream_t s = (hipStream_t)stream;
+  // scratch records so the caller's buffers are untouched
+  std::vector<uint64_t *> bufs(n, nullptr);
+  std::vector<size_t> lds(n);
+  std::vector<uint64_t> seeds(n, 0);
+  int st = QCF_OK;
+  for (size_t k = 0; k < n && st == QCF_OK; ++k) {
+    lds[k] = std::max<uint64_t>(1, (shots[k] + 63) / 64);
+    const size_t bytes = std::max<size_t>(1, progs[k]->nbits) * lds[k] * 8;
+    if (hipMalloc(&bufs[k], bytes) != hipSuccess) st = set_error(QCF_ERR_ALLOC, "autotune scratch");
+  }
+  qcf_batch *B = nullptr;
+  if (st == QCF_OK) st = qcf_batch_create(progs, n, shots, seeds.data(), bufs.data(), lds.data(), nullptr, &B);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (st == QCF_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    st = set_error(QCF_ERR_HIP, "autotune events");
+  float best = 1e30f;
+  qcf_tune best_t{0, 0};
+  const uint32_t lds_cap = B ? std::max<uint32_t>(1, (160u * 1024u) / std::max(1u, B->lds_bytes)) : 1;
+  const uint32_t wpcs[] = {1, 2, 4, 8, 16};
+  const uint32_t tbs[] = {1, 2, 4, 8};
+  for (uint32_t wpc : wpcs) {
+    if (st != QCF_OK || wpc > lds_cap) break;
+    for (uint32_t tb : tbs) {
+      qcf_tune t{wpc, tb};
+      if ((st = qcf::set_tune(B, &t)) != QCF_OK) break;
+      if ((st = qcf::batch_launch(B, s)) != QCF_OK) break;  // warm
+      if (hipEventRecord(e0, s) != hipSuccess) { st = set_error(QCF_ERR_HIP, "event"); break; }
+      if ((st = qcf::batch_launch(B, s)) != QCF_OK) break;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+        st = set_error(QCF_ERR_HIP, "event");
+        break;
+      }
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        best_t = t;
+      }
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (B) qcf_batch_free(B);
+  for (auto *p : bufs)
+    if (p) (void)hipFree(p);
+  if (st == QCF_OK) *out = best_t;
+  return st;
+}

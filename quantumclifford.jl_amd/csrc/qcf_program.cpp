@@ -1,0 +1,695 @@
+// Host compiler: reference operation list -> device plane program.
+//
+// Lowering rules (reference semantics cited per case):
+//   * symbolic Cliffords act on (x,z) bits through a GF(2)-linear map; phases
+//     are discarded on frames (src/pauli_frames.jl:58-61 passes
+//     phases=Val(false)).  Each map M is realised as P*M = E_k...E_1 with
+//     elementary plane XORs E_i and a plane renaming P, found by breadth-first
+//     search over GL(n,2), so H, SWAP, ... cost no device work.
+//   * sMZ / sMRZ / sMX / sMRX follow src/pauli_frames.jl:74-116 exactly,
+//     including "bit == 0 is a no-op" for sMZ/sMX and "reset still happens"
+//     for sMRZ/sMRX.
+//   * PauliError / PauliNoise sites follow the thresholds of
+//     src/pauli_frames.jl:136-174; DEPOLARIZE2 is this library's two-qubit
+//     channel (uniform over the 15 non-identity Paulis).
+//   * ClassicalXOR follows src/pauli_frames.jl:63-72; bit values that are read
+//     again later stay on chip in classical planes.
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "qcf_common.hpp"
+#include "qcf_program.hpp"
+
+namespace qcf {
+namespace {
+
+// GF(2) bit maps of the symbolic gates.  Row i is the mask of inputs (bit0=x1,
+// bit1=z1, bit2=x2, bit3=z2) whose XOR gives output i (x1', z1', x2', z2').
+// Taken from the qubit_kernel tables of src/symbolic_cliffords.jl:105-118
+// (single qubit) and :359-395 (two qubit), phase outputs dropped.
+struct GateMap {
+  uint32_t kind;
+  int nq;
+  uint8_t rows[4];
+};
+
+constexpr GateMap kGateMaps[] = {
+    {QCF_OP_ID1, 1, {1, 2}},         {QCF_OP_X, 1, {1, 2}},
+    {QCF_OP_Y, 1, {1, 2}},           {QCF_OP_Z, 1, {1, 2}},
+    {QCF_OP_HADAMARD, 1, {2, 1}},    {QCF_OP_HADAMARD_XY, 1, {1, 3}},
+    {QCF_OP_HADAMARD_YZ, 1, {3, 2}}, {QCF_OP_PHASE, 1, {1, 3}},
+    {QCF_OP_INV_PHASE, 1, {1, 3}},   {QCF_OP_SQRTX, 1, {3, 2}},
+    {QCF_OP_INV_SQRTX, 1, {3, 2}},   {QCF_OP_SQRTY, 1, {2, 1}},
+    {QCF_OP_INV_SQRTY, 1, {2, 1}},   {QCF_OP_CXYZ, 1, {3, 1}},
+    {QCF_OP_CZYX, 1, {2, 3}},
+    {QCF_OP_SWAP, 2, {4, 8, 1, 2}},        {QCF_OP_SWAPCX, 2, {4, 10, 5, 2}},
+    {QCF_OP_INV_SWAPCX, 2, {5, 8, 1, 10}}, {QCF_OP_ISWAP, 2, {4, 13, 1, 7}},
+    {QCF_OP_INV_ISWAP, 2, {4, 13, 1, 7}},  {QCF_OP_CZSWAP, 2, {4, 9, 1, 6}},
+    {QCF_OP_CXSWAP, 2, {5, 8, 1, 10}},     {QCF_OP_CNOT, 2, {1, 10, 5, 8}},
+    {QCF_OP_CPHASE, 2, {1, 6, 4, 9}},      {QCF_OP_ZCX, 2, {1, 10, 5, 8}},
+    {QCF_OP_ZCY, 2, {1, 14, 5, 9}},        {QCF_OP_ZCZ, 2, {1, 6, 4, 9}},
+    {QCF_OP_XCX, 2, {9, 2, 6, 8}},         {QCF_OP_XCY, 2, {13, 2, 6, 10}},
+    {QCF_OP_XCZ, 2, {5, 2, 4, 10}},        {QCF_OP_YCX, 2, {9, 10, 7, 8}},
+    {QCF_OP_YCY, 2, {13, 14, 7, 11}},      {QCF_OP_YCZ, 2, {5, 6, 4, 11}},
+    {QCF_OP_ZCRY, 2, {1, 15, 5, 9}},       {QCF_OP_INV_ZCRY, 2, {1, 15, 5, 9}},
+    {QCF_OP_SQRTZZ, 2, {1, 7, 4, 13}},     {QCF_OP_INV_SQRTZZ, 2, {1, 7, 4, 13}},
+    {QCF_OP_SQRTXX, 2, {11, 2, 14, 8}},    {QCF_OP_INV_SQRTXX, 2, {11, 2, 14, 8}},
+    {QCF_OP_SQRTYY, 2, {14, 13, 11, 7}},   {QCF_OP_INV_SQRTYY, 2, {14, 13, 11, 7}},
+};
+
+const GateMap *find_gate(uint32_t kind) {
+  for (const auto &g : kGateMaps)
+    if (g.kind == kind) return &g;
+  return nullptr;
+}
+
+// ---- decomposition of GF(2) maps into plane XORs + renaming --------------
+// A matrix of n<=4 rows is packed 4 bits per row into a uint16.
+struct Decomp {
+  int len = -1;              // -1: singular (no decomposition)
+  uint8_t ops[16][2];        // (dst, src): slot[dst] ^= slot[src]
+  uint8_t perm[4];           // output i ends in slot perm_inv...: see apply
+};
+
+struct GLTables {
+  int n;
+  std::vector<int8_t> dist;
+  std::vector<uint16_t> parent;
+  std::vector<uint8_t> pop;  // op index used to reach a state
+  explicit GLTables(int n_) : n(n_), dist(65536, -1), parent(65536), pop(65536) {
+    uint16_t id = 0;
+    for (int i = 0; i < n; ++i) id |= (uint16_t)(1u << i) << (4 * i);
+    std::vector<uint16_t> q{id};
+    dist[id] = 0;
+    for (size_t h = 0; h < q.size(); ++h) {
+      const uint16_t s = q[h];
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+          if (i == j) continue;
+          const uint16_t rj = (s >> (4 * j)) & 0xF;
+          const uint16_t t = s ^ (uint16_t)(rj << (4 * i));
+          if (dist[t] >= 0) continue;
+          dist[t] = dist[s] + 1;
+          parent[t] = s;
+          pop[t] = (uint8_t)(i * 4 + j);
+          q.push_back(t);
+        }
+    }
+  }
+};
+
+const GLTables &gl_tables(int n) {
+  static GLTables t2(2), t4(4);
+  return n == 2 ? t2 : t4;
+}
+
+// Find E_k..E_1 and a permutation sigma with E_k..E_1 = P_sigma * M, where
+// (P_sigma M) row i = M row sigma(i).  After executing the XORs, slot i holds
+// logical output sigma(i).
+Decomp decompose(const uint8_t *rows, int n) {
+  const GLTables &T = gl_tables(n);
+  std::array<int, 4> sigma{0, 1, 2, 3}, best_sigma{};
+  int best = 1 << 30;
+  uint16_t best_state = 0;
+  do {
+    bool valid = true;
+    for (int i = n; i < 4; ++i)
+      if (sigma[i] != i) valid = false;
+    if (!valid) continue;
+    uint16_t s = 0;
+    for (int i = 0; i < n; ++i) s |= (uint16_t)(rows[sigma[i]] & 0xF) << (4 * i);
+    if (T.dist[s] >= 0 && T.dist[s] < best) {
+      best = T.dist[s];
+      best_state = s;
+      best_sigma = sigma;
+    }
+  } while (std::next_permutation(sigma.begin(), sigma.end()));
+  Decomp d;
+  if (best == (1 << 30)) return d;
+  d.len = best;
+  uint16_t s = best_state;
+  for (int k = best - 1; k >= 0; --k) {
+    const uint8_t o = T.pop[s];
+    d.ops[k][0] = o / 4;
+    d.ops[k][1] = o % 4;
+    s = T.parent[s];
+  }
+  for (int i = 0; i < 4; ++i) d.perm[i] = (uint8_t)best_sigma[i];
+  return d;
+}
+
+const Decomp &gate_decomp(const GateMap &g) {
+  static std::mutex mu;
+  static std::vector<std::pair<uint32_t, Decomp>> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  for (auto &e : cache)
+    if (e.first == g.kind) return e.second;
+  cache.emplace_back(g.kind, decompose(g.rows, 2 * g.nq));
+  return cache.back().second;
+}
+
+int popcount4(uint32_t v) { return __builtin_popcount(v & 0xF); }
+
+// ---------------------------------------------------------------- compiler -
+struct Compiler {
+  qcf_program *P;
+  uint32_t nq, nbits;
+  std::vector<uint32_t> xs, zs;   // plane slot of x / z of qubit q (0-based)
+  std::vector<uint32_t> bit_slot; // classical slot holding bit b, or kNone
+  std::vector<uint32_t> reads_left;
+  std::vector<uint32_t> free_slots;
+  uint32_t next_slot;
+  uint32_t rs = 0;                // next body replay plane
+  std::vector<double> site_pe;    // event rate of each site
+  std::vector<uint8_t> written, zero_first;
+
+  Compiler(qcf_program *p, uint32_t nq_, uint32_t nbits_)
+      : P(p), nq(nq_), nbits(nbits_), xs(nq_), zs(nq_), bit_slot(nbits_, kNone),
+        reads_left(nbits_, 0), next_slot(2 * nq_), written(nbits_, 0),
+        zero_first(nbits_, 0) {
+    for (uint32_t q = 0; q < nq; ++q) {
+      xs[q] = 2 * q;
+      zs[q] = 2 * q + 1;
+    }
+  }
+
+  void emit(uint32_t op, uint32_t a, uint32_t b = 0, uint32_t c = 0) {
+    P->body.push_back(Instr{op, a, b, c});
+  }
+
+  uint32_t alloc_slot() {
+    if (!free_slots.empty()) {
+      uint32_t s = free_slots.back();
+      free_slots.pop_back();
+      return s;
+    }
+    return next_slot++;
+  }
+
+  // ---- gates
+  void gate(const GateMap &g, uint32_t q1, uint32_t q2) {
+    uint32_t slot[4] = {xs[q1], zs[q1], 0, 0};
+    if (g.nq == 2) {
+      slot[2] = xs[q2];
+      slot[3] = zs[q2];
+    }
+    const Decomp &d = gate_decomp(g);
+    for (int k = 0; k < d.len; ++k)
+      emit(I_XOR, slot[d.ops[k][0]], slot[d.ops[k][1]]);
+    uint32_t ns[4];
+    for (int i = 0; i < 2 * g.nq; ++i) ns[d.perm[i]] = slot[i];
+    xs[q1] = ns[0];
+    zs[q1] = ns[1];
+    if (g.nq == 2) {
+      xs[q2] = ns[2];
+      zs[q2] = ns[3];
+    }
+    for (int i = 0; i < 2 * g.nq; ++i)
+      if (g.rows[i] != (1u << i)) P->bitops += 1;
+  }
+
+  int single_qubit_operator(uint32_t q, const uint32_t *m) {
+    // x' = (x & xx) ^ (z & zx),  z' = (x & xz) ^ (z & zz)
+    // (src/symbolic_cliffords.jl:184-189)
+    const uint8_t rows[2] = {(uint8_t)((m[0] ? 1 : 0) | (m[2] ? 2 : 0)),
+                             (uint8_t)((m[1] ? 1 : 0) | (m[3] ? 2 : 0))};
+    Decomp d = decompose(rows, 2);
+    if (d.len < 0) {
+      // singular map (not a Clifford, but the reference applies it anyway)
+      const uint32_t code = (rows[0] & 1) | ((rows[0] & 2)) | ((rows[1] & 1) << 2) |
+                            ((rows[1] & 2) << 2);
+      emit(I_LIN1, xs[q], zs[q], code);
+      P->bitops += 2;
+      return QCF_OK;
+    }
+    uint32_t slot[2] = {xs[q], zs[q]};
+    for (int k = 0; k < d.len; ++k) emit(I_XOR, slot[d.ops[k][0]], slot[d.ops[k][1]]);
+    uint32_t ns[2];
+    for (int i = 0; i < 2; ++i) ns[d.perm[i]] = slot[i];
+    xs[q] = ns[0];
+    zs[q] = ns[1];
+    for (int i = 0; i < 2; ++i)
+      if (rows[i] != (1u << i)) P->bitops += 1;
+    return QCF_OK;
+  }
+
+  // ---- classical bits
+  void write_bit(uint32_t col, uint32_t reads_after, uint32_t from_plane,
+                 bool from_acc) {
+    // value is in plane `from_plane` (measurement) or in acc (XOR)
+    if (bit_slot[col] != kNone) {  // previous value no longer needed
+      free_slots.push_back(bit_slot[col]);
+      bit_slot[col] = kNone;
+    }
+    uint32_t keep = kNone;
+    if (reads_after > 0) {
+      keep = alloc_slot();
+      bit_slot[col] = keep;
+      reads_left[col] = reads_after;
+    }
+    if (from_acc)
+      emit(I_ACC_STORE, col, keep);
+    else
+      emit(I_MEAS, from_plane, col, keep);
+    written[col] = 1;
+  }
+
+  void measure(uint32_t q, uint32_t bit, uint32_t reads_after) {
+    if (bit == 0) return;
+    write_bit(bit - 1, reads_after, xs[q], false);
+    P->bitops += 1;
+  }
+
+  void reset(uint32_t q) {
+    emit(I_RESETZ, xs[q], zs[q], rs++);
+    P->bitops += 2;
+  }
+
+  // ---- noise
+  int noise1(uint32_t q, double c1, double c2, double c3, uint32_t bitops) {
+    auto clamp = [](double v) { return std::isnan(v) ? 0.0 : std::min(1.0, std::max(0.0, v)); };
+    c1 = clamp(c1);
+    c2 = std::max(c1, clamp(c2));
+    c3 = std::max(c2, clamp(c3));
+    Site s{};
+    s.kind = 1;
+    s.rslot = rs;
+    rs += 2;
+    if (c3 > 0) {
+      const double two32 = 4294967296.0;
+      s.t1 = (uint64_t)std::floor(c1 / c3 * two32);
+      s.t2 = (uint64_t)std::floor(c2 / c3 * two32);
+      if (c1 >= c3) s.t1 = (uint64_t)1 << 32;
+      if (c2 >= c3) s.t2 = (uint64_t)1 << 32;
+    }
+    s.bitops = bitops;
+    const uint32_t id = (uint32_t)P->sites.size();
+    P->sites.push_back(s);
+    site_pe.push_back(c3);
+    emit(I_NOISE1, xs[q], zs[q], id);
+    P->bitops += bitops;
+    return QCF_OK;
+  }
+
+  int noise2(uint32_t q1, uint32_t q2, double e) {
+    e = std::isnan(e) ? 0.0 : std::min(1.0, std::max(0.0, e));
+    Site s{};
+    s.kind = 2;
+    s.rslot = rs;
+    rs += 4;
+    s.bitops = 4;
+    const uint32_t id = (uint32_t)P->sites.size();
+    P->sites.push_back(s);
+    site_pe.push_back(e);
+    emit(I_NOISE2, xs[q1] | (zs[q1] << 16), xs[q2] | (zs[q2] << 16), id);
+    P->bitops += 4;
+    return QCF_OK;
+  }
+
+  int assign_classes() {
+    std::vector<double> distinct;
+    for (double p : site_pe)
+      if (p > 0) distinct.push_back(p);
+    std::sort(distinct.begin(), distinct.end());
+    distinct.erase(std::unique(distinct.begin(), distinct.end()), distinct.end());
+    std::vector<int> bucket(distinct.size());
+    std::vector<double> cls_p;
+    if (distinct.size() <= kMaxClasses) {
+      for (size_t i = 0; i < distinct.size(); ++i) {
+        bucket[i] = (int)i;
+        cls_p.push_back(distinct[i]);
+      }
+    } else {
+      // equal-width buckets in log(p); each class runs at its largest rate
+      const double lo = std::log(distinct.front()), hi = std::log(distinct.back());
+      const double w = (hi - lo) / kMaxClasses;
+      std::vector<int> raw(distinct.size());
+      for (size_t i = 0; i < distinct.size(); ++i) {
+        int b = w > 0 ? (int)((std::log(distinct[i]) - lo) / w) : 0;
+        raw[i] = std::min<int>(b, kMaxClasses - 1);
+      }
+      int cur = -1, prev = -1;
+      for (size_t i = 0; i < distinct.size(); ++i) {
+        if (raw[i] != prev) {
+          ++cur;
+          prev = raw[i];
+          cls_p.push_back(0);
+        }
+        bucket[i] = cur;
+        cls_p[cur] = std::max(cls_p[cur], distinct[i]);
+      }
+    }
+    P->classes.clear();
+    for (double p : cls_p) {
+      ClassInfo ci;
+      ci.p = p;
+      const double l2 = std::log1p(-p) / std::log(2.0);  // log2(1-p) <= 0
+      ci.c = (float)(1.0 / l2);
+      P->classes.push_back(ci);
+    }
+    std::vector<uint32_t> ord(cls_p.size(), 0);
+    for (size_t i = 0; i < P->sites.size(); ++i) {
+      Site &s = P->sites[i];
+      const double pe = site_pe[i];
+      if (!(pe > 0)) {
+        s.cls = kNoFire;
+        continue;
+      }
+      const size_t k = std::lower_bound(distinct.begin(), distinct.end(), pe) - distinct.begin();
+      const int c = bucket[k];
+      s.cls = (uint32_t)c;
+      s.ordinal = ord[c]++;
+      const double r = pe / cls_p[c];
+      s.accept = r >= 1.0 ? (1u << 31) : (uint32_t)std::llround(r * 2147483648.0);
+    }
+    for (uint32_t n : ord)
+      if (n > kMaxOrdinals)
+        return set_error(QCF_ERR_UNSUPPORTED,
+                         "pftrajectories: %u noise sites share one rate class (limit %u)",
+                         n, kMaxOrdinals);
+    return QCF_OK;
+  }
+
+  // peephole: fuse XOR pairs whose second source is not the first target
+  void fuse_xors() {
+    std::vector<Instr> out;
+    out.reserve(P->body.size());
+    for (size_t i = 0; i < P->body.size(); ++i) {
+      const Instr &a = P->body[i];
+      if (a.op == I_XOR && i + 1 < P->body.size() && P->body[i + 1].op == I_XOR) {
+        const Instr &b = P->body[i + 1];
+        if (b.b != a.a && a.a < 0x10000 && b.a < 0x10000 && b.b < 0x10000) {
+          out.push_back(Instr{I_XOR2, a.a, a.b, b.a | (b.b << 16)});
+          ++i;
+          continue;
+        }
+      }
+      out.push_back(a);
+    }
+    P->body.swap(out);
+  }
+};
+
+int op_qubit_check(const qcf_op &op, int nqargs, uint32_t nq) {
+  if (op.q1 == 0 || (nqargs == 2 && op.q2 == 0))
+    return set_error(QCF_ERR_ARGUMENT,
+                     "Qubit indices have to be larger than zero (op kind %u)", op.kind);
+  if (nqargs == 2 && op.q1 == op.q2)
+    return set_error(QCF_ERR_ARGUMENT,
+                     "Failed to create a two qubit gate because the two qubits it "
+                     "acts on have the same index (%u).", op.q1);
+  const uint32_t qmax = nqargs == 2 ? std::max(op.q1, op.q2) : op.q1;
+  if (qmax > nq)
+    return set_error(QCF_ERR_BOUNDS,
+                     "qubit %u is outside the frame of %u qubits (incl. ancilla)", qmax, nq);
+  return QCF_OK;
+}
+
+bool is_gate(uint32_t kind) { return find_gate(kind) != nullptr; }
+
+}  // namespace
+}  // namespace qcf
+
+using namespace qcf;
+
+qcf_program::~qcf_program() {
+  for (auto &d : dev) {
+    if (d.device < 0) continue;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(d.device);
+    for (void *p : d.prog)
+      if (p) (void)hipFree(p);
+    if (d.sites) (void)hipFree(d.sites);
+    (void)hipSetDevice(cur);
+  }
+}
+
+extern "C" int qcf_program_build(const qcf_op *ops, size_t nops,
+                                 const uint32_t *args, size_t nargs,
+                                 uint32_t nqubits, uint32_t nbits,
+                                 qcf_program **out) {
+  clear_error();
+  if (!out) return set_error(QCF_ERR_ARGUMENT, "null output pointer");
+  *out = nullptr;
+  if (nops && !ops) return set_error(QCF_ERR_ARGUMENT, "null op array");
+  // ---- size the frame like _create_pauliframe (src/pauli_frames.jl:197-201)
+  uint32_t qmax = 1, bmax = 1;
+  for (size_t i = 0; i < nops; ++i) {
+    const qcf_op &op = ops[i];
+    const GateMap *g = find_gate(op.kind);
+    if (g) {
+      qmax = std::max(qmax, op.q1);
+      if (g->nq == 2) qmax = std::max(qmax, op.q2);
+    } else {
+      switch (op.kind) {
+        case QCF_OP_SINGLE_QUBIT_OPERATOR:
+        case QCF_OP_PAULI_ERROR:
+        case QCF_OP_PAULI_NOISE:
+          qmax = std::max(qmax, op.q1);
+          break;
+        case QCF_OP_DEPOLARIZE2:
+          qmax = std::max(qmax, std::max(op.q1, op.q2));
+          break;
+        case QCF_OP_MZ: case QCF_OP_MRZ: case QCF_OP_MX: case QCF_OP_MRX:
+          qmax = std::max(qmax, op.q1);
+          bmax = std::max(bmax, op.bit);
+          break;
+        case QCF_OP_PAULI_MEASUREMENT:
+          qmax = std::max(qmax, op.arg_len ? op.arg_len - 1 : 0);
+          bmax = std::max(bmax, op.bit);
+          break;
+        case QCF_OP_CLASSICAL_XOR:
+          bmax = std::max(bmax, op.bit);
+          if (op.arg_off + (size_t)op.arg_len > nargs)
+            return set_error(QCF_ERR_ARGUMENT, "ClassicalXOR arguments out of range");
+          for (uint32_t k = 0; k < op.arg_len; ++k) bmax = std::max(bmax, args[op.arg_off + k]);
+          break;
+        case QCF_OP_NOP:
+          break;
+        default:
+          return set_error(QCF_ERR_UNSUPPORTED,
+                           "no PauliFrame method for operation kind %u", op.kind);
+      }
+    }
+  }
+  const uint32_t n = nqubits ? nqubits : qmax;
+  const uint32_t B = nbits ? nbits : bmax;
+  const uint32_t nq = n + 1;  // hidden ancilla (src/pauli_frames.jl:30-37)
+  if (nq > 0x7FFF)
+    return set_error(QCF_ERR_UNSUPPORTED, "frame of %u qubits exceeds the plane index range", nq);
+
+  qcf_program *P = new (std::nothrow) qcf_program();
+  if (!P) return set_error(QCF_ERR_ALLOC, "out of host memory");
+  P->nq = nq;
+  P->nbits = B;
+  Compiler C(P, nq, B);
+
+  // ---- classical liveness: reads of each written value before it is
+  //      overwritten (ClassicalXOR reads precede its own store)
+  std::vector<uint32_t> reads_after_write(nops, 0), pending(B, 0);
+  auto write_bit_of = [&](const qcf_op &op) -> uint32_t {
+    switch (op.kind) {
+      case QCF_OP_MZ: case QCF_OP_MX: case QCF_OP_MRZ: case QCF_OP_MRX:
+      case QCF_OP_PAULI_MEASUREMENT: case QCF_OP_CLASSICAL_XOR:
+        return op.bit;
+      default:
+        return 0;
+    }
+  };
+  for (size_t i = 0; i < nops; ++i) {
+    const qcf_op &op = ops[i];
+    const uint32_t w = write_bit_of(op);
+    if (w > B) {
+      delete P;
+      return set_error(QCF_ERR_BOUNDS, "classical bit %u outside the record of %u bits", w, B);
+    }
+    if (op.kind == QCF_OP_CLASSICAL_XOR) {
+      if (op.arg_len == 0) {
+        delete P;
+        return set_error(QCF_ERR_BOUNDS, "ClassicalXOR needs at least one input bit");
+      }
+      if (op.bit == 0) {
+        delete P;
+        return set_error(QCF_ERR_BOUNDS, "ClassicalXOR store bit must be >= 1");
+      }
+      for (uint32_t k = 0; k < op.arg_len; ++k) {
+        const uint32_t b = args[op.arg_off + k];
+        if (b == 0 || b > B) {
+          delete P;
+          return set_error(QCF_ERR_BOUNDS, "ClassicalXOR input bit %u outside 1..%u", b, B);
+        }
+      }
+    }
+  }
+  for (size_t i = nops; i-- > 0;) {
+    const qcf_op &op = ops[i];
+    const uint32_t w = write_bit_of(op);
+    if (w) {
+      reads_after_write[i] = pending[w - 1];
+      pending[w - 1] = 0;
+    }
+    if (op.kind == QCF_OP_CLASSICAL_XOR)
+      for (uint32_t k = 0; k < op.arg_len; ++k) pending[args[op.arg_off + k] - 1]++;
+  }
+
+  // ---- lower every operation
+  for (size_t i = 0; i < nops; ++i) {
+    const qcf_op &op = ops[i];
+    int st = QCF_OK;
+    if (const GateMap *g = find_gate(op.kind)) {
+      st = op_qubit_check(op, g->nq, nq);
+      if (st == QCF_OK) C.gate(*g, op.q1 - 1, g->nq == 2 ? op.q2 - 1 : 0);
+    } else {
+      switch (op.kind) {
+        case QCF_OP_NOP:
+          break;
+        case QCF_OP_SINGLE_QUBIT_OPERATOR:
+          st = op_qubit_check(op, 1, nq);
+          if (st == QCF_OK && (op.arg_len < 4 || op.arg_off + (size_t)4 > nargs))
+            st = set_error(QCF_ERR_ARGUMENT, "SingleQubitOperator needs 4 matrix entries");
+          if (st == QCF_OK) st = C.single_qubit_operator(op.q1 - 1, args + op.arg_off);
+          break;
+        case QCF_OP_MZ:
+          st = op_qubit_check(op, 1, nq);
+          if (st == QCF_OK) C.measure(op.q1 - 1, op.bit, reads_after_write[i]);
+          break;
+        case QCF_OP_MX:
+          st = op_qubit_check(op, 1, nq);
+          if (st == QCF_OK && op.bit != 0) {
+            C.gate(*find_gate(QCF_OP_HADAMARD), op.q1 - 1, 0);
+            C.measure(op.q1 - 1, op.bit, reads_after_write[i]);
+          }
+          break;
+        case QCF_OP_MRZ:
+        case QCF_OP_MRX:
+          st = op_qubit_check(op, 1, nq);
+          if (st == QCF_OK) {
+            if (op.kind == QCF_OP_MRX) C.gate(*find_gate(QCF_OP_HADAMARD), op.q1 - 1, 0);
+            C.measure(op.q1 - 1, op.bit, reads_after_write[i]);
+            C.reset(op.q1 - 1);
+          }
+          break;
+        case QCF_OP_PAULI_MEASUREMENT: {
+          // args: one code per qubit 1..len (0=I 1=X 2=Z 3=Y), then the phase
+          if (op.arg_len < 1 || op.arg_off + (size_t)op.arg_len > nargs) {
+            st = set_error(QCF_ERR_ARGUMENT, "PauliMeasurement arguments out of range");
+            break;
+          }
+          const uint32_t len = op.arg_len - 1;
+          const uint32_t anc = len + 1;  // nqubits(op.pauli) + 1
+          if (anc > nq) {
+            st = set_error(QCF_ERR_BOUNDS, "PauliMeasurement ancilla %u outside the frame of %u qubits", anc, nq);
+            break;
+          }
+          for (uint32_t q = 1; q <= len; ++q) {
+            const uint32_t code = args[op.arg_off + q - 1] & 3;
+            if (code == 1) C.gate(*find_gate(QCF_OP_XCZ), q - 1, anc - 1);
+            else if (code == 2) C.gate(*find_gate(QCF_OP_CNOT), q - 1, anc - 1);
+            else if (code == 3) C.gate(*find_gate(QCF_OP_YCX), q - 1, anc - 1);
+          }
+          // a nonzero phase applies sX to the ancilla: no effect on frame bits
+          C.measure(anc - 1, op.bit, reads_after_write[i]);
+          C.reset(anc - 1);
+          break;
+        }
+        case QCF_OP_PAULI_ERROR:
+          st = op_qubit_check(op, 1, nq);
+          if (st == QCF_OK) {
+            const double t = op.p[0] / 3;
+            st = C.noise1(op.q1 - 1, t, 2 * t, 3 * t, 2);
+          }
+          break;
+        case QCF_OP_PAULI_NOISE:
+          st = op_qubit_check(op, 1, nq);
+          if (st == QCF_OK) {
+            const double px = op.p[0], py = op.p[1], pz = op.p[2];
+            const uint32_t w = (px > 0 || py > 0 ? 1 : 0) + (pz > 0 || py > 0 ? 1 : 0);
+            st = C.noise1(op.q1 - 1, px, px + pz, px + pz + py, w);
+          }
+          break;
+        case QCF_OP_DEPOLARIZE2:
+          st = op_qubit_check(op, 2, nq);
+          if (st == QCF_OK) st = C.noise2(op.q1 - 1, op.q2 - 1, op.p[0]);
+          break;
+        case QCF_OP_CLASSICAL_XOR: {
+          for (uint32_t k = 0; k < op.arg_len; ++k) {
+            const uint32_t col = args[op.arg_off + k] - 1;
+            const uint32_t slot = C.bit_slot[col];
+            if (slot != kNone) {
+              C.emit(k == 0 ? I_ACC_LDS : I_ACCX_LDS, slot);
+              if (--C.reads_left[col] == 0) {
+                C.free_slots.push_back(slot);
+                C.bit_slot[col] = kNone;
+              }
+            } else {
+              C.emit(k == 0 ? I_ACC_REC : I_ACCX_REC, col);
+              if (!C.written[col]) C.zero_first[col] = 1;
+            }
+          }
+          C.write_bit(op.bit - 1, reads_after_write[i], 0, true);
+          C.P->bitops += op.arg_len - 1;
+          break;
+        }
+        default:
+          st = set_error(QCF_ERR_UNSUPPORTED, "no PauliFrame method for operation kind %u", op.kind);
+      }
+    }
+    if (st != QCF_OK) {
+      delete P;
+      return st;
+    }
+  }
+  C.fuse_xors();
+  if (int st = C.assign_classes(); st != QCF_OK) {
+    delete P;
+    return st;
+  }
+  P->nplanes = C.next_slot;
+  if (P->nplanes > 0xFFFF) {
+    delete P;
+    return set_error(QCF_ERR_UNSUPPORTED, "circuit needs %u on-chip planes", P->nplanes);
+  }
+  P->nbody_rs = C.rs;
+  P->ninit_rs = nq;
+  // ---- prologues / epilogue
+  for (uint32_t q = 0; q < nq; ++q)
+    P->prologue_fresh.push_back(Instr{I_INIT, 2 * q, 2 * q + 1, q});
+  for (uint32_t b = 0; b < B; ++b)
+    if (!C.written[b] || C.zero_first[b]) P->prologue_fresh.push_back(Instr{I_REC_ZERO, b, 0, 0});
+  for (uint32_t q = 0; q < nq; ++q) {
+    P->prologue_inplace.push_back(Instr{I_LOAD, 2 * q, q, 0});
+    P->prologue_inplace.push_back(Instr{I_LOAD, 2 * q + 1, nq + q, 0});
+    P->epilogue_store.push_back(Instr{I_STORE, C.xs[q], q, 0});
+    P->epilogue_store.push_back(Instr{I_STORE, C.zs[q], nq + q, 0});
+  }
+  *out = P;
+  return QCF_OK;
+}
+
+extern "C" int qcf_program_free(qcf_program *prog) {
+  delete prog;
+  return QCF_OK;
+}
+
+extern "C" int qcf_program_get_info(const qcf_program *P, qcf_program_info *info) {
+  clear_error();
+  if (!P || !info) return set_error(QCF_ERR_ARGUMENT, "null pointer");
+  info->nqubits = P->nq;
+  info->nbits = P->nbits;
+  info->nplanes = P->nplanes;
+  info->ninstr = (uint32_t)P->body.size();
+  info->nsites = (uint32_t)P->sites.size();
+  info->nclasses = (uint32_t)P->classes.size();
+  info->nreplay = P->ninit_rs + P->nbody_rs;
+  info->lds_bytes = P->nplanes * 64u * 4u;
+  info->bitops_per_shot = P->bitops;
+  return QCF_OK;
+}
+
+extern "C" double qcf_program_bitops(const qcf_program *P) { return P ? P->bitops : 0.0; }

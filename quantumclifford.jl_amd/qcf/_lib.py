@@ -1,0 +1,126 @@
+"""ctypes binding of libqcf.so (the C ABI declared in include/qcf.h).
+
+The library is the only compute path: if it is missing or fails to load, every
+entry point raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libqcf.so")
+
+
+class QCFError(RuntimeError):
+    """Base class of errors reported by libqcf."""
+
+
+class DimensionMismatch(QCFError, ValueError):
+    """Mirror of Julia's DimensionMismatch (src/mul_leftright.jl:138)."""
+
+
+class BoundsError(QCFError, IndexError):
+    """Mirror of Julia's BoundsError."""
+
+
+class ArgumentError(QCFError, ValueError):
+    """Mirror of Julia's ArgumentError / NoZeroQubit (src/throws.jl:1)."""
+
+
+class MethodError(QCFError, TypeError):
+    """Mirror of Julia's MethodError: no PauliFrame method for an operation."""
+
+
+class HIPError(QCFError):
+    """HIP runtime failure."""
+
+
+_ERRORS = {1: DimensionMismatch, 2: BoundsError, 3: ArgumentError, 4: MethodError,
+           5: HIPError, 6: MemoryError}
+
+# qcf_op (include/qcf.h)
+OP_DTYPE = np.dtype([("kind", "<u4"), ("q1", "<u4"), ("q2", "<u4"), ("bit", "<u4"),
+                     ("arg_off", "<u4"), ("arg_len", "<u4"), ("p", "<f8", (3,))],
+                    align=True)
+assert OP_DTYPE.itemsize == 48
+
+
+class ProgramInfo(C.Structure):
+    _fields_ = [("nqubits", C.c_uint32), ("nbits", C.c_uint32), ("nplanes", C.c_uint32),
+                ("ninstr", C.c_uint32), ("nsites", C.c_uint32), ("nclasses", C.c_uint32),
+                ("nreplay", C.c_uint32), ("lds_bytes", C.c_uint32),
+                ("bitops_per_shot", C.c_double)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("waves_per_cu", C.c_uint32), ("tile_blocks", C.c_uint32)]
+
+
+_P = C.c_void_p
+_SZ = C.c_size_t
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+_I = C.c_int
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "qcf_last_error": (C.c_char_p, []),
+    "qcf_version": (C.c_char_p, []),
+    "qcf_mul_left": (_I, [_P, _P, _P, _P, _SZ, _I, _P]),
+    "qcf_mul_right": (_I, [_P, _P, _P, _P, _SZ, _I, _P]),
+    "qcf_mul_vec": (_I, [_P, _P, _SZ, _I, _I, _P, _P]),
+    "qcf_mul_rows": (_I, [_P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ, _P, _I, _I, _P]),
+    "qcf_program_build": (_I, [_P, _SZ, _P, _SZ, _U32, _U32, C.POINTER(_P)]),
+    "qcf_program_free": (_I, [_P]),
+    "qcf_program_get_info": (_I, [_P, C.POINTER(ProgramInfo)]),
+    "qcf_pf_sample": (_I, [_P, _U64, _U64, _P, _P, _P, _SZ, _U32, _P]),
+    "qcf_batch_create": (_I, [_P, _SZ, _P, _P, _P, _P, C.POINTER(Tune), C.POINTER(_P)]),
+    "qcf_batch_launch": (_I, [_P, _P]),
+    "qcf_batch_launch_graph": (_I, [_P, _P]),
+    "qcf_batch_free": (_I, [_P]),
+    "qcf_pf_sample_batched": (_I, [_P, _SZ, _P, _P, _P, _P, C.POINTER(Tune), _P]),
+    "qcf_autotune": (_I, [_P, _SZ, _P, C.POINTER(Tune), _P]),
+    "qcf_debug_philox": (_I, [_P, _P, _SZ, _P]),
+    "qcf_record_to_bool": (_I, [_P, _SZ, _U64, _U32, _P, _P]),
+    "qcf_record_from_bool": (_I, [_P, _U64, _U32, _P, _SZ, _P]),
+    "qcf_frame_to_reference": (_I, [_P, _SZ, _U64, _U32, _P, _P]),
+    "qcf_frame_from_reference": (_I, [_P, _U64, _U32, _P, _SZ, _P]),
+    "qcf_program_bitops": (C.c_double, [_P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libqcf.so once; raise if it is absent (there is no other path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with quantumclifford.jl_amd/build.py "
+                "(there is no non-HIP fallback)")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def check(status: int) -> None:
+    if status == 0:
+        return
+    msg = lib().qcf_last_error().decode(errors="replace")
+    raise _ERRORS.get(status, QCFError)(msg or f"qcf status {status}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args))
