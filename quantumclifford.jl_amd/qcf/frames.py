@@ -1,0 +1,234 @@
+"""Pauli frames on the GPU: PauliFrame, pftrajectories, measurements
+(src/pauli_frames.jl), backed by the libqcf frame sampler.
+
+Frames are shot-packed (64 shots per UInt64 word, plane-major): `frame` holds
+2*nq planes (x of qubits 1..nq, then z), `record` holds one plane per
+classical bit.  `measurements(f)` returns the reference's Matrix{Bool}
+(shots x bits) view.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import OP_DTYPE, ProgramInfo, call
+from .ops import Operation
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(0 if t is None else t.data_ptr())
+
+
+class _SeedSource:
+    """Stands in for Julia's global RNG: hands out a fresh 64-bit seed per call."""
+
+    def __init__(self, seed: int = 0x5EED):
+        self.rng = np.random.default_rng(seed)
+
+    def next(self) -> int:
+        return int(self.rng.integers(0, 2**63, dtype=np.int64))
+
+
+_SEEDS = _SeedSource()
+
+
+def seed_(seed: int) -> None:
+    """Reseed the source of per-call seeds (Random.seed! for the native sampler)."""
+    global _SEEDS
+    _SEEDS = _SeedSource(seed)
+
+
+def lower_circuit(circuit: Sequence[Operation], nqubits: int | None = None):
+    """Lower reference ops to (qcf_op records, args).  `nqubits` is the frame's
+    qubit count (needed by NoiseOpAll); by default the largest affected qubit."""
+    if nqubits is None:
+        nqubits = max([max(op.affectedqubits(), default=1) for op in circuit] + [1])
+    recs, args = [], []
+
+    def emit(kind, q1=0, q2=0, bit=0, arg_off=0, arg_len=0, p=(0.0, 0.0, 0.0)):
+        recs.append((kind, q1, q2, bit, arg_off, arg_len, p))
+
+    for op in circuit:
+        op.lower(emit, args, nqubits)
+    ops = np.zeros(len(recs), dtype=OP_DTYPE)
+    for i, r in enumerate(recs):
+        ops[i] = r
+    return ops, np.asarray(args, dtype=np.uint32)
+
+
+class Program:
+    """A compiled circuit (qcf_program_build)."""
+
+    def __init__(self, circuit: Sequence[Operation], nqubits: int = 0, nbits: int = 0):
+        ops, args = lower_circuit(circuit, nqubits or None)
+        self._ops, self._args = ops, args  # keep alive for the oracle
+        h = C.c_void_p()
+        call("qcf_program_build", ops.ctypes.data_as(C.c_void_p), len(ops),
+             args.ctypes.data_as(C.c_void_p) if len(args) else None, len(args),
+             int(nqubits), int(nbits), C.byref(h))
+        self.handle = h
+        info = ProgramInfo()
+        call("qcf_program_get_info", h, C.byref(info))
+        self.info = info
+
+    @property
+    def ops(self):
+        return self._ops
+
+    @property
+    def args(self):
+        return self._args
+
+    @property
+    def nq(self) -> int:
+        return self.info.nqubits
+
+    @property
+    def nbits(self) -> int:
+        return self.info.nbits
+
+    @property
+    def nreplay(self) -> int:
+        return self.info.nreplay
+
+    @property
+    def bitops_per_shot(self) -> float:
+        return self.info.bitops_per_shot
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h and h.value and _lib._lib is not None:
+            _lib._lib.qcf_program_free(h)
+            self.handle = None
+
+
+def words(shots: int) -> int:
+    return (shots + 63) // 64
+
+
+class PauliFrame:
+    """PauliFrame(frames, qubits, measurements) (src/pauli_frames.jl:9-37)."""
+
+    def __init__(self, frames: int, qubits: int, measurements: int, *, device=None,
+                 init: bool = True):
+        self.shots = int(frames)
+        self.nq = int(qubits) + 1  # hidden ancilla for PauliMeasurement
+        self.nbits = int(measurements)
+        self.ld = max(1, words(self.shots))
+        dev = torch.device(device) if device is not None else torch.device(
+            "cuda", torch.cuda.current_device())
+        self.frame = torch.zeros(2 * self.nq * self.ld, dtype=torch.int64, device=dev)
+        self.record = torch.zeros(max(1, self.nbits) * self.ld, dtype=torch.int64, device=dev)
+        if init:
+            initZ_(self)
+
+    def nqubits(self) -> int:
+        return self.nq - 1
+
+    def __len__(self) -> int:
+        return self.shots
+
+    def copy(self) -> "PauliFrame":
+        f = PauliFrame.__new__(PauliFrame)
+        f.shots, f.nq, f.nbits, f.ld = self.shots, self.nq, self.nbits, self.ld
+        f.frame = self.frame.clone()
+        f.record = self.record.clone()
+        return f
+
+    # host views for tests ---------------------------------------------
+    def frame_host(self) -> np.ndarray:
+        return self.frame.cpu().numpy().view(np.uint64).reshape(2 * self.nq, self.ld)
+
+    def record_host(self) -> np.ndarray:
+        return self.record.cpu().numpy().view(np.uint64).reshape(-1, self.ld)[: self.nbits]
+
+
+def initZ_(frame: PauliFrame, seed: int | None = None) -> PauliFrame:
+    """initZ!: x = 0, z uniformly random on every frame qubit (src/pauli_frames.jl:47-56)."""
+    prog = Program([], nqubits=frame.nq - 1, nbits=frame.nbits)
+    record = torch.empty_like(frame.record)  # initZ! leaves the record alone
+    call("qcf_pf_sample", prog.handle, frame.shots, _SEEDS.next() if seed is None else seed,
+         _ptr(frame.frame), _ptr(record), None, frame.ld, 1, _stream())
+    return frame
+
+
+def pftrajectories_state(state: PauliFrame, circuit, *, seed: int | None = None,
+                         masks: torch.Tensor | None = None) -> PauliFrame:
+    """pftrajectories(state::PauliFrame, circuit) (src/pauli_frames.jl:234-239)."""
+    prog = circuit if isinstance(circuit, Program) else Program(
+        circuit, nqubits=state.nq - 1, nbits=state.nbits)
+    if prog.nq != state.nq or prog.nbits > state.nbits:
+        raise _lib.BoundsError(
+            f"circuit needs {prog.nq} frame qubits / {prog.nbits} bits; frame has "
+            f"{state.nq} / {state.nbits}")
+    call("qcf_pf_sample", prog.handle, state.shots, _SEEDS.next() if seed is None else seed,
+         _ptr(state.frame), _ptr(state.record), _ptr(masks), state.ld, 0, _stream())
+    return state
+
+
+def pftrajectories(circuit, trajectories: int = 5000, threads: bool = True, *,
+                   seed: int | None = None, masks: torch.Tensor | None = None,
+                   export_masks: torch.Tensor | None = None, keep_frame: bool = True,
+                   device=None) -> PauliFrame:
+    """pftrajectories(circuit; trajectories, threads) (src/pauli_frames.jl:193-227):
+    a fresh frame sized from the circuit, initZ!, then the circuit.  `threads` is
+    accepted for signature compatibility (the GPU runs all shots in parallel).
+    `masks` replays pre-drawn randomness; `export_masks` records the native draws."""
+    if isinstance(circuit, PauliFrame):
+        raise TypeError("use pftrajectories_state(state, circuit)")
+    prog = circuit if isinstance(circuit, Program) else Program(circuit)
+    f = PauliFrame(trajectories, prog.nq - 1, prog.nbits, device=device, init=False)
+    if not keep_frame:
+        f.frame = torch.empty(0, dtype=torch.int64, device=f.record.device)
+    flags = 1
+    mptr = masks
+    if export_masks is not None:
+        flags |= 2
+        mptr = export_masks
+    call("qcf_pf_sample", prog.handle, f.shots, _SEEDS.next() if seed is None else seed,
+         _ptr(f.frame) if keep_frame else None, _ptr(f.record), _ptr(mptr), f.ld, flags,
+         _stream())
+    return f
+
+
+def alloc_masks(prog: Program, shots: int, device=None) -> torch.Tensor:
+    dev = torch.device(device) if device is not None else torch.device(
+        "cuda", torch.cuda.current_device())
+    return torch.zeros(max(1, prog.nreplay) * words(shots), dtype=torch.int64, device=dev)
+
+
+def measurements(frame: PauliFrame) -> torch.Tensor:
+    """measurements(frame): shots x bits Bool matrix, column-major like the
+    reference's Matrix{Bool} (src/pauli_frames.jl:280)."""
+    out = torch.empty(max(1, frame.nbits) * max(1, frame.shots), dtype=torch.uint8,
+                      device=frame.record.device)
+    call("qcf_record_to_bool", _ptr(frame.record), frame.ld, frame.shots, frame.nbits,
+         _ptr(out), _stream())
+    m = out[: frame.nbits * frame.shots].view(frame.nbits, frame.shots).t()
+    return m.bool()
+
+
+pfmeasurements = measurements  # deprecated alias (src/pauli_frames.jl:291-293)
+
+
+def frame_to_reference(frame: PauliFrame) -> torch.Tensor:
+    """The reference `fastcolumn` xzs: shots x 2W UInt64, returned as (2W, shots)."""
+    W = (frame.nq + 63) // 64
+    out = torch.empty(2 * W * max(1, frame.shots), dtype=torch.int64, device=frame.frame.device)
+    call("qcf_frame_to_reference", _ptr(frame.frame), frame.ld, frame.shots, frame.nq,
+         _ptr(out), _stream())
+    return out.view(2 * W, frame.shots)
+
+
+def frame_from_reference(xzs: torch.Tensor, frame: PauliFrame) -> PauliFrame:
+    call("qcf_frame_from_reference", _ptr(xzs.contiguous()), frame.shots, frame.nq,
+         _ptr(frame.frame), frame.ld, _stream())
+    return frame

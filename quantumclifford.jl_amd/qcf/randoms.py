@@ -1,0 +1,56 @@
+"""Host-side generators for benchmark and test inputs (src/randoms.jl semantics
+for random Paulis; a random-Clifford stabilizer tableau for the row sweep)."""
+from __future__ import annotations
+
+import numpy as np
+
+
+def random_pauli_words(rng: np.random.Generator, n: int):
+    """random_pauli(n) (src/randoms.jl:47-52, pad bits cleared as :13-18):
+    i.i.d. Bernoulli(1/2) x/z bits packed in UInt64 and a phase in {0,1,2,3}.
+    Returns (phase, xz words [x;z])."""
+    w = (n + 63) // 64
+    xz = rng.integers(0, 2**64, size=2 * w, dtype=np.uint64)
+    r = n % 64
+    if r:
+        m = np.uint64((1 << r) - 1)
+        xz[w - 1] &= m
+        xz[2 * w - 1] &= m
+    return int(rng.integers(4)), xz
+
+
+def random_clifford_tableau(rng: np.random.Generator, n: int, ngates: int | None = None):
+    """Stabilizer tableau of a random Clifford circuit applied to |0...0>:
+    start from rows Z_i (x = 0, z = e_i, phase 0) and conjugate every row by
+    `ngates` random H / S / CNOT gates with the reference's phase rules
+    (src/symbolic_cliffords.jl:105-118, 359-395).  Returns (phases uint8[n],
+    xzs uint64[n, 2W]) row-contiguous."""
+    w = (n + 63) // 64
+    ngates = 4 * n if ngates is None else ngates
+    x = np.zeros((n, n), np.uint8)
+    z = np.eye(n, dtype=np.uint8)
+    ph = np.zeros(n, np.uint8)
+    kinds = rng.integers(0, 3, ngates)
+    qa = rng.integers(0, n, ngates)
+    qb = rng.integers(0, n - 1, ngates) if n > 1 else np.zeros(ngates, np.int64)
+    for k, a, b in zip(kinds, qa, qb):
+        if k == 0:  # H: (z, x), phase if x&z
+            ph = (ph + 2 * (x[:, a] & z[:, a])) & 3
+            x[:, a], z[:, a] = z[:, a].copy(), x[:, a].copy()
+        elif k == 1:  # S: (x, x^z), phase if x&z
+            ph = (ph + 2 * (x[:, a] & z[:, a])) & 3
+            z[:, a] ^= x[:, a]
+        elif n > 1:  # CNOT(a, b)
+            b = b + (b >= a)
+            x1, z1, x2, z2 = x[:, a], z[:, a], x[:, b], z[:, b]
+            flip = (x1 & z1 & x2 & z2) | (x1 & z2 & (1 - (z1 | x2)))
+            ph = (ph + 2 * flip) & 3
+            z[:, a] = z1 ^ z2
+            x[:, b] = x2 ^ x1
+    xs = np.zeros((n, w * 64), np.uint8)
+    zs = np.zeros((n, w * 64), np.uint8)
+    xs[:, :n] = x
+    zs[:, :n] = z
+    px = np.packbits(xs, axis=1, bitorder="little").view(np.uint64)
+    pz = np.packbits(zs, axis=1, bitorder="little").view(np.uint64)
+    return ph.astype(np.uint8), np.ascontiguousarray(np.concatenate([px, pz], axis=1))

@@ -1,0 +1,241 @@
+"""Frame sampler on the GPU against the oracle.
+
+Bit-exact: noiseless circuits from the same initial frames (the pattern of
+test/test_gpu.jl:70-90), and noisy circuits under replayed masks -- both
+random masks and the masks the native RNG itself drew (exported).
+Statistical: closed-form rates (test/test_gpu.jl:116-134)."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import qcf
+from qcf import circuits
+
+pytestmark = pytest.mark.gpu
+
+SHOT_COUNTS = [1, 31, 32, 33, 100, 2047, 2048, 2049, 4096 + 5]
+
+
+def gpu_inplace(circ, nqubits, nbits, shots, frame_bits, meas_bits, masks_bits=None):
+    """Run pftrajectories(state, circuit) on the GPU from given frame bits."""
+    f = qcf.PauliFrame(shots, nqubits, nbits, init=False)
+    f.frame.copy_(torch.from_numpy(oracle.pack_shots(frame_bits).view(np.int64).reshape(-1)))
+    f.record.copy_(torch.from_numpy(oracle.pack_shots(meas_bits).view(np.int64).reshape(-1)))
+    masks = None
+    if masks_bits is not None:
+        masks = torch.from_numpy(oracle.pack_shots(masks_bits).view(np.int64).reshape(-1)).cuda()
+    qcf.pftrajectories_state(f, circ, masks=masks)
+    return (oracle.unpack_shots(f.frame_host(), shots),
+            oracle.unpack_shots(f.record_host(), shots))
+
+
+def oracle_inplace(circ, nqubits, nbits, shots, frame_bits, meas_bits, masks_bits=None):
+    nq = nqubits + 1
+    ops, args = qcf.lower_circuit(circ, nqubits)
+    xzs = oracle.frame_to_reference(oracle.pack_shots(frame_bits), nq, shots)
+    masks = None if masks_bits is None else oracle.pack_shots(masks_bits)
+    xzs, meas = oracle.pf_run(ops, args, nq, nbits, shots, masks=masks, fresh=False, xzs=xzs,
+                              meas=meas_bits.copy())
+    return oracle.unpack_shots(oracle.reference_to_frame(xzs, nq, shots), shots), meas
+
+
+def test_device_philox_matches_known_answers(gpu):
+    recs = np.array([[0, 0, 0, 0, 0, 0], [0xFFFFFFFF] * 6,
+                     [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344, 0xA4093822, 0x299F31D0]],
+                    np.uint32)
+    rng = np.random.default_rng(0)
+    recs = np.concatenate([recs, rng.integers(0, 2**32, (100, 6), dtype=np.uint32)])
+    inp = torch.from_numpy(recs.view(np.int32).reshape(-1)).cuda()
+    out = torch.zeros(4 * len(recs), dtype=torch.int32, device="cuda")
+    qcf._lib.call("qcf_debug_philox", qcf.pauli._ptr(inp), qcf.pauli._ptr(out), len(recs),
+                  qcf.pauli._stream())
+    got = out.cpu().numpy().view(np.uint32).reshape(-1, 4)
+    assert got[0].tolist() == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert got[1].tolist() == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert got[2].tolist() == [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+    for r, g in zip(recs, got):
+        assert g.tolist() == oracle.philox4x32_10(r[:4], r[4:]).tolist()
+
+
+GATES_1Q = qcf.ops.SINGLE_QUBIT_GATES
+GATES_2Q = qcf.ops.TWO_QUBIT_GATES
+
+
+@pytest.mark.parametrize("shots", [100, 2049])
+def test_every_gate_from_random_frames(gpu, shots):
+    n = 4
+    rng = np.random.default_rng(shots)
+    circ = []
+    for g in GATES_1Q:
+        circ.append(g(int(rng.integers(1, n + 1))))
+    for g in GATES_2Q:
+        a, b = rng.choice(np.arange(1, n + 1), 2, replace=False)
+        circ.append(g(int(a), int(b)))
+    circ.append(qcf.SingleQubitOperator(2, True, True, False, True))
+    circ.append(qcf.SingleQubitOperator(3, False, True, True, False))
+    circ.append(qcf.SingleQubitOperator(1, True, True, True, True))  # singular map
+    for q in range(1, n + 1):
+        circ.append(qcf.sMZ(q, q))
+    fb = rng.integers(0, 2, (2 * (n + 1), shots)).astype(np.uint8)
+    mb = rng.integers(0, 2, (n, shots)).astype(np.uint8)
+    gf, gm = gpu_inplace(circ, n, n, shots, fb, mb)
+    of, om = oracle_inplace(circ, n, n, shots, fb, mb)
+    assert np.array_equal(gf, of)
+    assert np.array_equal(gm, om)
+
+
+def random_circuit(rng, n, nbits, length):
+    circ = []
+    written = []
+    for _ in range(length):
+        k = rng.integers(0, 10)
+        q = int(rng.integers(1, n + 1))
+        if k < 3:
+            g = GATES_1Q[rng.integers(len(GATES_1Q))]
+            circ.append(g(q))
+        elif k < 6:
+            a, b = rng.choice(np.arange(1, n + 1), 2, replace=False)
+            g = GATES_2Q[rng.integers(len(GATES_2Q))]
+            circ.append(g(int(a), int(b)))
+        elif k == 6:
+            b = int(rng.integers(0, nbits + 1))
+            circ.append((qcf.sMZ, qcf.sMRZ, qcf.sMX, qcf.sMRX)[rng.integers(4)](q, b))
+            if b:
+                written.append(b)
+        elif k == 7:
+            circ.append(qcf.PauliError(q, float(rng.uniform(0, 0.3))))
+        elif k == 8:
+            if rng.random() < 0.5:
+                circ.append(qcf.PauliError(q, *rng.uniform(0, 0.1, 3)))
+            else:
+                a, b = rng.choice(np.arange(1, n + 1), 2, replace=False)
+                circ.append(qcf.Depolarize2(int(a), int(b), float(rng.uniform(0, 0.3))))
+        else:
+            ar = int(rng.integers(1, 5))
+            bits = [int(x) for x in rng.integers(1, nbits + 1, ar)]
+            circ.append(qcf.ClassicalXOR(bits, int(rng.integers(1, nbits + 1))))
+    return circ
+
+
+@pytest.mark.parametrize("shots", SHOT_COUNTS)
+def test_random_circuits_replayed_masks(gpu, shots):
+    """Noisy random circuits: the same pre-drawn masks replayed on both sides."""
+    rng = np.random.default_rng(77 + shots)
+    for trial in range(3):
+        n, nbits = int(rng.integers(2, 9)), int(rng.integers(1, 12))
+        circ = random_circuit(rng, n, nbits, 120)
+        prog = qcf.Program(circ, nqubits=n, nbits=nbits)
+        fb = rng.integers(0, 2, (2 * (n + 1), shots)).astype(np.uint8)
+        mb = rng.integers(0, 2, (nbits, shots)).astype(np.uint8)
+        masks = rng.integers(0, 2, (prog.nreplay, shots)).astype(np.uint8)
+        gf, gm = gpu_inplace(circ, n, nbits, shots, fb, mb, masks)
+        of, om = oracle_inplace(circ, n, nbits, shots, fb, mb, masks)
+        assert np.array_equal(gf, of), trial
+        assert np.array_equal(gm, om), trial
+
+
+def fresh_export_and_replay(circ, shots, seed):
+    """Native RNG on the GPU with every draw exported, then the oracle replays
+    the exported masks from a fresh PauliFrame: records must agree bit for bit."""
+    prog = qcf.Program(circ)
+    masks = qcf.alloc_masks(prog, shots)
+    f = qcf.pftrajectories(prog, shots, seed=seed, export_masks=masks)
+    ld = qcf.frames.words(shots)
+    mk = masks.cpu().numpy().view(np.uint64).reshape(-1, ld)[: prog.nreplay]
+    ops, args = prog.ops, prog.args
+    xzs, om = oracle.pf_run(ops, args, prog.nq, prog.nbits, shots, masks=mk)
+    gm = oracle.unpack_shots(f.record_host(), shots)
+    gf = oracle.unpack_shots(f.frame_host(), shots)
+    of = oracle.unpack_shots(oracle.reference_to_frame(xzs, prog.nq, shots), shots)
+    return prog, f, gm, om, gf, of, mk
+
+
+def test_repetition_code_native_seed0_bit_exact(gpu):
+    """Distance-3 repetition memory, 10^4 shots at seed 0: detector, observable and
+    raw record bits equal the oracle replaying the masks the GPU drew."""
+    c = circuits.repetition_code(3, 3, 1e-3)
+    prog, f, gm, om, gf, of, mk = fresh_export_and_replay(c.ops, 10**4, seed=0)
+    assert np.array_equal(gm, om[: prog.nbits])
+    assert np.array_equal(gf, of)
+    assert mk.any()  # noise and gauge draws happened
+
+
+@pytest.mark.parametrize("d,rounds,shots", [(3, 3, 5000), (5, 25, 4096 + 77)])
+def test_surface_code_native_export_replay(gpu, d, rounds, shots):
+    c = circuits.rotated_surface_code(d, rounds, 1e-2)
+    prog, f, gm, om, gf, of, mk = fresh_export_and_replay(c.ops, shots, seed=1234 + d)
+    assert np.array_equal(gm, om[: prog.nbits])
+    assert np.array_equal(gf, of)
+    # some detectors fire at p = 1e-2
+    assert gm[np.array(c.det_bits) - 1].any()
+
+
+def test_noiseless_surface_code_native(gpu):
+    c = circuits.rotated_surface_code(5, 25, 0.0, noise=False)
+    f = qcf.pftrajectories(c.ops, 200_000, seed=3)
+    rec = f.record_host()
+    assert not rec[np.array(c.det_bits) - 1].any()
+    assert not rec[np.array(c.obs_bits) - 1].any()
+    # first-round X-check outcomes are uniformly random (gauge bits)
+    first = oracle.unpack_shots(rec[:12], 200_000)
+    assert abs(first.mean() - 0.5) < 0.01
+
+
+def test_same_seed_same_bits_different_seed_differs(gpu):
+    c = circuits.rotated_surface_code(3, 3, 5e-3)
+    prog = qcf.Program(c.ops)
+    a = qcf.pftrajectories(prog, 50_000, seed=11).record_host()
+    b = qcf.pftrajectories(prog, 50_000, seed=11).record_host()
+    d = qcf.pftrajectories(prog, 50_000, seed=12).record_host()
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, d)
+
+
+def test_noise_rates_closed_form(gpu):
+    """test/test_gpu.jl:116-134: rates [0, .5, 2p/3, .5]."""
+    p = 0.1
+    shots = 10**5
+    circ = [qcf.sMZ(1, 1), qcf.sHadamard(2), qcf.sMZ(2, 2),
+            qcf.NoiseOp(qcf.UnbiasedUncorrelatedNoise(p), [3, 4]), qcf.sMZ(3, 3),
+            qcf.sHadamard(4), qcf.sMZ(4, 4)]
+    f = qcf.PauliFrame(shots, 4, 4)
+    qcf.pftrajectories_state(f, circ)
+    m = qcf.measurements(f).float().mean(dim=0).cpu().numpy()
+    assert np.allclose(m, [0, 0.5, 2 * p / 3, 0.5], atol=0.01)
+
+
+def test_measurements_matrix_layout(gpu):
+    c = circuits.repetition_code(3, 3, 0.05)
+    f = qcf.pftrajectories(c.ops, 3000, seed=5)
+    m = qcf.measurements(f)
+    assert m.shape == (3000, c.nbits) and m.dtype == torch.bool
+    bits = oracle.unpack_shots(f.record_host(), 3000)
+    assert np.array_equal(m.cpu().numpy().T.astype(np.uint8), bits)
+
+
+def test_reference_frame_layout_roundtrip(gpu):
+    c = circuits.rotated_surface_code(3, 2, 0.01)
+    f = qcf.pftrajectories(c.ops, 777, seed=9)
+    xzs = qcf.frame_to_reference(f)
+    want = oracle.frame_to_reference(f.frame_host(), f.nq, 777)
+    assert np.array_equal(xzs.cpu().numpy().view(np.uint64), want)
+    g = qcf.PauliFrame(777, f.nq - 1, f.nbits, init=False)
+    qcf.frame_from_reference(xzs, g)
+    assert np.array_equal(g.frame_host(), f.frame_host())
+
+
+def test_batched_equals_single_launches(gpu):
+    rng = np.random.default_rng(4)
+    progs, shots, seeds = [], [], []
+    for k in range(12):
+        d = int(rng.choice([3, 5, 7]))
+        c = circuits.rotated_surface_code(d, d, float(10 ** rng.uniform(-4, -2)))
+        progs.append(qcf.Program(c.ops))
+        shots.append(int(10 ** rng.uniform(3, 4.5)))
+        seeds.append(int(rng.integers(0, 2**62)))
+    from qcf import batch
+    outs = batch.sample_batched(progs, shots, seeds)
+    for p, s, sd, o in zip(progs, shots, seeds, outs):
+        single = qcf.pftrajectories(p, s, seed=sd, keep_frame=False).record_host()
+        assert np.array_equal(o, single)
