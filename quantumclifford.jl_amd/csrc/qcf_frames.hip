@@ -33,8 +33,7 @@ namespace {
 enum Mode : int { kNative = 0, kReplay = 1, kExport = 2 };
 
 struct KParams {
-  const Instr *prog;
-  const Site *sites;
+  const uint4 *prog;   // 2 uint4 per instruction
   uint32_t ninstr;
   uint32_t nblocks;    // 2048-shot tiles
   uint32_t nw32;       // valid 32-bit shot words
@@ -101,9 +100,14 @@ __device__ __forceinline__ void export_flip(const KParams &P, uint32_t plane,
   atomicXor(&P.masks[(uint64_t)plane * P.ld32 + gword], bit);
 }
 
+// Inline site fields of a noise instruction (see Instr).
+struct SiteRef {
+  uint32_t kind, ordinal, accept, t1, t2, rslot;
+};
+
 // Apply the events of class C that fall into this site's 2048 trials.
 template <int C, int MODE>
-__device__ __forceinline__ void site_events(EvState &ev, const Site &s,
+__device__ __forceinline__ void site_events(EvState &ev, const SiteRef &s,
                                             uint32_t pa, uint32_t pb,
                                             uint32_t pc, uint32_t pd,
                                             uint32_t *lds, const KParams &P,
@@ -119,8 +123,9 @@ __device__ __forceinline__ void site_events(EvState &ev, const Site &s,
       const uint32_t rb = P.body_rbase + s.rslot;
       const uint32_t gword = blk * 64u + tl;
       if (s.kind == 1) {
-        const bool fx = (uint64_t)u < s.t1 || (uint64_t)u >= s.t2;  // X or Y
-        const bool fz = (uint64_t)u >= s.t1;                        // Z or Y
+        const uint32_t u31 = u >> 1;
+        const bool fx = u31 < s.t1 || u31 >= s.t2;  // X or Y
+        const bool fz = u31 >= s.t1;                // Z or Y
         if (fx) {
           atomicXor(&lds[pa * 64u + tl], bit);
           export_flip<MODE>(P, rb, gword, bit);
@@ -158,136 +163,163 @@ __device__ __forceinline__ void site_events(EvState &ev, const Site &s,
   }
 }
 
+// vmcnt(0) only (expcnt and lgkmcnt left at their maxima): waits for the
+// program prefetch and the record stores once per 64-instruction chunk.
+#define QCF_WAIT_VMEM() __builtin_amdgcn_s_waitcnt(0x0F70)
+
 // Run the whole program on one 2048-shot tile.
 template <int MODE>
 __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
                                          uint32_t *lds, uint32_t lane) {
-  const uint4 *prog4 = reinterpret_cast<const uint4 *>(P.prog);
-  {
-    const uint32_t gw = blk * 64u + lane;
-    const bool live = gw < P.nw32;
-    const uint32_t lmask = !live ? 0u : (gw == P.nw32 - 1 ? P.lastmask : 0xFFFFFFFFu);
-    const uint64_t goff = gw;
-    Xoshiro128 gauge{0, 0, 0, 1};
-    if (MODE != kReplay) gauge = gauge_stream(P.seed, gw);
-    EvState ev0{0, 0, 0xFFFFFFFFu, 0xFFFFFFFFu, 0}, ev1 = ev0, ev2 = ev0, ev3 = ev0;
-    if (MODE != kReplay) {
-      if (P.nclasses > 0) gen_batch<0>(ev0, P, blk, lane);
-      if (P.nclasses > 1) gen_batch<1>(ev1, P, blk, lane);
-      if (P.nclasses > 2) gen_batch<2>(ev2, P, blk, lane);
-      if (P.nclasses > 3) gen_batch<3>(ev3, P, blk, lane);
-    }
-    auto draw = [&](uint32_t rplane) -> uint32_t {
-      if (MODE == kReplay) return live ? P.masks[(uint64_t)rplane * P.ld32 + goff] : 0u;
-      const uint32_t v = gauge.next();
-      if (MODE == kExport && live) P.masks[(uint64_t)rplane * P.ld32 + goff] = v & lmask;
-      return v;
-    };
-    uint32_t acc = 0;
-    uint4 cur = lane < P.ninstr ? prog4[lane] : make_uint4(0, 0, 0, 0);
-    for (uint32_t base = 0; base < P.ninstr; base += 64u) {
-      const uint32_t nb = base + 64u;
-      const uint4 nxt = nb + lane < P.ninstr ? prog4[nb + lane] : make_uint4(0, 0, 0, 0);
-      const uint32_t cnt = min(64u, P.ninstr - base);
-      for (uint32_t k = 0; k < cnt; ++k) {
-        const uint32_t op = readlane(cur.x, k);
-        const uint32_t a = readlane(cur.y, k);
-        const uint32_t b = readlane(cur.z, k);
-        const uint32_t c = readlane(cur.w, k);
-        switch (op) {
-          case I_XOR2: {
-            const uint32_t c0 = c & 0xFFFFu, c1 = c >> 16;
-            const uint32_t vb = lds[b * 64u + lane];
-            const uint32_t vd = lds[c1 * 64u + lane];
-            atomicXor(&lds[a * 64u + lane], vb);
-            atomicXor(&lds[c0 * 64u + lane], vd);
-            break;
-          }
-          case I_XOR:
-            atomicXor(&lds[a * 64u + lane], lds[b * 64u + lane]);
-            break;
-          case I_NOISE1:
-          case I_NOISE2: {
-            const Site s = P.sites[c];
-            uint32_t pa, pb, pc = 0, pd = 0;
-            if (op == I_NOISE1) {
-              pa = a;
-              pb = b;
-            } else {
-              pa = a & 0xFFFFu;
-              pb = a >> 16;
-              pc = b & 0xFFFFu;
-              pd = b >> 16;
-            }
-            if (MODE == kReplay) {
-              const uint32_t rb = P.body_rbase + s.rslot;
-              const uint32_t m0 = draw(rb), m1 = draw(rb + 1);
-              lds[pa * 64u + lane] ^= m0;
-              lds[pb * 64u + lane] ^= m1;
-              if (op == I_NOISE2) {
-                const uint32_t m2 = draw(rb + 2), m3 = draw(rb + 3);
-                lds[pc * 64u + lane] ^= m2;
-                lds[pd * 64u + lane] ^= m3;
-              }
-            } else {
-              switch (s.cls) {
-                case 0: site_events<0, MODE>(ev0, s, pa, pb, pc, pd, lds, P, blk, lane); break;
-                case 1: site_events<1, MODE>(ev1, s, pa, pb, pc, pd, lds, P, blk, lane); break;
-                case 2: site_events<2, MODE>(ev2, s, pa, pb, pc, pd, lds, P, blk, lane); break;
-                case 3: site_events<3, MODE>(ev3, s, pa, pb, pc, pd, lds, P, blk, lane); break;
-                default: break;  // zero-rate site
-              }
-            }
-            break;
-          }
-          case I_MEAS: {
-            const uint32_t v = lds[a * 64u + lane];
-            if (live) P.rec[(uint64_t)b * P.ld32 + goff] = v & lmask;
-            if (c != kNone) lds[c * 64u + lane] = v;
-            break;
-          }
-          case I_RESETZ: {
-            lds[a * 64u + lane] = 0u;
-            const uint32_t r = draw(P.body_rbase + c);
-            lds[b * 64u + lane] ^= r;
-            break;
-          }
-          case I_ACC_LDS: acc = lds[a * 64u + lane]; break;
-          case I_ACCX_LDS: acc ^= lds[a * 64u + lane]; break;
-          case I_ACC_REC: acc = live ? P.rec[(uint64_t)a * P.ld32 + goff] : 0u; break;
-          case I_ACCX_REC: acc ^= live ? P.rec[(uint64_t)a * P.ld32 + goff] : 0u; break;
-          case I_ACC_STORE:
-            if (live) P.rec[(uint64_t)a * P.ld32 + goff] = acc & lmask;
-            if (b != kNone) lds[b * 64u + lane] = acc;
-            break;
-          case I_REC_ZERO:
-            if (live) P.rec[(uint64_t)a * P.ld32 + goff] = 0u;
-            break;
-          case I_INIT: {
-            lds[a * 64u + lane] = 0u;
-            const uint32_t r = draw(c);
-            lds[b * 64u + lane] = r;
-            break;
-          }
-          case I_LOAD:
-            lds[a * 64u + lane] = live ? P.frame[(uint64_t)b * P.ld32 + goff] : 0u;
-            break;
-          case I_STORE:
-            if (live) P.frame[(uint64_t)b * P.ld32 + goff] = lds[a * 64u + lane] & lmask;
-            break;
-          case I_ZERO: lds[a * 64u + lane] = 0u; break;
-          case I_LIN1: {
-            const uint32_t x = lds[a * 64u + lane], z = lds[b * 64u + lane];
-            const uint32_t nx = ((c & 1u) ? x : 0u) ^ ((c & 2u) ? z : 0u);
-            const uint32_t nz = ((c & 4u) ? x : 0u) ^ ((c & 8u) ? z : 0u);
-            lds[a * 64u + lane] = nx;
-            lds[b * 64u + lane] = nz;
-            break;
-          }
-          default: break;
+  const uint4 *__restrict__ prog4 = P.prog;
+  const uint32_t gw = blk * 64u + lane;
+  const bool live = gw < P.nw32;
+  const uint32_t lmask = !live ? 0u : (gw == P.nw32 - 1 ? P.lastmask : 0xFFFFFFFFu);
+  const uint64_t goff = gw;
+  Xoshiro128 gauge{0, 0, 0, 1};
+  if (MODE != kReplay) gauge = gauge_stream(P.seed, gw);
+  EvState ev0{0, 0, 0xFFFFFFFFu, 0xFFFFFFFFu, 0};
+  if (MODE != kReplay && P.nclasses > 0) gen_batch<0>(ev0, P, blk, lane);
+  // Every global load in the instruction loop is waited for on the spot, so
+  // no load is ever pending where control flow merges (which would make the
+  // compiler wait for all outstanding record stores there).
+  auto gload = [&](const uint32_t *base, uint32_t plane) -> uint32_t {
+    const uint32_t v = live ? base[(uint64_t)plane * P.ld32 + goff] : 0u;
+    QCF_WAIT_VMEM();
+    return v;
+  };
+  auto draw = [&](uint32_t rplane) -> uint32_t {
+    if (MODE == kReplay) return gload(P.masks, rplane);
+    const uint32_t v = gauge.next();
+    if (MODE == kExport && live) P.masks[(uint64_t)rplane * P.ld32 + goff] = v & lmask;
+    return v;
+  };
+  auto L = [&](uint32_t plane) -> uint32_t & { return lds[plane * 64u + lane]; };
+  uint32_t acc = 0;
+  const uint4 zero4 = make_uint4(0, 0, 0, 0);
+  uint4 nA = lane < P.ninstr ? prog4[2 * lane] : zero4;
+  uint4 nB = lane < P.ninstr ? prog4[2 * lane + 1] : zero4;
+  for (uint32_t base = 0; base < P.ninstr; base += 64u) {
+    QCF_WAIT_VMEM();
+    const uint4 cA = nA, cB = nB;
+    const uint32_t nb = base + 64u + lane;
+    nA = nb < P.ninstr ? prog4[2 * nb] : zero4;
+    nB = nb < P.ninstr ? prog4[2 * nb + 1] : zero4;
+    const uint32_t cnt = min(64u, P.ninstr - base);
+    for (uint32_t k = 0; k < cnt; ++k) {
+      const uint32_t opw = readlane(cA.x, k);
+      const uint32_t op = opw & 0xFFu;
+      const uint32_t a = readlane(cA.y, k);
+      const uint32_t b = readlane(cA.z, k);
+      const uint32_t c = readlane(cA.w, k);
+      if (op == I_XORN) {
+        const uint32_t n = opw >> 8;
+        uint32_t w[7] = {a, b, c, 0, 0, 0, 0};
+        if (n > 3) {
+          w[3] = readlane(cB.x, k);
+          w[4] = readlane(cB.y, k);
+          w[5] = readlane(cB.z, k);
+          w[6] = readlane(cB.w, k);
         }
+        uint32_t v[7];
+#pragma unroll
+        for (int i = 0; i < 7; ++i)
+          if ((uint32_t)i < n) v[i] = L(w[i] >> 16);
+#pragma unroll
+        for (int i = 0; i < 7; ++i)
+          if ((uint32_t)i < n) atomicXor(&L(w[i] & 0xFFFFu), v[i]);
+        continue;
       }
-      cur = nxt;
+      if (op == I_NOISE1 || op == I_NOISE2) {
+        uint32_t pa, pb, pc = 0, pd = 0;
+        if (op == I_NOISE1) {
+          pa = a;
+          pb = b;
+        } else {
+          pa = a & 0xFFFFu;
+          pb = a >> 16;
+          pc = b & 0xFFFFu;
+          pd = b >> 16;
+        }
+        if (MODE == kReplay) {
+          const uint32_t rb = P.body_rbase + c;
+          const uint32_t m0 = draw(rb), m1 = draw(rb + 1);
+          L(pa) ^= m0;
+          L(pb) ^= m1;
+          if (op == I_NOISE2) {
+            const uint32_t m2 = draw(rb + 2), m3 = draw(rb + 3);
+            L(pc) ^= m2;
+            L(pd) ^= m3;
+          }
+          continue;
+        }
+        const uint32_t d = readlane(cB.x, k);
+        const uint32_t cls = d & 0xFFu;
+        if (cls == kNoFire) continue;
+        SiteRef s;
+        s.kind = (d >> 8) & 0xFu;
+        s.ordinal = d >> 12;
+        s.accept = readlane(cB.y, k);
+        s.t1 = readlane(cB.z, k);
+        s.t2 = readlane(cB.w, k);
+        s.rslot = c;
+        site_events<0, MODE>(ev0, s, pa, pb, pc, pd, lds, P, blk, lane);
+        continue;
+      }
+      switch (op) {
+        case I_MEAS: {
+          const uint32_t v = L(a);
+          if (live) P.rec[(uint64_t)b * P.ld32 + goff] = v & lmask;
+          if (c != kNone) L(c) = v;
+          break;
+        }
+        case I_XSTORE2: {
+          const uint32_t v = L(a) ^ L(b);
+          if (live) P.rec[(uint64_t)c * P.ld32 + goff] = v & lmask;
+          const uint32_t keep = readlane(cB.x, k);
+          if (keep != kNone) L(keep) = v;
+          break;
+        }
+        case I_RESETZ: {
+          L(a) = 0u;
+          const uint32_t r = draw(P.body_rbase + c);
+          L(b) ^= r;
+          break;
+        }
+        case I_ACC_LDS: acc = L(a); break;
+        case I_ACCX_LDS: acc ^= L(a); break;
+        case I_ACC_REC: acc = gload(P.rec, a); break;
+        case I_ACCX_REC: acc ^= gload(P.rec, a); break;
+        case I_ACC_STORE:
+          if (live) P.rec[(uint64_t)a * P.ld32 + goff] = acc & lmask;
+          if (b != kNone) L(b) = acc;
+          break;
+        case I_REC_ZERO:
+          if (live) P.rec[(uint64_t)a * P.ld32 + goff] = 0u;
+          break;
+        case I_INIT: {
+          L(a) = 0u;
+          const uint32_t r = draw(c);
+          L(b) = r;
+          break;
+        }
+        case I_LOAD:
+          L(a) = gload(P.frame, b);
+          break;
+        case I_STORE:
+          if (live) P.frame[(uint64_t)b * P.ld32 + goff] = L(a) & lmask;
+          break;
+        case I_ZERO: L(a) = 0u; break;
+        case I_LIN1: {
+          const uint32_t x = L(a), z = L(b);
+          const uint32_t nx = ((c & 1u) ? x : 0u) ^ ((c & 2u) ? z : 0u);
+          const uint32_t nz = ((c & 4u) ? x : 0u) ^ ((c & 8u) ? z : 0u);
+          L(a) = nx;
+          L(b) = nz;
+          break;
+        }
+        default: break;
+      }
     }
   }
 }
@@ -406,8 +438,8 @@ __global__ void frame_from_reference_kernel(const uint64_t *xzs, uint64_t shots,
 }
 
 // ---- host side
-int ensure_device_copy(qcf_program *P, int variant, const Instr **prog,
-                       uint32_t *ninstr, const Site **sites) {
+int ensure_device_copy(qcf_program *P, int variant, const uint4 **prog,
+                       uint32_t *ninstr) {
   int dev = 0;
   QCF_HIP_CHECK(hipGetDevice(&dev));
   static std::mutex mu;
@@ -427,23 +459,15 @@ int ensure_device_copy(qcf_program *P, int variant, const Instr **prog,
     code.insert(code.end(), pro.begin(), pro.end());
     code.insert(code.end(), P->body.begin(), P->body.end());
     if (variant & 2) code.insert(code.end(), P->epilogue_store.begin(), P->epilogue_store.end());
-    if (code.empty()) code.push_back(Instr{I_END, 0, 0, 0});
+    if (code.empty()) code.push_back(Instr{I_END, 0, 0, 0, 0, 0, 0, 0});
     void *d = nullptr;
     QCF_HIP_CHECK(hipMalloc(&d, code.size() * sizeof(Instr)));
     QCF_HIP_CHECK(hipMemcpy(d, code.data(), code.size() * sizeof(Instr), hipMemcpyHostToDevice));
     dc->prog[variant] = d;
     dc->ninstr[variant] = (uint32_t)code.size();
   }
-  if (!dc->sites) {
-    const size_t n = std::max<size_t>(1, P->sites.size());
-    QCF_HIP_CHECK(hipMalloc(&dc->sites, n * sizeof(Site)));
-    if (!P->sites.empty())
-      QCF_HIP_CHECK(hipMemcpy(dc->sites, P->sites.data(), P->sites.size() * sizeof(Site),
-                              hipMemcpyHostToDevice));
-  }
-  *prog = (const Instr *)dc->prog[variant];
+  *prog = (const uint4 *)dc->prog[variant];
   *ninstr = dc->ninstr[variant];
-  *sites = (const Site *)dc->sites;
   return QCF_OK;
 }
 
@@ -501,7 +525,7 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
                      lds_bytes, lds_max);
   const int variant = (fresh ? 1 : 0) | (frame ? 2 : 0);
   KParams kp{};
-  QCF_TRY(ensure_device_copy(P, variant, &kp.prog, &kp.ninstr, &kp.sites));
+  QCF_TRY(ensure_device_copy(P, variant, &kp.prog, &kp.ninstr));
   const uint64_t nw32 = (shots + 31) / 32;
   kp.nw32 = (uint32_t)nw32;
   kp.nblocks = (uint32_t)((nw32 + 63) / 64);
@@ -635,7 +659,7 @@ int build_kparams(const qcf_program *prog, uint64_t shots, uint64_t seed,
   if (!meas && P->nbits > 0)
     return set_error(QCF_ERR_ARGUMENT, "batched pftrajectories: null record");
   *kp = KParams{};
-  QCF_TRY(ensure_device_copy(P, 1, &kp->prog, &kp->ninstr, &kp->sites));
+  QCF_TRY(ensure_device_copy(P, 1, &kp->prog, &kp->ninstr));
   const uint64_t nw32 = (shots + 31) / 32;
   kp->nw32 = (uint32_t)nw32;
   kp->nblocks = (uint32_t)((nw32 + 63) / 64);

@@ -179,7 +179,10 @@ struct Compiler {
   }
 
   void emit(uint32_t op, uint32_t a, uint32_t b = 0, uint32_t c = 0) {
-    P->body.push_back(Instr{op, a, b, c});
+    P->body.push_back(Instr{op, a, b, c, 0, 0, 0, 0});
+  }
+  void emit_xor(uint32_t dst, uint32_t src) {
+    emit(I_XORN | (1u << 8), dst | (src << 16));
   }
 
   uint32_t alloc_slot() {
@@ -199,8 +202,7 @@ struct Compiler {
       slot[3] = zs[q2];
     }
     const Decomp &d = gate_decomp(g);
-    for (int k = 0; k < d.len; ++k)
-      emit(I_XOR, slot[d.ops[k][0]], slot[d.ops[k][1]]);
+    for (int k = 0; k < d.len; ++k) emit_xor(slot[d.ops[k][0]], slot[d.ops[k][1]]);
     uint32_t ns[4];
     for (int i = 0; i < 2 * g.nq; ++i) ns[d.perm[i]] = slot[i];
     xs[q1] = ns[0];
@@ -228,7 +230,7 @@ struct Compiler {
       return QCF_OK;
     }
     uint32_t slot[2] = {xs[q], zs[q]};
-    for (int k = 0; k < d.len; ++k) emit(I_XOR, slot[d.ops[k][0]], slot[d.ops[k][1]]);
+    for (int k = 0; k < d.len; ++k) emit_xor(slot[d.ops[k][0]], slot[d.ops[k][1]]);
     uint32_t ns[2];
     for (int i = 0; i < 2; ++i) ns[d.perm[i]] = slot[i];
     xs[q] = ns[0];
@@ -281,11 +283,9 @@ struct Compiler {
     s.rslot = rs;
     rs += 2;
     if (c3 > 0) {
-      const double two32 = 4294967296.0;
-      s.t1 = (uint64_t)std::floor(c1 / c3 * two32);
-      s.t2 = (uint64_t)std::floor(c2 / c3 * two32);
-      if (c1 >= c3) s.t1 = (uint64_t)1 << 32;
-      if (c2 >= c3) s.t2 = (uint64_t)1 << 32;
+      const double two31 = 2147483648.0;
+      s.t1 = c1 >= c3 ? (1u << 31) : (uint32_t)std::floor(c1 / c3 * two31);
+      s.t2 = c2 >= c3 ? (1u << 31) : (uint32_t)std::floor(c2 / c3 * two31);
     }
     s.bitops = bitops;
     const uint32_t id = (uint32_t)P->sites.size();
@@ -375,22 +375,62 @@ struct Compiler {
     return QCF_OK;
   }
 
-  // peephole: fuse XOR pairs whose second source is not the first target
-  void fuse_xors() {
+  // Write each noise site's data into its instruction (class, ordinal,
+  // thinning and Pauli thresholds travel with the instruction stream).
+  void patch_sites() {
+    for (Instr &in : P->body) {
+      if (in.op != I_NOISE1 && in.op != I_NOISE2) continue;
+      const Site &s = P->sites[in.c];
+      in.c = s.rslot;
+      in.d = (s.cls & 0xFFu) | (s.kind << 8) | (s.ordinal << 12);
+      in.e = s.accept;
+      in.f = s.t1;
+      in.g = s.t2;
+    }
+  }
+
+  // Peepholes: bundle runs of plane XORs into XORN (up to 7 pairs, a pair is
+  // added only if its source is not a target already in the bundle, so
+  // reading all sources first equals the sequential order), and fuse the
+  // ClassicalXOR sequences ACC,ACCX,STORE -> XSTORE2 and ACC,STORE -> MEAS.
+  void peephole() {
     std::vector<Instr> out;
     out.reserve(P->body.size());
-    for (size_t i = 0; i < P->body.size(); ++i) {
-      const Instr &a = P->body[i];
-      if (a.op == I_XOR && i + 1 < P->body.size() && P->body[i + 1].op == I_XOR) {
-        const Instr &b = P->body[i + 1];
-        if (b.b != a.a && a.a < 0x10000 && b.a < 0x10000 && b.b < 0x10000) {
-          out.push_back(Instr{I_XOR2, a.a, a.b, b.a | (b.b << 16)});
-          ++i;
-          continue;
-        }
+    std::vector<uint32_t> pairs, dsts;
+    auto flush = [&]() {
+      if (pairs.empty()) return;
+      Instr in{I_XORN | ((uint32_t)pairs.size() << 8), 0, 0, 0, 0, 0, 0, 0};
+      uint32_t *w = &in.a;
+      for (size_t i = 0; i < pairs.size(); ++i) w[i] = pairs[i];
+      out.push_back(in);
+      pairs.clear();
+      dsts.clear();
+    };
+    const auto &B = P->body;
+    for (size_t i = 0; i < B.size(); ++i) {
+      const Instr &in = B[i];
+      if (in.op == (I_XORN | (1u << 8))) {
+        const uint32_t dst = in.a & 0xFFFFu, src = in.a >> 16;
+        if (pairs.size() == 7 || std::find(dsts.begin(), dsts.end(), src) != dsts.end()) flush();
+        pairs.push_back(in.a);
+        dsts.push_back(dst);
+        continue;
       }
-      out.push_back(a);
+      flush();
+      if (in.op == I_ACC_LDS && i + 2 < B.size() && B[i + 1].op == I_ACCX_LDS &&
+          B[i + 2].op == I_ACC_STORE) {
+        out.push_back(Instr{I_XSTORE2, in.a, B[i + 1].a, B[i + 2].a, B[i + 2].b, 0, 0, 0});
+        i += 2;
+        continue;
+      }
+      if (in.op == I_ACC_LDS && i + 1 < B.size() && B[i + 1].op == I_ACC_STORE) {
+        out.push_back(Instr{I_MEAS, in.a, B[i + 1].a, B[i + 1].b, 0, 0, 0, 0});
+        i += 1;
+        continue;
+      }
+      out.push_back(in);
     }
+    flush();
     P->body.swap(out);
   }
 };
@@ -425,7 +465,6 @@ qcf_program::~qcf_program() {
     (void)hipSetDevice(d.device);
     for (void *p : d.prog)
       if (p) (void)hipFree(p);
-    if (d.sites) (void)hipFree(d.sites);
     (void)hipSetDevice(cur);
   }
 }
@@ -645,11 +684,12 @@ extern "C" int qcf_program_build(const qcf_op *ops, size_t nops,
       return st;
     }
   }
-  C.fuse_xors();
   if (int st = C.assign_classes(); st != QCF_OK) {
     delete P;
     return st;
   }
+  C.patch_sites();
+  C.peephole();
   P->nplanes = C.next_slot;
   if (P->nplanes > 0xFFFF) {
     delete P;
@@ -659,14 +699,15 @@ extern "C" int qcf_program_build(const qcf_op *ops, size_t nops,
   P->ninit_rs = nq;
   // ---- prologues / epilogue
   for (uint32_t q = 0; q < nq; ++q)
-    P->prologue_fresh.push_back(Instr{I_INIT, 2 * q, 2 * q + 1, q});
+    P->prologue_fresh.push_back(Instr{I_INIT, 2 * q, 2 * q + 1, q, 0, 0, 0, 0});
   for (uint32_t b = 0; b < B; ++b)
-    if (!C.written[b] || C.zero_first[b]) P->prologue_fresh.push_back(Instr{I_REC_ZERO, b, 0, 0});
+    if (!C.written[b] || C.zero_first[b])
+      P->prologue_fresh.push_back(Instr{I_REC_ZERO, b, 0, 0, 0, 0, 0, 0});
   for (uint32_t q = 0; q < nq; ++q) {
-    P->prologue_inplace.push_back(Instr{I_LOAD, 2 * q, q, 0});
-    P->prologue_inplace.push_back(Instr{I_LOAD, 2 * q + 1, nq + q, 0});
-    P->epilogue_store.push_back(Instr{I_STORE, C.xs[q], q, 0});
-    P->epilogue_store.push_back(Instr{I_STORE, C.zs[q], nq + q, 0});
+    P->prologue_inplace.push_back(Instr{I_LOAD, 2 * q, q, 0, 0, 0, 0, 0});
+    P->prologue_inplace.push_back(Instr{I_LOAD, 2 * q + 1, nq + q, 0, 0, 0, 0, 0});
+    P->epilogue_store.push_back(Instr{I_STORE, C.xs[q], q, 0, 0, 0, 0, 0});
+    P->epilogue_store.push_back(Instr{I_STORE, C.zs[q], nq + q, 0, 0, 0, 0, 0});
   }
   *out = P;
   return QCF_OK;

@@ -18,33 +18,38 @@ namespace qcf {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint32_t kBlockShots = 2048;  // shots per wave tile (64 lanes x 32)
-constexpr uint32_t kMaxClasses = 4;     // noise-rate classes per program
+constexpr uint32_t kMaxClasses = 1;     // noise-rate classes per program (others thinned)
 constexpr uint32_t kNoFire = 0xFFu;     // site class of a zero-rate site
 constexpr uint32_t kMaxOrdinals = 1u << 20;  // sites per class
 
 enum InstrOp : uint32_t {
   I_END = 0,
-  I_XOR,        // P[a] ^= P[b]
-  I_XOR2,       // P[a] ^= P[b]; P[c & 0xffff] ^= P[c >> 16]   (independent pairs)
-  I_ZERO,       // P[a] = 0
-  I_INIT,       // P[a] = 0; P[b] = draw(c)            (PauliFrame + initZ!)
-  I_LOAD,       // P[a] = frame[b]
-  I_STORE,      // frame[b] = P[a]
+  I_XORN,       // n = op>>8 (1..7) pairs in words a..g: P[lo16] ^= P[hi16]; all
+                // sources are read before any target is written (the host only
+                // bundles pairs where that equals sequential order)
+  I_NOISE1,     // planes x=a, z=b; c = replay plane; d..g = site (see below)
+  I_NOISE2,     // planes x1=a&0xffff z1=a>>16 x2=b&0xffff z2=b>>16; c, d..g as NOISE1
   I_MEAS,       // v = P[a]; rec[b] = v; if c != none: P[c] = v
+  I_XSTORE2,    // v = P[a] ^ P[b]; rec[c] = v; if d != none: P[d] = v
   I_RESETZ,     // P[a] = 0; P[b] ^= draw(c)           (sMRZ reset part)
-  I_NOISE1,     // site c on planes x=a, z=b
-  I_NOISE2,     // site c on planes x1=a&0xffff z1=a>>16 x2=b&0xffff z2=b>>16
   I_ACC_LDS,    // acc = P[a]
   I_ACCX_LDS,   // acc ^= P[a]
   I_ACC_REC,    // acc = rec[a]
   I_ACCX_REC,   // acc ^= rec[a]
   I_ACC_STORE,  // rec[a] = acc; if b != none: P[b] = acc
   I_REC_ZERO,   // rec[a] = 0
+  I_INIT,       // P[a] = 0; P[b] = draw(c)            (PauliFrame + initZ!)
+  I_LOAD,       // P[a] = frame[b]
+  I_STORE,      // frame[b] = P[a]
+  I_ZERO,       // P[a] = 0
   I_LIN1,       // (P[a],P[b]) := linear map c (bit0 x<-x, bit1 x<-z, bit2 z<-x, bit3 z<-z)
 };
 
+// 32-byte device instruction.  Noise instructions carry their site inline:
+//   d = class | kind << 8 | ordinal << 12,  e = accept,  f = t1,  g = t2
+// (thresholds are 2^31-scaled and compared with the top 31 bits of a draw).
 struct Instr {
-  uint32_t op, a, b, c;
+  uint32_t op, a, b, c, d, e, f, g;
 };
 
 // One noise site.  Native sampling draws events at rate p_class along the
@@ -57,7 +62,7 @@ struct Site {
   uint32_t ordinal;  // index of this site within its class
   uint32_t accept;   // thinning threshold in [0, 2^31]
   uint32_t rslot;    // first replay plane (body relative)
-  uint64_t t1, t2;   // 2^32-scaled cumulative thresholds
+  uint32_t t1, t2;   // 2^31-scaled cumulative thresholds
   uint32_t kind;     // 1 or 2 qubits
   uint32_t bitops;   // accounting weight
 };
@@ -88,7 +93,6 @@ struct qcf_program {
     int device = -1;
     void *prog[4] = {nullptr, nullptr, nullptr, nullptr};  // variant -> Instr*
     uint32_t ninstr[4] = {0, 0, 0, 0};
-    void *sites = nullptr;
   };
   std::vector<DeviceCopy> dev;
   ~qcf_program();

@@ -173,7 +173,8 @@ def rotated_surface_code(d: int = 5, rounds: int | None = None, p: float = 1e-3,
             bit += 1
             ops.append(sMRZ(idx[m], bit))
             cur[m] = bit
-        for m in meas:
+            # each detector right after its measurement: the previous round's
+            # value is then the only one that has to stay on chip
             if r == 0:
                 if m in zm:
                     dets.append((cur[m],))

@@ -79,7 +79,9 @@ def test_surface_code_program_fits_on_chip():
     assert abs(p.bitops_per_shot - 18217) < 1e-9
 
 
-def test_rate_classes_group_many_probabilities():
+def test_sites_of_all_rates_share_one_thinned_class():
     circ = [qcf.PauliError(1, p) for p in np.geomspace(1e-5, 1e-1, 9)]
     p = qcf.Program(circ)
-    assert p.info.nclasses == 4
+    assert p.info.nclasses == 1 and p.info.nsites == 9
+    # a zero-rate site needs no class
+    assert qcf.Program([qcf.PauliError(1, 0.0)]).info.nclasses == 0
