@@ -37,6 +37,7 @@ struct KParams {
   uint32_t ninstr;
   uint32_t nblocks;    // 2048-shot tiles
   uint32_t nw32;       // valid 32-bit shot words
+  uint32_t nwbuf;      // 32-bit words up to the end of the last 64-bit word
   uint32_t lastmask;   // valid bits of word nw32-1
   uint64_t ld32;       // 32-bit words per plane of frame / record / masks
   uint64_t seed;
@@ -44,6 +45,9 @@ struct KParams {
   uint32_t *rec;
   uint32_t *masks;
   uint32_t body_rbase; // replay plane of the first body draw
+  uint32_t *scratch;   // global plane storage (NULL: planes in LDS)
+  uint32_t nplanes;    // planes per tile
+  uint32_t sstride;    // 32-bit words of global plane storage per wave slot
   uint32_t nclasses;
   float cls_c[kMaxClasses];
 };
@@ -100,66 +104,85 @@ __device__ __forceinline__ void export_flip(const KParams &P, uint32_t plane,
   atomicXor(&P.masks[(uint64_t)plane * P.ld32 + gword], bit);
 }
 
-// Inline site fields of a noise instruction (see Instr).
-struct SiteRef {
-  uint32_t kind, ordinal, accept, t1, t2, rslot;
+// Plane storage of a tile: LDS (the fast path) or, for circuits whose frame
+// does not fit on chip, a per-wave region of global memory.  In global memory
+// every access is an agent-scope atomic so that bit flips written by other
+// lanes of the wave (noise events) are never hidden by a stale L1 line.
+template <bool GL>
+struct Planes {
+  uint32_t *base;
+  __device__ __forceinline__ uint32_t rd(uint32_t p, uint32_t l) const {
+    if (GL) return __hip_atomic_load(base + p * 64u + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return base[p * 64u + l];
+  }
+  __device__ __forceinline__ void wr(uint32_t p, uint32_t l, uint32_t v) const {
+    if (GL) __hip_atomic_store(base + p * 64u + l, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else base[p * 64u + l] = v;
+  }
+  __device__ __forceinline__ void xr(uint32_t p, uint32_t l, uint32_t v) const {
+    if (GL) __hip_atomic_fetch_xor(base + p * 64u + l, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else atomicXor(base + p * 64u + l, v);
+  }
 };
 
-// Apply the events of class C that fall into this site's 2048 trials.
-template <int C, int MODE>
-__device__ __forceinline__ void site_events(EvState &ev, const SiteRef &s,
-                                            uint32_t pa, uint32_t pb,
-                                            uint32_t pc, uint32_t pd,
-                                            uint32_t *lds, const KParams &P,
-                                            uint32_t blk, uint32_t lane) {
-  const uint32_t lo = s.ordinal * kBlockShots;
-  const uint32_t hi = lo + kBlockShots;
+__device__ __forceinline__ uint32_t bperm(uint32_t src_lane, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
+}
+
+// Native events of one noise group: G sites with consecutive ordinals from
+// `ord0`, whose data sit in lanes k+1 .. k+G of the current chunk (cA, cB).
+// Every event lane fetches its site's data with ds_bpermute (executed with all
+// lanes active, as the instruction requires) and flips the target shot's bits
+// with LDS atomics.
+template <int MODE, bool GL>
+__device__ __forceinline__ void group_events(EvState &ev, uint32_t ord0, uint32_t G,
+                                             uint32_t k, const uint4 &cA, const uint4 &cB,
+                                             const Planes<GL> &F, const KParams &P,
+                                             uint32_t blk, uint32_t lane) {
+  const uint32_t lo = ord0 * kBlockShots;
+  const uint32_t span = G * kBlockShots;
   while (true) {
     const uint32_t rel = ev.pos - lo;
-    if (rel < kBlockShots && ev.uacc < s.accept) {
-      const uint32_t tl = rel >> 5;
-      const uint32_t bit = 1u << (rel & 31u);
-      const uint32_t u = ev.upau;
-      const uint32_t rb = P.body_rbase + s.rslot;
-      const uint32_t gword = blk * 64u + tl;
-      if (s.kind == 1) {
-        const uint32_t u31 = u >> 1;
-        const bool fx = u31 < s.t1 || u31 >= s.t2;  // X or Y
-        const bool fz = u31 >= s.t1;                // Z or Y
-        if (fx) {
-          atomicXor(&lds[pa * 64u + tl], bit);
-          export_flip<MODE>(P, rb, gword, bit);
+    const bool inr = rel < span;
+    if (__builtin_amdgcn_ballot_w64(inr && ev.uacc != 0xFFFFFFFFu) != 0) {
+      const uint32_t site = inr ? (rel >> 11) : 0u;
+      const uint32_t src = k + 1 + site;
+      const uint32_t pab = bperm(src, cA.y);
+      const uint32_t pcd = bperm(src, cA.z);
+      const uint32_t rs = bperm(src, cA.w);
+      const uint32_t kind = bperm(src, cB.x);
+      const uint32_t acc = bperm(src, cB.y);
+      const uint32_t t1 = bperm(src, cB.z);
+      const uint32_t t2 = bperm(src, cB.w);
+      if (inr && ev.uacc < acc) {
+        const uint32_t within = rel & (kBlockShots - 1);
+        const uint32_t tl = within >> 5;
+        const uint32_t bit = 1u << (within & 31u);
+        const uint32_t u = ev.upau;
+        const uint32_t rb = P.body_rbase + rs;
+        const uint32_t gword = blk * 64u + tl;
+        uint32_t fl;  // flip mask over (x1, z1, x2, z2)
+        if (kind == 1) {
+          const uint32_t u31 = u >> 1;
+          fl = (u31 < t1 || u31 >= t2 ? 1u : 0u) | (u31 >= t1 ? 2u : 0u);  // X/Z/Y
+        } else {
+          // uniform over the 15 non-identity Paulis; digit order as
+          // _apply_depolarization_pauli! (src/noise.jl:112-118): first qubit
+          // is the least significant base-4 digit, 1=X 2=Z 3=Y
+          fl = __umulhi(u, 15u) + 1u;
         }
-        if (fz) {
-          atomicXor(&lds[pb * 64u + tl], bit);
-          export_flip<MODE>(P, rb + 1, gword, bit);
-        }
-      } else {
-        // uniform over the 15 non-identity Paulis; digit order as
-        // _apply_depolarization_pauli! (src/noise.jl:112-118): first qubit
-        // is the least significant base-4 digit, 1=X 2=Z 3=Y
-        const uint32_t code = __umulhi(u, 15u) + 1u;
-        const uint32_t d1 = code & 3u, d2 = code >> 2;
-        if (d1 & 1u) {
-          atomicXor(&lds[pa * 64u + tl], bit);
-          export_flip<MODE>(P, rb, gword, bit);
-        }
-        if (d1 & 2u) {
-          atomicXor(&lds[pb * 64u + tl], bit);
-          export_flip<MODE>(P, rb + 1, gword, bit);
-        }
-        if (d2 & 1u) {
-          atomicXor(&lds[pc * 64u + tl], bit);
-          export_flip<MODE>(P, rb + 2, gword, bit);
-        }
-        if (d2 & 2u) {
-          atomicXor(&lds[pd * 64u + tl], bit);
-          export_flip<MODE>(P, rb + 3, gword, bit);
+        const uint32_t planes[4] = {pab & 0xFFFFu, pab >> 16, pcd & 0xFFFFu, pcd >> 16};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (fl & (1u << j)) {
+            F.xr(planes[j], tl, bit);
+            export_flip<MODE>(P, rb + j, gword, bit);
+          }
         }
       }
     }
-    if (ev.last >= hi) break;
-    gen_batch<C>(ev, P, blk, lane);
+    if (ev.last >= lo + span) break;
+    gen_batch<0>(ev, P, blk, lane);
   }
 }
 
@@ -168,13 +191,14 @@ __device__ __forceinline__ void site_events(EvState &ev, const SiteRef &s,
 #define QCF_WAIT_VMEM() __builtin_amdgcn_s_waitcnt(0x0F70)
 
 // Run the whole program on one 2048-shot tile.
-template <int MODE>
+template <int MODE, bool GL>
 __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
-                                         uint32_t *lds, uint32_t lane) {
+                                         const Planes<GL> &F, uint32_t lane) {
   const uint4 *__restrict__ prog4 = P.prog;
   const uint32_t gw = blk * 64u + lane;
-  const bool live = gw < P.nw32;
-  const uint32_t lmask = !live ? 0u : (gw == P.nw32 - 1 ? P.lastmask : 0xFFFFFFFFu);
+  // lanes up to the end of the last 64-bit word store (zeros past the shots)
+  const bool live = gw < P.nwbuf;
+  const uint32_t lmask = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1 ? P.lastmask : 0xFFFFFFFFu);
   const uint64_t goff = gw;
   Xoshiro128 gauge{0, 0, 0, 1};
   if (MODE != kReplay) gauge = gauge_stream(P.seed, gw);
@@ -194,7 +218,9 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
     if (MODE == kExport && live) P.masks[(uint64_t)rplane * P.ld32 + goff] = v & lmask;
     return v;
   };
-  auto L = [&](uint32_t plane) -> uint32_t & { return lds[plane * 64u + lane]; };
+  auto RD = [&](uint32_t p) -> uint32_t { return F.rd(p, lane); };
+  auto WR = [&](uint32_t p, uint32_t v) { F.wr(p, lane, v); };
+  auto XR = [&](uint32_t p, uint32_t v) { F.xr(p, lane, v); };
   uint32_t acc = 0;
   const uint4 zero4 = make_uint4(0, 0, 0, 0);
   uint4 nA = lane < P.ninstr ? prog4[2 * lane] : zero4;
@@ -224,98 +250,83 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
         uint32_t v[7];
 #pragma unroll
         for (int i = 0; i < 7; ++i)
-          if ((uint32_t)i < n) v[i] = L(w[i] >> 16);
+          if ((uint32_t)i < n) v[i] = RD(w[i] >> 16);
 #pragma unroll
         for (int i = 0; i < 7; ++i)
-          if ((uint32_t)i < n) atomicXor(&L(w[i] & 0xFFFFu), v[i]);
+          if ((uint32_t)i < n) XR(w[i] & 0xFFFFu, v[i]);
         continue;
       }
-      if (op == I_NOISE1 || op == I_NOISE2) {
-        uint32_t pa, pb, pc = 0, pd = 0;
-        if (op == I_NOISE1) {
-          pa = a;
-          pb = b;
-        } else {
-          pa = a & 0xFFFFu;
-          pb = a >> 16;
-          pc = b & 0xFFFFu;
-          pd = b >> 16;
-        }
+      if (op == I_NOISEG) {
+        const uint32_t G = opw >> 8;
         if (MODE == kReplay) {
-          const uint32_t rb = P.body_rbase + c;
-          const uint32_t m0 = draw(rb), m1 = draw(rb + 1);
-          L(pa) ^= m0;
-          L(pb) ^= m1;
-          if (op == I_NOISE2) {
-            const uint32_t m2 = draw(rb + 2), m3 = draw(rb + 3);
-            L(pc) ^= m2;
-            L(pd) ^= m3;
+          for (uint32_t i = 1; i <= G; ++i) {
+            const uint32_t pab = readlane(cA.y, k + i), pcd = readlane(cA.z, k + i);
+            const uint32_t rb = P.body_rbase + readlane(cA.w, k + i);
+            const uint32_t m0 = draw(rb), m1 = draw(rb + 1);
+            XR(pab & 0xFFFFu, m0);
+            XR(pab >> 16, m1);
+            if (readlane(cB.x, k + i) == 2) {
+              const uint32_t m2 = draw(rb + 2), m3 = draw(rb + 3);
+              XR(pcd & 0xFFFFu, m2);
+              XR(pcd >> 16, m3);
+            }
           }
-          continue;
+        } else if (P.nclasses > 0) {
+          group_events<MODE, GL>(ev0, a, G, k, cA, cB, F, P, blk, lane);
         }
-        const uint32_t d = readlane(cB.x, k);
-        const uint32_t cls = d & 0xFFu;
-        if (cls == kNoFire) continue;
-        SiteRef s;
-        s.kind = (d >> 8) & 0xFu;
-        s.ordinal = d >> 12;
-        s.accept = readlane(cB.y, k);
-        s.t1 = readlane(cB.z, k);
-        s.t2 = readlane(cB.w, k);
-        s.rslot = c;
-        site_events<0, MODE>(ev0, s, pa, pb, pc, pd, lds, P, blk, lane);
+        k += G;
         continue;
       }
       switch (op) {
         case I_MEAS: {
-          const uint32_t v = L(a);
+          const uint32_t v = RD(a);
           if (live) P.rec[(uint64_t)b * P.ld32 + goff] = v & lmask;
-          if (c != kNone) L(c) = v;
+          if (c != kNone) WR(c, v);
           break;
         }
         case I_XSTORE2: {
-          const uint32_t v = L(a) ^ L(b);
+          const uint32_t v = RD(a) ^ RD(b);
           if (live) P.rec[(uint64_t)c * P.ld32 + goff] = v & lmask;
           const uint32_t keep = readlane(cB.x, k);
-          if (keep != kNone) L(keep) = v;
+          if (keep != kNone) WR(keep, v);
           break;
         }
         case I_RESETZ: {
-          L(a) = 0u;
+          WR(a, 0u);
           const uint32_t r = draw(P.body_rbase + c);
-          L(b) ^= r;
+          XR(b, r);
           break;
         }
-        case I_ACC_LDS: acc = L(a); break;
-        case I_ACCX_LDS: acc ^= L(a); break;
+        case I_ACC_LDS: acc = RD(a); break;
+        case I_ACCX_LDS: acc ^= RD(a); break;
         case I_ACC_REC: acc = gload(P.rec, a); break;
         case I_ACCX_REC: acc ^= gload(P.rec, a); break;
         case I_ACC_STORE:
           if (live) P.rec[(uint64_t)a * P.ld32 + goff] = acc & lmask;
-          if (b != kNone) L(b) = acc;
+          if (b != kNone) WR(b, acc);
           break;
         case I_REC_ZERO:
           if (live) P.rec[(uint64_t)a * P.ld32 + goff] = 0u;
           break;
         case I_INIT: {
-          L(a) = 0u;
+          WR(a, 0u);
           const uint32_t r = draw(c);
-          L(b) = r;
+          WR(b, r);
           break;
         }
         case I_LOAD:
-          L(a) = gload(P.frame, b);
+          WR(a, gload(P.frame, b));
           break;
         case I_STORE:
-          if (live) P.frame[(uint64_t)b * P.ld32 + goff] = L(a) & lmask;
+          if (live) P.frame[(uint64_t)b * P.ld32 + goff] = RD(a) & lmask;
           break;
-        case I_ZERO: L(a) = 0u; break;
+        case I_ZERO: WR(a, 0u); break;
         case I_LIN1: {
-          const uint32_t x = L(a), z = L(b);
+          const uint32_t x = RD(a), z = RD(b);
           const uint32_t nx = ((c & 1u) ? x : 0u) ^ ((c & 2u) ? z : 0u);
           const uint32_t nz = ((c & 4u) ? x : 0u) ^ ((c & 8u) ? z : 0u);
-          L(a) = nx;
-          L(b) = nz;
+          WR(a, nx);
+          WR(b, nz);
           break;
         }
         default: break;
@@ -324,12 +335,14 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
   }
 }
 
-template <int MODE>
+template <int MODE, bool GL>
 __global__ __launch_bounds__(64) void pf_kernel(KParams P) {
   extern __shared__ uint32_t lds[];
   const uint32_t lane = threadIdx.x;
+  Planes<GL> F;
+  F.base = GL ? P.scratch + (uint64_t)blockIdx.x * P.sstride : lds;
   for (uint32_t blk = blockIdx.x; blk < P.nblocks; blk += gridDim.x)
-    run_tile<MODE>(P, blk, lds, lane);
+    run_tile<MODE, GL>(P, blk, F, lane);
 }
 
 // Persistent batched kernel: workgroups pull (circuit, tile range) items from
@@ -352,7 +365,13 @@ __global__ __launch_bounds__(64) void pf_batch_kernel(const KParams *params,
     if (it >= nitems) break;
     const WorkItem w = items[it];
     const KParams P = params[w.circuit];
-    for (uint32_t b = 0; b < w.nblk; ++b) run_tile<kNative>(P, w.blk0 + b, lds, lane);
+    if (P.scratch) {
+      Planes<true> F{P.scratch + (uint64_t)blockIdx.x * P.sstride};
+      for (uint32_t b = 0; b < w.nblk; ++b) run_tile<kNative, true>(P, w.blk0 + b, F, lane);
+    } else {
+      Planes<false> F{lds};
+      for (uint32_t b = 0; b < w.nblk; ++b) run_tile<kNative, false>(P, w.blk0 + b, F, lane);
+    }
   }
   if (lane == 0) {
     __threadfence();
@@ -457,6 +476,7 @@ int ensure_device_copy(qcf_program *P, int variant, const uint4 **prog,
     std::vector<Instr> code;
     const auto &pro = (variant & 1) ? P->prologue_fresh : P->prologue_inplace;
     code.insert(code.end(), pro.begin(), pro.end());
+    while (code.size() % kChunk) code.push_back(Instr{I_END, 0, 0, 0, 0, 0, 0, 0});
     code.insert(code.end(), P->body.begin(), P->body.end());
     if (variant & 2) code.insert(code.end(), P->epilogue_store.begin(), P->epilogue_store.end());
     if (code.empty()) code.push_back(Instr{I_END, 0, 0, 0, 0, 0, 0, 0});
@@ -489,10 +509,48 @@ int allow_lds(const void *fn, uint32_t lds_bytes) {
 
 template <int MODE>
 int launch_pf(const KParams &kp, uint32_t lds_bytes, uint32_t grid, hipStream_t s) {
-  QCF_TRY(allow_lds((const void *)pf_kernel<MODE>, lds_bytes));
-  hipLaunchKernelGGL(pf_kernel<MODE>, dim3(grid), dim3(64), lds_bytes, s, kp);
+  if (kp.scratch) {
+    hipLaunchKernelGGL((pf_kernel<MODE, true>), dim3(grid), dim3(64), 0, s, kp);
+  } else {
+    QCF_TRY(allow_lds((const void *)pf_kernel<MODE, false>, lds_bytes));
+    hipLaunchKernelGGL((pf_kernel<MODE, false>), dim3(grid), dim3(64), lds_bytes, s, kp);
+  }
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
+}
+
+// Frames too large for LDS live in a per-wave region of this device buffer
+// (grown on demand; only single-circuit launches use it, batches own theirs).
+constexpr uint32_t kGlobalWavesPerCU = 8;
+struct FrameScratch {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+int frame_scratch(size_t bytes, uint32_t **out) {
+  static std::mutex mu;
+  static std::vector<FrameScratch> per_dev;
+  int dev = 0;
+  QCF_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1);
+  FrameScratch &fs = per_dev[dev];
+  if (fs.bytes < bytes) {
+    if (fs.ptr) {
+      QCF_HIP_CHECK(hipDeviceSynchronize());
+      QCF_HIP_CHECK(hipFree(fs.ptr));
+      fs.ptr = nullptr;
+      fs.bytes = 0;
+    }
+    QCF_HIP_CHECK(hipMalloc(&fs.ptr, bytes));
+    fs.bytes = bytes;
+  }
+  *out = (uint32_t *)fs.ptr;
+  return QCF_OK;
+}
+
+bool force_global_frames() {
+  const char *e = getenv("QCF_FORCE_GLOBAL_FRAMES");
+  return e && e[0] == '1';
 }
 
 }  // namespace
@@ -519,15 +577,13 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
   int lds_max = 0;
   QCF_TRY(max_lds_per_block(&lds_max));
   const uint32_t lds_bytes = P->nplanes * 256u;
-  if ((int)lds_bytes > lds_max)
-    return set_error(QCF_ERR_UNSUPPORTED,
-                     "pftrajectories: circuit needs %u bytes of on-chip frame per tile (limit %d)",
-                     lds_bytes, lds_max);
+  const bool global = (int)lds_bytes > lds_max || force_global_frames();
   const int variant = (fresh ? 1 : 0) | (frame ? 2 : 0);
   KParams kp{};
   QCF_TRY(ensure_device_copy(P, variant, &kp.prog, &kp.ninstr));
   const uint64_t nw32 = (shots + 31) / 32;
   kp.nw32 = (uint32_t)nw32;
+  kp.nwbuf = (uint32_t)(2 * ((shots + 63) / 64));
   kp.nblocks = (uint32_t)((nw32 + 63) / 64);
   const uint32_t tail = (uint32_t)(shots % 32);
   kp.lastmask = tail ? ((1u << tail) - 1u) : 0xFFFFFFFFu;
@@ -540,11 +596,18 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
   kp.nclasses = (uint32_t)P->classes.size();
   for (uint32_t c = 0; c < kMaxClasses; ++c)
     kp.cls_c[c] = c < P->classes.size() ? P->classes[c].c : 0.f;
+  kp.nplanes = P->nplanes;
   DeviceState *st;
   QCF_TRY(device_state(&st));
   uint32_t wpc = waves_per_cu;
-  if (wpc == 0) wpc = std::max<uint32_t>(1, std::min<uint32_t>(16, (160u * 1024u) / std::max(1u, lds_bytes)));
+  if (wpc == 0)
+    wpc = global ? kGlobalWavesPerCU
+                 : std::max<uint32_t>(1, std::min<uint32_t>(16, (uint32_t)lds_max / std::max(1u, lds_bytes)));
   const uint64_t grid = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * wpc);
+  if (global) {
+    QCF_TRY(frame_scratch(grid * (size_t)P->nplanes * 256u, &kp.scratch));
+    kp.sstride = P->nplanes * 64u;
+  }
   if (masks && !exp) return launch_pf<kReplay>(kp, lds_bytes, (uint32_t)grid, stream);
   if (exp) return launch_pf<kExport>(kp, lds_bytes, (uint32_t)grid, stream);
   return launch_pf<kNative>(kp, lds_bytes, (uint32_t)grid, stream);
@@ -634,6 +697,7 @@ struct qcf_batch {
   qcf::KParams *d_params = nullptr;
   qcf::WorkItem *d_items = nullptr;
   uint32_t *d_queue = nullptr;
+  uint32_t *d_frames = nullptr;
   uint32_t lds_bytes = 0;
   uint32_t grid = 0;
   hipGraph_t graph = nullptr;
@@ -644,6 +708,7 @@ struct qcf_batch {
     if (d_params) (void)hipFree(d_params);
     if (d_items) (void)hipFree(d_items);
     if (d_queue) (void)hipFree(d_queue);
+    if (d_frames) (void)hipFree(d_frames);
   }
 };
 
@@ -661,7 +726,9 @@ int build_kparams(const qcf_program *prog, uint64_t shots, uint64_t seed,
   *kp = KParams{};
   QCF_TRY(ensure_device_copy(P, 1, &kp->prog, &kp->ninstr));
   const uint64_t nw32 = (shots + 31) / 32;
+  kp->nplanes = P->nplanes;
   kp->nw32 = (uint32_t)nw32;
+  kp->nwbuf = (uint32_t)(2 * ((shots + 63) / 64));
   kp->nblocks = (uint32_t)((nw32 + 63) / 64);
   const uint32_t tail = (uint32_t)(shots % 32);
   kp->lastmask = tail ? ((1u << tail) - 1u) : 0xFFFFFFFFu;
@@ -762,18 +829,40 @@ extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
       delete B;
       return set_error(QCF_ERR_ARGUMENT, "null program %zu", k);
     }
-    const uint32_t lb = progs[k]->nplanes * 256u;
-    if ((int)lb > lds_max) {
-      delete B;
-      return set_error(QCF_ERR_UNSUPPORTED,
-                       "circuit %zu needs %u bytes of on-chip frame per tile (limit %d)", k, lb,
-                       lds_max);
-    }
-    B->lds_bytes = std::max(B->lds_bytes, lb);
     if (int st = build_kparams(progs[k], shots[k], seeds[k], meas[k], ld[k], &B->params[k]);
         st != QCF_OK) {
       delete B;
       return st;
+    }
+  }
+  // circuits whose frame does not fit in LDS keep their planes in a
+  // batch-owned global buffer: one slot of the largest such frame per wave
+  const bool force_gl = force_global_frames();
+  uint32_t gl_planes = 0;
+  for (size_t k = 0; k < n; ++k) {
+    const uint32_t lb = progs[k]->nplanes * 256u;
+    if ((int)lb > lds_max || force_gl)
+      gl_planes = std::max(gl_planes, progs[k]->nplanes);
+    else
+      B->lds_bytes = std::max(B->lds_bytes, lb);
+  }
+  if (gl_planes) {
+    DeviceState *st;
+    if (int r = device_state(&st); r != QCF_OK) {
+      delete B;
+      return r;
+    }
+    const size_t slots = (size_t)st->cu_count * 16;  // the largest grid set_tune picks
+    if (hipMalloc(&B->d_frames, slots * gl_planes * 256u) != hipSuccess) {
+      delete B;
+      return set_error(QCF_ERR_ALLOC, "batch frame buffer (%zu bytes)", slots * gl_planes * 256u);
+    }
+    for (size_t k = 0; k < n; ++k) {
+      const uint32_t lb = progs[k]->nplanes * 256u;
+      if ((int)lb > lds_max || force_gl) {
+        B->params[k].scratch = B->d_frames;
+        B->params[k].sstride = gl_planes * 64u;
+      }
     }
   }
   hipError_t e = hipMalloc(&B->d_params, std::max<size_t>(1, n) * sizeof(KParams));

@@ -356,15 +356,18 @@ struct Compiler {
     for (size_t i = 0; i < P->sites.size(); ++i) {
       Site &s = P->sites[i];
       const double pe = site_pe[i];
-      if (!(pe > 0)) {
+      if (cls_p.empty()) {
         s.cls = kNoFire;
         continue;
       }
-      const size_t k = std::lower_bound(distinct.begin(), distinct.end(), pe) - distinct.begin();
-      const int c = bucket[k];
+      int c = 0;
+      if (pe > 0) {
+        const size_t k = std::lower_bound(distinct.begin(), distinct.end(), pe) - distinct.begin();
+        c = bucket[k];
+      }
       s.cls = (uint32_t)c;
       s.ordinal = ord[c]++;
-      const double r = pe / cls_p[c];
+      const double r = pe > 0 ? pe / cls_p[c] : 0.0;
       s.accept = r >= 1.0 ? (1u << 31) : (uint32_t)std::llround(r * 2147483648.0);
     }
     for (uint32_t n : ord)
@@ -375,18 +378,43 @@ struct Compiler {
     return QCF_OK;
   }
 
-  // Write each noise site's data into its instruction (class, ordinal,
-  // thinning and Pauli thresholds travel with the instruction stream).
-  void patch_sites() {
-    for (Instr &in : P->body) {
-      if (in.op != I_NOISE1 && in.op != I_NOISE2) continue;
-      const Site &s = P->sites[in.c];
-      in.c = s.rslot;
-      in.d = (s.cls & 0xFFu) | (s.kind << 8) | (s.ordinal << 12);
-      in.e = s.accept;
-      in.f = s.t1;
-      in.g = s.t2;
+  // Turn every run of consecutive noise instructions into groups: a header
+  // I_NOISEG followed by one I_SITE data entry per site, never straddling a
+  // 64-instruction chunk (NOPs pad to the next chunk when needed).  Pauli
+  // noise commutes with Pauli noise, so a run is one unit of work: the native
+  // sampler checks the events of all its sites at once.  The body is assumed
+  // to start on a chunk boundary (the prologues are padded).
+  void group_sites() {
+    std::vector<Instr> out;
+    out.reserve(P->body.size() * 2);
+    const auto &B = P->body;
+    size_t i = 0;
+    while (i < B.size()) {
+      if (B[i].op != I_NOISE1 && B[i].op != I_NOISE2) {
+        out.push_back(B[i++]);
+        continue;
+      }
+      size_t j = i;
+      while (j < B.size() && (B[j].op == I_NOISE1 || B[j].op == I_NOISE2) &&
+             j - i < kChunk - 1) {
+        // ordinals must stay consecutive inside a group
+        if (j > i && P->sites[B[j].c].ordinal != P->sites[B[j - 1].c].ordinal + 1) break;
+        ++j;
+      }
+      const uint32_t G = (uint32_t)(j - i);
+      const size_t pos = out.size() % kChunk;
+      if (pos + 1 + G > kChunk)
+        for (size_t k = pos; k < kChunk; ++k) out.push_back(Instr{I_END, 0, 0, 0, 0, 0, 0, 0});
+      out.push_back(Instr{I_NOISEG | (G << 8), P->sites[B[i].c].ordinal, 0, 0, 0, 0, 0, 0});
+      for (size_t k = i; k < j; ++k) {
+        const Site &s = P->sites[B[k].c];
+        const uint32_t pb = B[k].op == I_NOISE2 ? B[k].b : 0u;
+        const uint32_t pa = B[k].op == I_NOISE2 ? B[k].a : (B[k].a | (B[k].b << 16));
+        out.push_back(Instr{I_SITE, pa, pb, s.rslot, s.kind, s.accept, s.t1, s.t2});
+      }
+      i = j;
     }
+    P->body.swap(out);
   }
 
   // Peepholes: bundle runs of plane XORs into XORN (up to 7 pairs, a pair is
@@ -688,8 +716,8 @@ extern "C" int qcf_program_build(const qcf_op *ops, size_t nops,
     delete P;
     return st;
   }
-  C.patch_sites();
   C.peephole();
+  C.group_sites();
   P->nplanes = C.next_slot;
   if (P->nplanes > 0xFFFF) {
     delete P;

@@ -21,14 +21,20 @@ constexpr uint32_t kBlockShots = 2048;  // shots per wave tile (64 lanes x 32)
 constexpr uint32_t kMaxClasses = 1;     // noise-rate classes per program (others thinned)
 constexpr uint32_t kNoFire = 0xFFu;     // site class of a zero-rate site
 constexpr uint32_t kMaxOrdinals = 1u << 20;  // sites per class
+constexpr uint32_t kChunk = 64;          // instructions fetched per wave step
 
 enum InstrOp : uint32_t {
   I_END = 0,
   I_XORN,       // n = op>>8 (1..7) pairs in words a..g: P[lo16] ^= P[hi16]; all
                 // sources are read before any target is written (the host only
                 // bundles pairs where that equals sequential order)
-  I_NOISE1,     // planes x=a, z=b; c = replay plane; d..g = site (see below)
-  I_NOISE2,     // planes x1=a&0xffff z1=a>>16 x2=b&0xffff z2=b>>16; c, d..g as NOISE1
+  I_NOISE1,     // (compiler internal) single-qubit site c on planes x=a, z=b
+  I_NOISE2,     // (compiler internal) two-qubit site c on x1|z1<<16 = a, x2|z2<<16 = b
+  I_NOISEG,     // G = op>>8 noise sites with consecutive ordinals starting at a;
+                // their data are the next G instructions (I_SITE), always in
+                // the same 64-instruction chunk
+  I_SITE,       // site data: a = x1|z1<<16, b = x2|z2<<16, c = replay plane,
+                // d = kind (1 or 2), e = accept, f = t1, g = t2
   I_MEAS,       // v = P[a]; rec[b] = v; if c != none: P[c] = v
   I_XSTORE2,    // v = P[a] ^ P[b]; rec[c] = v; if d != none: P[d] = v
   I_RESETZ,     // P[a] = 0; P[b] ^= draw(c)           (sMRZ reset part)
@@ -45,9 +51,8 @@ enum InstrOp : uint32_t {
   I_LIN1,       // (P[a],P[b]) := linear map c (bit0 x<-x, bit1 x<-z, bit2 z<-x, bit3 z<-z)
 };
 
-// 32-byte device instruction.  Noise instructions carry their site inline:
-//   d = class | kind << 8 | ordinal << 12,  e = accept,  f = t1,  g = t2
-// (thresholds are 2^31-scaled and compared with the top 31 bits of a draw).
+// 32-byte device instruction.  Thresholds of I_SITE are 2^31-scaled and
+// compared with the top 31 bits of a draw.
 struct Instr {
   uint32_t op, a, b, c, d, e, f, g;
 };

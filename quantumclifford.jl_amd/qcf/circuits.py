@@ -154,6 +154,7 @@ def rotated_surface_code(d: int = 5, rounds: int | None = None, p: float = 1e-3,
             ops.append(sHadamard(q))
         dep1(xq)
         for layer in range(4):
+            pairs = []
             for m in meas:
                 isx = m in xm
                 dx, dy = (x_order if isx else z_order)[layer]
@@ -162,7 +163,11 @@ def rotated_surface_code(d: int = 5, rounds: int | None = None, p: float = 1e-3,
                     continue
                 c, t = (idx[m], idx[nb]) if isx else (idx[nb], idx[m])
                 ops.append(sCNOT(c, t))
-                if noise:
+                pairs.append((c, t))
+            # the layer's CNOTs act on disjoint qubits, so the depolarizing
+            # channel after each of them may follow the whole layer
+            if noise:
+                for c, t in pairs:
                     ops.append(Depolarize2(c, t, p))
         for q in xq:
             ops.append(sHadamard(q))

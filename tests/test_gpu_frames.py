@@ -284,3 +284,41 @@ def test_batched_equals_single_launches(gpu):
         single = qcf.pftrajectories(p, s, seed=sd, keep_frame=False).record_host()
         diff = np.argwhere(o != single)
         assert len(diff) == 0, (k, p.nq, p.info.lds_bytes, s, len(diff), diff[:5].tolist())
+
+
+def test_global_frame_storage_matches_lds(gpu, monkeypatch):
+    """Frames too large for LDS run from global memory; forcing that path on
+    circuits that fit must give the same bits."""
+    c = circuits.rotated_surface_code(5, 5, 3e-3)
+    prog = qcf.Program(c.ops)
+    a = qcf.pftrajectories(prog, 20000, seed=99)
+    monkeypatch.setenv("QCF_FORCE_GLOBAL_FRAMES", "1")
+    b = qcf.pftrajectories(prog, 20000, seed=99)
+    torch.cuda.synchronize()
+    assert np.array_equal(a.record_host(), b.record_host())
+    assert np.array_equal(a.frame_host(), b.frame_host())
+
+
+def test_large_circuit_global_frames_export_replay(gpu):
+    """d=13 needs more than the 160 KB of LDS per tile: global-memory planes."""
+    c = circuits.rotated_surface_code(13, 2, 5e-3)
+    prog = qcf.Program(c.ops)
+    assert prog.info.lds_bytes > 160 * 1024
+    prog, f, gm, om, gf, of, mk = fresh_export_and_replay(c.ops, 2100, seed=5)
+    assert np.array_equal(gm, om[: prog.nbits])
+    assert np.array_equal(gf, of)
+
+
+def test_batched_with_large_circuits(gpu):
+    rng = np.random.default_rng(8)
+    progs, shots, seeds = [], [], []
+    for d in (3, 15, 5, 13, 11, 3):
+        c = circuits.rotated_surface_code(d, d, float(10 ** rng.uniform(-4, -2)))
+        progs.append(qcf.Program(c.ops))
+        shots.append(int(10 ** rng.uniform(3, 4)))
+        seeds.append(int(rng.integers(0, 2**62)))
+    from qcf import batch
+    outs = batch.sample_batched(progs, shots, seeds, graph=True)
+    for k, (p, s, sd, o) in enumerate(zip(progs, shots, seeds, outs)):
+        single = qcf.pftrajectories(p, s, seed=sd, keep_frame=False).record_host()
+        assert np.array_equal(o, single), k
