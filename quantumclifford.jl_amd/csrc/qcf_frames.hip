@@ -235,25 +235,36 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
     for (uint32_t k = 0; k < cnt; ++k) {
       const uint32_t opw = readlane(cA.x, k);
       const uint32_t op = opw & 0xFFu;
-      const uint32_t a = readlane(cA.y, k);
-      const uint32_t b = readlane(cA.z, k);
-      const uint32_t c = readlane(cA.w, k);
+      // hot operations first (plane XOR bundles, noise groups, measure-reset)
       if (op == I_XORN) {
         const uint32_t n = opw >> 8;
-        uint32_t w[7] = {a, b, c, 0, 0, 0, 0};
-        if (n > 3) {
-          w[3] = readlane(cB.x, k);
-          w[4] = readlane(cB.y, k);
-          w[5] = readlane(cB.z, k);
-          w[6] = readlane(cB.w, k);
-        }
+        uint32_t w[7];
+        w[0] = readlane(cA.y, k);
+        w[1] = n > 1 ? readlane(cA.z, k) : 0u;
+        w[2] = n > 2 ? readlane(cA.w, k) : 0u;
+        w[3] = n > 3 ? readlane(cB.x, k) : 0u;
+        w[4] = n > 4 ? readlane(cB.y, k) : 0u;
+        w[5] = n > 5 ? readlane(cB.z, k) : 0u;
+        w[6] = n > 6 ? readlane(cB.w, k) : 0u;
         uint32_t v[7];
-#pragma unroll
-        for (int i = 0; i < 7; ++i)
-          if ((uint32_t)i < n) v[i] = RD(w[i] >> 16);
-#pragma unroll
-        for (int i = 0; i < 7; ++i)
-          if ((uint32_t)i < n) XR(w[i] & 0xFFFFu, v[i]);
+        switch (n) {  // all sources are read before any target is written
+          case 7: v[6] = RD(w[6] >> 16); [[fallthrough]];
+          case 6: v[5] = RD(w[5] >> 16); [[fallthrough]];
+          case 5: v[4] = RD(w[4] >> 16); [[fallthrough]];
+          case 4: v[3] = RD(w[3] >> 16); [[fallthrough]];
+          case 3: v[2] = RD(w[2] >> 16); [[fallthrough]];
+          case 2: v[1] = RD(w[1] >> 16); [[fallthrough]];
+          default: v[0] = RD(w[0] >> 16);
+        }
+        switch (n) {
+          case 7: XR(w[6] & 0xFFFFu, v[6]); [[fallthrough]];
+          case 6: XR(w[5] & 0xFFFFu, v[5]); [[fallthrough]];
+          case 5: XR(w[4] & 0xFFFFu, v[4]); [[fallthrough]];
+          case 4: XR(w[3] & 0xFFFFu, v[3]); [[fallthrough]];
+          case 3: XR(w[2] & 0xFFFFu, v[2]); [[fallthrough]];
+          case 2: XR(w[1] & 0xFFFFu, v[1]); [[fallthrough]];
+          default: XR(w[0] & 0xFFFFu, v[0]);
+        }
         continue;
       }
       if (op == I_NOISEG) {
@@ -272,23 +283,42 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
             }
           }
         } else if (P.nclasses > 0) {
-          group_events<MODE, GL>(ev0, a, G, k, cA, cB, F, P, blk, lane);
+          group_events<MODE, GL>(ev0, readlane(cA.y, k), G, k, cA, cB, F, P, blk, lane);
         }
         k += G;
         continue;
       }
+      const uint32_t a = readlane(cA.y, k);
+      if (op == I_MR || op == I_MRX) {
+        const uint32_t b = readlane(cA.z, k), c = readlane(cA.w, k);
+        const uint32_t col = readlane(cB.x, k), keep = readlane(cB.y, k);
+        const uint32_t v = RD(a);
+        if (col != kNone && live) P.rec[(uint64_t)col * P.ld32 + goff] = v & lmask;
+        if (op == I_MRX) {
+          const uint32_t other = readlane(cB.z, k), dcol = readlane(cB.w, k);
+          const uint32_t det = v ^ RD(other);
+          if (live) P.rec[(uint64_t)dcol * P.ld32 + goff] = det & lmask;
+        }
+        if (keep != kNone) WR(keep, v);
+        WR(a, 0u);
+        XR(b, draw(P.body_rbase + c));
+        continue;
+      }
+      const uint32_t b = readlane(cA.z, k);
+      if (op == I_XSTORE2) {
+        const uint32_t c = readlane(cA.w, k);
+        const uint32_t v = RD(a) ^ RD(b);
+        if (live) P.rec[(uint64_t)c * P.ld32 + goff] = v & lmask;
+        const uint32_t keep = readlane(cB.x, k);
+        if (keep != kNone) WR(keep, v);
+        continue;
+      }
+      const uint32_t c = readlane(cA.w, k);
       switch (op) {
         case I_MEAS: {
           const uint32_t v = RD(a);
           if (live) P.rec[(uint64_t)b * P.ld32 + goff] = v & lmask;
           if (c != kNone) WR(c, v);
-          break;
-        }
-        case I_XSTORE2: {
-          const uint32_t v = RD(a) ^ RD(b);
-          if (live) P.rec[(uint64_t)c * P.ld32 + goff] = v & lmask;
-          const uint32_t keep = readlane(cB.x, k);
-          if (keep != kNone) WR(keep, v);
           break;
         }
         case I_RESETZ: {

@@ -460,6 +460,38 @@ struct Compiler {
     }
     flush();
     P->body.swap(out);
+    // measure-and-reset: MEAS(x) + RESETZ(x, z) -> MR; a lone RESETZ -> MR
+    // without a record; MR followed by the XOR of its value with another
+    // kept bit (a detector against the previous round) -> MRX
+    out.clear();
+    const auto &C2 = P->body;
+    for (size_t i = 0; i < C2.size(); ++i) {
+      const Instr &in = C2[i];
+      Instr mr{};
+      bool have = false;
+      if (in.op == I_MEAS && i + 1 < C2.size() && C2[i + 1].op == I_RESETZ && C2[i + 1].a == in.a) {
+        mr = Instr{I_MR, in.a, C2[i + 1].b, C2[i + 1].c, in.b, in.c, kNone, kNone};
+        have = true;
+        i += 1;
+      } else if (in.op == I_RESETZ) {
+        mr = Instr{I_MR, in.a, in.b, in.c, kNone, kNone, kNone, kNone};
+        have = true;
+      }
+      if (!have) {
+        out.push_back(in);
+        continue;
+      }
+      if (mr.e != kNone && i + 1 < C2.size() && C2[i + 1].op == I_XSTORE2 && C2[i + 1].d == kNone &&
+          (C2[i + 1].a == mr.e || C2[i + 1].b == mr.e) && C2[i + 1].a != C2[i + 1].b) {
+        const Instr &x = C2[i + 1];
+        mr.op = I_MRX;
+        mr.f = x.a == mr.e ? x.b : x.a;
+        mr.g = x.c;
+        i += 1;
+      }
+      out.push_back(mr);
+    }
+    P->body.swap(out);
   }
 };
 

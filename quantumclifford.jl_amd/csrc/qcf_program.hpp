@@ -49,6 +49,9 @@ enum InstrOp : uint32_t {
   I_STORE,      // frame[b] = P[a]
   I_ZERO,       // P[a] = 0
   I_LIN1,       // (P[a],P[b]) := linear map c (bit0 x<-x, bit1 x<-z, bit2 z<-x, bit3 z<-z)
+  I_MR,         // sMRZ: v = P[a]; if d != none: rec[d] = v; if e != none: P[e] = v;
+                //       P[a] = 0; P[b] ^= draw(c)
+  I_MRX,        // I_MR, then the detector rec[g] = v ^ P[f]
 };
 
 // 32-byte device instruction.  Thresholds of I_SITE are 2^31-scaled and
