@@ -267,6 +267,11 @@ int qcf_autotune(const qcf_program *const *progs, size_t n,
 int qcf_debug_philox(const uint32_t *in, uint32_t *out, size_t n,
                      void *stream);
 
+/* Inclusive prefix sums over each group of 64 words of `in`, computed by the
+ * wave-wide scan the sampler uses to place noise events (self-check). */
+int qcf_debug_wave_scan(const uint32_t *in, uint32_t *out, size_t nwaves,
+                        void *stream);
+
 /* ---- reference-layout converters (to_gpu / to_cpu counterparts,
  *      ext/QuantumCliffordGPUExt/adapters.jl:7-41) --------------------- */
 

@@ -64,6 +64,20 @@ __device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t k) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k);
 }
 
+// Inclusive prefix sum over the 64 lanes with DPP moves only (no LDS round
+// trips): shifts by 1, 2, 4, 8 inside each row of 16 lanes, then row 15 -> next
+// row and lane 31 -> upper half broadcasts.  Lanes without a source read 0.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+  int v = (int)x;
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return (uint32_t)v;
+}
+
 template <int C>
 __device__ __forceinline__ void gen_batch(EvState &ev, const KParams &P,
                                           uint32_t blk, uint32_t lane) {
@@ -82,12 +96,7 @@ __device__ __forceinline__ void gen_batch(EvState &ev, const KParams &P,
     incr = (uint32_t)g + 1u;
     null_ev = false;
   }
-  uint32_t v = incr;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t t = (uint32_t)__shfl_up((int)v, d, 64);
-    if ((int)lane >= d) v += t;
-  }
+  const uint32_t v = wave_inclusive_scan(incr);
   ev.pos = ev.last + v;
   ev.last = readlane(ev.pos, 63);
   ev.upau = r.y;
@@ -222,15 +231,16 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
   auto WR = [&](uint32_t p, uint32_t v) { F.wr(p, lane, v); };
   auto XR = [&](uint32_t p, uint32_t v) { F.xr(p, lane, v); };
   uint32_t acc = 0;
-  const uint4 zero4 = make_uint4(0, 0, 0, 0);
-  uint4 nA = lane < P.ninstr ? prog4[2 * lane] : zero4;
-  uint4 nB = lane < P.ninstr ? prog4[2 * lane + 1] : zero4;
+  // lanes past the end re-read the last instruction (never executed)
+  const uint32_t last = P.ninstr - 1;
+  uint4 nA = prog4[2 * min(lane, last)];
+  uint4 nB = prog4[2 * min(lane, last) + 1];
   for (uint32_t base = 0; base < P.ninstr; base += 64u) {
     QCF_WAIT_VMEM();
     const uint4 cA = nA, cB = nB;
     const uint32_t nb = base + 64u + lane;
-    nA = nb < P.ninstr ? prog4[2 * nb] : zero4;
-    nB = nb < P.ninstr ? prog4[2 * nb + 1] : zero4;
+    nA = prog4[2 * min(nb, last)];
+    nB = prog4[2 * min(nb, last) + 1];
     const uint32_t cnt = min(64u, P.ninstr - base);
     for (uint32_t k = 0; k < cnt; ++k) {
       const uint32_t opw = readlane(cA.x, k);
@@ -423,6 +433,12 @@ __global__ void philox_kernel(const uint32_t *in, uint32_t *out, size_t n) {
   out[4 * i + 1] = r.y;
   out[4 * i + 2] = r.z;
   out[4 * i + 3] = r.w;
+}
+
+// ---- wave scan self-check kernel (one wave per 64 inputs)
+__global__ __launch_bounds__(64) void scan_kernel(const uint32_t *in, uint32_t *out) {
+  const uint32_t i = blockIdx.x * 64u + threadIdx.x;
+  out[i] = wave_inclusive_scan(in[i]);
 }
 
 // ---- reference-layout converters
@@ -663,6 +679,16 @@ extern "C" int qcf_debug_philox(const uint32_t *in, uint32_t *out, size_t n,
   if (n == 0) return QCF_OK;
   hipLaunchKernelGGL(philox_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, in, out, n);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+extern "C" int qcf_debug_wave_scan(const uint32_t *in, uint32_t *out, size_t nwaves,
+                                   void *stream) {
+  clear_error();
+  if (nwaves == 0) return QCF_OK;
+  hipLaunchKernelGGL(scan_kernel, dim3((unsigned)nwaves), dim3(64), 0, (hipStream_t)stream, in,
+                     out);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
 }

@@ -227,7 +227,7 @@ int launch_vec(uint64_t *dest, const uint64_t *src, size_t nw, int phases,
   const size_t items = aligned ? nw / 2 : nw;
   const int threads = 256;
   size_t blocks = (items + threads - 1) / threads;
-  size_t per_cu = 8;  // 256-thread workgroups per CU
+  size_t per_cu = 2;  // 256-thread workgroups per CU (measured best at 1 Gqubit)
   if (const char *e = getenv("QCF_MUL_BLOCKS_PER_CU")) per_cu = std::max(1, atoi(e));
   const size_t cap = (size_t)st->cu_count * per_cu;
   if (blocks > cap) blocks = cap;

@@ -84,6 +84,7 @@ _SIGNATURES = {
     "qcf_pf_sample_batched": (_I, [_P, _SZ, _P, _P, _P, _P, C.POINTER(Tune), _P]),
     "qcf_autotune": (_I, [_P, _SZ, _P, C.POINTER(Tune), _P]),
     "qcf_debug_philox": (_I, [_P, _P, _SZ, _P]),
+    "qcf_debug_wave_scan": (_I, [_P, _P, _SZ, _P]),
     "qcf_record_to_bool": (_I, [_P, _SZ, _U64, _U32, _P, _P]),
     "qcf_record_from_bool": (_I, [_P, _U64, _U32, _P, _SZ, _P]),
     "qcf_frame_to_reference": (_I, [_P, _SZ, _U64, _U32, _P, _P]),

@@ -322,3 +322,17 @@ def test_batched_with_large_circuits(gpu):
     for k, (p, s, sd, o) in enumerate(zip(progs, shots, seeds, outs)):
         single = qcf.pftrajectories(p, s, seed=sd, keep_frame=False).record_host()
         assert np.array_equal(o, single), k
+
+
+def test_wave_scan_is_exact(gpu):
+    """The DPP prefix sum that places noise events equals numpy's cumsum."""
+    rng = np.random.default_rng(1)
+    x = rng.integers(0, 1 << 24, (97, 64), dtype=np.uint32)
+    x[0] = 1
+    x[1] = np.arange(64)
+    inp = torch.from_numpy(x.view(np.int32).reshape(-1)).cuda()
+    out = torch.zeros_like(inp)
+    qcf._lib.call("qcf_debug_wave_scan", qcf.pauli._ptr(inp), qcf.pauli._ptr(out), 97,
+                  qcf.pauli._stream())
+    got = out.cpu().numpy().view(np.uint32).reshape(97, 64)
+    assert np.array_equal(got, np.cumsum(x.astype(np.uint64), axis=1).astype(np.uint32))
