@@ -267,6 +267,14 @@ int qcf_autotune(const qcf_program *const *progs, size_t n,
 int qcf_debug_philox(const uint32_t *in, uint32_t *out, size_t n,
                      void *stream);
 
+/* Source of the kernel specialised to `prog` for a launch variant (bit 0:
+ * fresh frame, bit 1: frame stored at the end) and its run-time compilation
+ * with hiprtc, without loading it on a device (self-check; needs no GPU). */
+int qcf_debug_jit_source(const qcf_program *prog, int variant, char *buf,
+                         size_t cap, size_t *needed);
+int qcf_debug_jit_compile(const qcf_program *prog, int variant,
+                          size_t *code_bytes);
+
 /* Inclusive prefix sums over each group of 64 words of `in`, computed by the
  * wave-wide scan the sampler uses to place noise events (self-check). */
 int qcf_debug_wave_scan(const uint32_t *in, uint32_t *out, size_t nwaves,

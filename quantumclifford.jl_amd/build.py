@@ -30,9 +30,24 @@ def _sources():
                   if f.endswith((".hip", ".cpp")))
 
 
+GEN = os.path.join(BUILD, "gen")
+
+
+def _embed_device_header() -> None:
+    """qcf_device.h as a C++ raw string literal, for run-time compilation."""
+    os.makedirs(GEN, exist_ok=True)
+    text = open(os.path.join(CSRC, "qcf_device.h")).read()
+    assert ")QCFDEV\"" not in text
+    out = 'R"QCFDEV(' + text + ')QCFDEV"\n'
+    path = os.path.join(GEN, "qcf_device_src.inc")
+    if not os.path.exists(path) or open(path).read() != out:
+        with open(path, "w") as f:
+            f.write(out)
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE_FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE_FLAGS, f"-I{GEN}", "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr}")
@@ -41,11 +56,13 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
+    _embed_device_header()
     srcs = _sources()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+           "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

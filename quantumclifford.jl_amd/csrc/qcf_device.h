@@ -1,0 +1,192 @@
+// Device code shared by the frame-sampler interpreter (qcf_frames.hip) and the
+// per-circuit kernels compiled at run time with hiprtc (qcf_jit.cpp): the
+// counter-based RNG, the wave prefix sum and the noise-event pass.  Both
+// paths include this one file, so for the same (seed, shot) they draw the
+// same events and produce the same bits.  The file has no includes so that
+// hiprtc can compile it as is (QCF_RTC is defined there).
+#ifndef QCF_DEVICE_H
+#define QCF_DEVICE_H
+
+#ifdef QCF_RTC
+typedef unsigned int uint32_t;
+typedef int int32_t;
+typedef unsigned long long uint64_t;
+#endif
+
+namespace qcfd {
+
+constexpr uint32_t kTileShots = 2048;  // shots per wave tile: 64 lanes x 32
+
+struct u4 {
+  uint32_t x, y, z, w;
+};
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of the 4x32
+// multiply-xorshift round with the Weyl key schedule.
+__device__ __forceinline__ u4 philox(u4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    c = u4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// Counter word 3 separates the gauge stream from the event stream.
+constexpr uint32_t kStreamGauge = 0x47415547u;
+constexpr uint32_t kStreamEvent = 0x45564E54u;
+
+// xoshiro128** (Blackman & Vigna 2021), one per lane, seeded from Philox.
+struct Xoshiro {
+  uint32_t s0, s1, s2, s3;
+  __device__ __forceinline__ uint32_t next() {
+    const uint32_t m = s1 * 5u;
+    const uint32_t r = ((m << 7) | (m >> 25)) * 9u;
+    const uint32_t t = s1 << 9;
+    s2 ^= s0;
+    s3 ^= s1;
+    s1 ^= s2;
+    s0 ^= s3;
+    s2 ^= t;
+    s3 = (s3 << 11) | (s3 >> 21);
+    return r;
+  }
+};
+
+// Gauge stream of one 32-shot word (initZ! and the sMRZ z-flips).
+__device__ __forceinline__ Xoshiro gauge_stream(uint64_t seed, uint32_t word) {
+  const u4 r = philox(u4{word, 0u, 0u, kStreamGauge}, (uint32_t)seed, (uint32_t)(seed >> 32));
+  Xoshiro g{r.x, r.y, r.z, r.w | 1u};
+  return g;
+}
+
+__device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t k) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k);
+}
+
+__device__ __forceinline__ uint32_t bperm(uint32_t src_lane, uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
+}
+
+// Inclusive prefix sum over the 64 lanes with DPP moves only: shifts by
+// 1, 2, 4, 8 inside each row of 16 lanes, then lane 15 -> next row and lane 31
+// -> upper half.  Lanes without a source read 0.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+  int v = (int)x;
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return (uint32_t)v;
+}
+
+// Noise events of one tile.  Events occur at rate p (one class for the whole
+// program) along the sequence (site ordinal, shot in tile): position
+// ordinal*2048 + shot.  Lane i holds the i-th event of the current batch of 64.
+struct EvState {
+  uint32_t pos;    // event position (per lane)
+  uint32_t upau;   // uniform for the Pauli choice
+  uint32_t uacc;   // 31-bit uniform for thinning; 0xFFFFFFFF = null event
+  uint32_t last;   // position held by lane 63 (wave uniform)
+  uint32_t batch;  // batches drawn so far (wave uniform)
+};
+
+__device__ __forceinline__ EvState ev_init() {
+  EvState e{0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u};
+  return e;
+}
+
+// Draw the next 64 events: one geometric gap per lane (failures before the
+// next event, floor(log2(u) * c) with c = 1/log2(1-p)) and a wave prefix sum.
+// Gaps beyond 2^24 trials advance without an event (memorylessness keeps the
+// process exact).  Specialised kernels keep one out-of-line copy of it.
+#ifdef QCF_RTC
+#define QCF_GEN_BATCH_ATTR __attribute__((noinline))
+#else
+#define QCF_GEN_BATCH_ATTR __forceinline__
+#endif
+__device__ QCF_GEN_BATCH_ATTR EvState gen_batch(EvState ev, uint64_t seed, float c,
+                                                uint32_t tile, uint32_t lane) {
+  const u4 r = philox(u4{tile, ev.batch, lane, kStreamEvent}, (uint32_t)seed,
+                      (uint32_t)(seed >> 32));
+  const float uf = ((float)(r.x >> 8) + 0.5f) * (1.0f / 16777216.0f);
+  const float g = __log2f(uf) * c;
+  uint32_t incr;
+  bool null_ev;
+  if (!(g < 16777216.0f)) {
+    incr = 1u << 24;
+    null_ev = true;
+  } else {
+    incr = (uint32_t)g + 1u;
+    null_ev = false;
+  }
+  ev.pos = ev.last + wave_inclusive_scan(incr);
+  ev.last = readlane(ev.pos, 63);
+  ev.upau = r.y;
+  ev.uacc = null_ev ? 0xFFFFFFFFu : (r.z >> 1);
+  ev.batch += 1;
+  return ev;
+}
+
+// Site entry (one per noise site, 4 words):
+//   w0 = x1 | z1 << 16        planes of the (first) qubit
+//   w1 = x2 | z2 << 16        planes of the second qubit, 0 for one qubit
+//   w2 = t1 | t2 << 16        2^15-scaled thresholds: X if u < t1, Z if
+//                             u < t2, else Y (u = top 15 bits of the draw)
+//   w3 = accept               2^31-scaled thinning threshold
+// Returns the flip mask over (x1, z1, x2, z2).
+__device__ __forceinline__ uint32_t event_flips(uint32_t u, uint32_t w1, uint32_t w2) {
+  if (w1 == 0u) {
+    const uint32_t u15 = u >> 17, t1 = w2 & 0xFFFFu, t2 = w2 >> 16;
+    return (u15 < t1 || u15 >= t2 ? 1u : 0u) | (u15 >= t1 ? 2u : 0u);
+  }
+  // uniform over the 15 non-identity two-qubit Paulis; the first qubit is the
+  // least significant base-4 digit, 1=X 2=Z 3=Y
+  return __umulhi(u, 15u) + 1u;
+}
+
+// Event pass of one noise group: G sites with ordinals ord0 .. ord0+G-1 whose
+// entries sit in lanes base .. base+G-1 of (e0..e3); `pay` is carried along
+// to the flip callback (the interpreter passes the replay plane there).
+// flip(plane, target_lane, bit, j, payload) is called for every flipped plane
+// j in 0..3 of an event.  ds_bpermute runs with all lanes active.
+template <class Flip>
+__device__ __forceinline__ void group_events(EvState &ev, uint32_t ord0, uint32_t G,
+                                             uint32_t base, uint32_t e0, uint32_t e1,
+                                             uint32_t e2, uint32_t e3, uint32_t pay,
+                                             uint64_t seed, float c, uint32_t tile,
+                                             uint32_t lane, Flip flip) {
+  const uint32_t lo = ord0 * kTileShots;
+  const uint32_t span = G * kTileShots;
+  while (true) {
+    const uint32_t rel = ev.pos - lo;
+    const bool inr = rel < span;
+    if (__builtin_amdgcn_ballot_w64(inr && ev.uacc != 0xFFFFFFFFu) != 0) {
+      const uint32_t src = base + (inr ? (rel >> 11) : 0u);
+      const uint32_t w0 = bperm(src, e0), w1 = bperm(src, e1);
+      const uint32_t w2 = bperm(src, e2), w3 = bperm(src, e3);
+      const uint32_t py = bperm(src, pay);
+      if (inr && ev.uacc < w3) {
+        const uint32_t within = rel & (kTileShots - 1u);
+        const uint32_t tl = within >> 5;
+        const uint32_t bit = 1u << (within & 31u);
+        const uint32_t fl = event_flips(ev.upau, w1, w2);
+        if (fl & 1u) flip(w0 & 0xFFFFu, tl, bit, 0u, py);
+        if (fl & 2u) flip(w0 >> 16, tl, bit, 1u, py);
+        if (fl & 4u) flip(w1 & 0xFFFFu, tl, bit, 2u, py);
+        if (fl & 8u) flip(w1 >> 16, tl, bit, 3u, py);
+      }
+    }
+    if (ev.last >= lo + span) break;
+    ev = gen_batch(ev, seed, c, tile, lane);
+  }
+}
+
+}  // namespace qcfd
+
+#endif  // QCF_DEVICE_H

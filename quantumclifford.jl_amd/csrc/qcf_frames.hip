@@ -25,6 +25,8 @@
 #include <mutex>
 
 #include "qcf_common.hpp"
+#include "qcf_device.h"
+#include "qcf_jit.hpp"
 #include "qcf_program.hpp"
 
 namespace qcf {
@@ -35,7 +37,8 @@ enum Mode : int { kNative = 0, kReplay = 1, kExport = 2 };
 struct KParams {
   const uint4 *prog;   // 2 uint4 per instruction
   uint32_t ninstr;
-  uint32_t nblocks;    // 2048-shot tiles
+  uint32_t nblocks;    // 2048-shot tiles (last tile index + 1)
+  uint32_t blk0;       // first tile of this launch
   uint32_t nw32;       // valid 32-bit shot words
   uint32_t nwbuf;      // 32-bit words up to the end of the last 64-bit word
   uint32_t lastmask;   // valid bits of word nw32-1
@@ -52,57 +55,8 @@ struct KParams {
   float cls_c[kMaxClasses];
 };
 
-struct EvState {
-  uint32_t pos;    // event position along the class sequence (per lane)
-  uint32_t upau;   // uniform for the Pauli choice
-  uint32_t uacc;   // 31-bit uniform for thinning; 0xFFFFFFFF = null event
-  uint32_t last;   // position held by lane 63 (wave uniform)
-  uint32_t batch;  // batches drawn so far (wave uniform)
-};
-
-__device__ __forceinline__ uint32_t readlane(uint32_t v, uint32_t k) {
-  return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k);
-}
-
-// Inclusive prefix sum over the 64 lanes with DPP moves only (no LDS round
-// trips): shifts by 1, 2, 4, 8 inside each row of 16 lanes, then row 15 -> next
-// row and lane 31 -> upper half broadcasts.  Lanes without a source read 0.
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
-  int v = (int)x;
-  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
-  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
-  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
-  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
-  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
-  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
-  return (uint32_t)v;
-}
-
-template <int C>
-__device__ __forceinline__ void gen_batch(EvState &ev, const KParams &P,
-                                          uint32_t blk, uint32_t lane) {
-  const u32x4 r = philox4x32_10(
-      u32x4{blk, ev.batch, ((uint32_t)C << 8) | lane, kStreamEvent},
-      (uint32_t)P.seed, (uint32_t)(P.seed >> 32));
-  // geometric gap: failures before the next event, g = floor(log2(u)/log2(1-p))
-  const float uf = ((float)(r.x >> 8) + 0.5f) * (1.0f / 16777216.0f);
-  const float g = __log2f(uf) * P.cls_c[C];
-  uint32_t incr;
-  bool null_ev;
-  if (!(g < 16777216.0f)) {  // gap beyond 2^24 trials: advance without an event
-    incr = 1u << 24;
-    null_ev = true;
-  } else {
-    incr = (uint32_t)g + 1u;
-    null_ev = false;
-  }
-  const uint32_t v = wave_inclusive_scan(incr);
-  ev.pos = ev.last + v;
-  ev.last = readlane(ev.pos, 63);
-  ev.upau = r.y;
-  ev.uacc = null_ev ? 0xFFFFFFFFu : (r.z >> 1);
-  ev.batch += 1;
-}
+using qcfd::EvState;
+using qcfd::readlane;
 
 template <int MODE>
 __device__ __forceinline__ void export_flip(const KParams &P, uint32_t plane,
@@ -134,67 +88,6 @@ struct Planes {
   }
 };
 
-__device__ __forceinline__ uint32_t bperm(uint32_t src_lane, uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
-}
-
-// Native events of one noise group: G sites with consecutive ordinals from
-// `ord0`, whose data sit in lanes k+1 .. k+G of the current chunk (cA, cB).
-// Every event lane fetches its site's data with ds_bpermute (executed with all
-// lanes active, as the instruction requires) and flips the target shot's bits
-// with LDS atomics.
-template <int MODE, bool GL>
-__device__ __forceinline__ void group_events(EvState &ev, uint32_t ord0, uint32_t G,
-                                             uint32_t k, const uint4 &cA, const uint4 &cB,
-                                             const Planes<GL> &F, const KParams &P,
-                                             uint32_t blk, uint32_t lane) {
-  const uint32_t lo = ord0 * kBlockShots;
-  const uint32_t span = G * kBlockShots;
-  while (true) {
-    const uint32_t rel = ev.pos - lo;
-    const bool inr = rel < span;
-    if (__builtin_amdgcn_ballot_w64(inr && ev.uacc != 0xFFFFFFFFu) != 0) {
-      const uint32_t site = inr ? (rel >> 11) : 0u;
-      const uint32_t src = k + 1 + site;
-      const uint32_t pab = bperm(src, cA.y);
-      const uint32_t pcd = bperm(src, cA.z);
-      const uint32_t rs = bperm(src, cA.w);
-      const uint32_t kind = bperm(src, cB.x);
-      const uint32_t acc = bperm(src, cB.y);
-      const uint32_t t1 = bperm(src, cB.z);
-      const uint32_t t2 = bperm(src, cB.w);
-      if (inr && ev.uacc < acc) {
-        const uint32_t within = rel & (kBlockShots - 1);
-        const uint32_t tl = within >> 5;
-        const uint32_t bit = 1u << (within & 31u);
-        const uint32_t u = ev.upau;
-        const uint32_t rb = P.body_rbase + rs;
-        const uint32_t gword = blk * 64u + tl;
-        uint32_t fl;  // flip mask over (x1, z1, x2, z2)
-        if (kind == 1) {
-          const uint32_t u31 = u >> 1;
-          fl = (u31 < t1 || u31 >= t2 ? 1u : 0u) | (u31 >= t1 ? 2u : 0u);  // X/Z/Y
-        } else {
-          // uniform over the 15 non-identity Paulis; digit order as
-          // _apply_depolarization_pauli! (src/noise.jl:112-118): first qubit
-          // is the least significant base-4 digit, 1=X 2=Z 3=Y
-          fl = __umulhi(u, 15u) + 1u;
-        }
-        const uint32_t planes[4] = {pab & 0xFFFFu, pab >> 16, pcd & 0xFFFFu, pcd >> 16};
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (fl & (1u << j)) {
-            F.xr(planes[j], tl, bit);
-            export_flip<MODE>(P, rb + j, gword, bit);
-          }
-        }
-      }
-    }
-    if (ev.last >= lo + span) break;
-    gen_batch<0>(ev, P, blk, lane);
-  }
-}
-
 // vmcnt(0) only (expcnt and lgkmcnt left at their maxima): waits for the
 // program prefetch and the record stores once per 64-instruction chunk.
 #define QCF_WAIT_VMEM() __builtin_amdgcn_s_waitcnt(0x0F70)
@@ -209,10 +102,10 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
   const bool live = gw < P.nwbuf;
   const uint32_t lmask = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1 ? P.lastmask : 0xFFFFFFFFu);
   const uint64_t goff = gw;
-  Xoshiro128 gauge{0, 0, 0, 1};
-  if (MODE != kReplay) gauge = gauge_stream(P.seed, gw);
-  EvState ev0{0, 0, 0xFFFFFFFFu, 0xFFFFFFFFu, 0};
-  if (MODE != kReplay && P.nclasses > 0) gen_batch<0>(ev0, P, blk, lane);
+  qcfd::Xoshiro gauge{0, 0, 0, 1};
+  if (MODE != kReplay) gauge = qcfd::gauge_stream(P.seed, gw);
+  EvState ev0 = qcfd::ev_init();
+  if (MODE != kReplay && P.nclasses > 0) ev0 = qcfd::gen_batch(ev0, P.seed, P.cls_c[0], blk, lane);
   // Every global load in the instruction loop is waited for on the spot, so
   // no load is ever pending where control flow merges (which would make the
   // compiler wait for all outstanding record stores there).
@@ -286,14 +179,19 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
             const uint32_t m0 = draw(rb), m1 = draw(rb + 1);
             XR(pab & 0xFFFFu, m0);
             XR(pab >> 16, m1);
-            if (readlane(cB.x, k + i) == 2) {
+            if (pcd != 0u) {
               const uint32_t m2 = draw(rb + 2), m3 = draw(rb + 3);
               XR(pcd & 0xFFFFu, m2);
               XR(pcd >> 16, m3);
             }
           }
         } else if (P.nclasses > 0) {
-          group_events<MODE, GL>(ev0, readlane(cA.y, k), G, k, cA, cB, F, P, blk, lane);
+          auto flip = [&](uint32_t plane, uint32_t tl, uint32_t bit, uint32_t j, uint32_t rs) {
+            F.xr(plane, tl, bit);
+            export_flip<MODE>(P, P.body_rbase + rs + j, blk * 64u + tl, bit);
+          };
+          qcfd::group_events(ev0, readlane(cA.y, k), G, k + 1, cA.y, cA.z, cB.x, cB.y, cA.w,
+                             P.seed, P.cls_c[0], blk, lane, flip);
         }
         k += G;
         continue;
@@ -381,7 +279,7 @@ __global__ __launch_bounds__(64) void pf_kernel(KParams P) {
   const uint32_t lane = threadIdx.x;
   Planes<GL> F;
   F.base = GL ? P.scratch + (uint64_t)blockIdx.x * P.sstride : lds;
-  for (uint32_t blk = blockIdx.x; blk < P.nblocks; blk += gridDim.x)
+  for (uint32_t blk = P.blk0 + blockIdx.x; blk < P.nblocks; blk += gridDim.x)
     run_tile<MODE, GL>(P, blk, F, lane);
 }
 
@@ -428,7 +326,7 @@ __global__ void philox_kernel(const uint32_t *in, uint32_t *out, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t *c = in + 6 * i;
-  const u32x4 r = philox4x32_10(u32x4{c[0], c[1], c[2], c[3]}, c[4], c[5]);
+  const qcfd::u4 r = qcfd::philox(qcfd::u4{c[0], c[1], c[2], c[3]}, c[4], c[5]);
   out[4 * i + 0] = r.x;
   out[4 * i + 1] = r.y;
   out[4 * i + 2] = r.z;
@@ -438,7 +336,7 @@ __global__ void philox_kernel(const uint32_t *in, uint32_t *out, size_t n) {
 // ---- wave scan self-check kernel (one wave per 64 inputs)
 __global__ __launch_bounds__(64) void scan_kernel(const uint32_t *in, uint32_t *out) {
   const uint32_t i = blockIdx.x * 64u + threadIdx.x;
-  out[i] = wave_inclusive_scan(in[i]);
+  out[i] = qcfd::wave_inclusive_scan(in[i]);
 }
 
 // ---- reference-layout converters
@@ -518,14 +416,7 @@ int ensure_device_copy(qcf_program *P, int variant, const uint4 **prog,
     dc->device = dev;
   }
   if (!dc->prog[variant]) {
-    // variant bit 0: fresh prologue (else in-place LOAD); bit 1: STORE epilogue
-    std::vector<Instr> code;
-    const auto &pro = (variant & 1) ? P->prologue_fresh : P->prologue_inplace;
-    code.insert(code.end(), pro.begin(), pro.end());
-    while (code.size() % kChunk) code.push_back(Instr{I_END, 0, 0, 0, 0, 0, 0, 0});
-    code.insert(code.end(), P->body.begin(), P->body.end());
-    if (variant & 2) code.insert(code.end(), P->epilogue_store.begin(), P->epilogue_store.end());
-    if (code.empty()) code.push_back(Instr{I_END, 0, 0, 0, 0, 0, 0, 0});
+    const std::vector<Instr> code = variant_code(P, variant);
     void *d = nullptr;
     QCF_HIP_CHECK(hipMalloc(&d, code.size() * sizeof(Instr)));
     QCF_HIP_CHECK(hipMemcpy(d, code.data(), code.size() * sizeof(Instr), hipMemcpyHostToDevice));
@@ -653,6 +544,36 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
   if (global) {
     QCF_TRY(frame_scratch(grid * (size_t)P->nplanes * 256u, &kp.scratch));
     kp.sstride = P->nplanes * 64u;
+  }
+  // Native launches of on-chip frames run the kernel specialised to this
+  // program on every complete 2048-shot tile; the interpreter (the general
+  // path) takes the partial last tile, so the specialised code has no bounds
+  // checks.  Both draw the same events for the same shots.
+  if (!global && !masks && !exp && !jit_disabled()) {
+    const uint32_t nfull = (uint32_t)(shots / kBlockShots);
+    void *fn = nullptr;
+    int js = nfull ? jit_get(P, variant, &fn) : QCF_OK;
+    if (js == QCF_OK && nfull) {
+      JitParams jp{};
+      jp.rec = kp.rec;
+      jp.frame = kp.frame;
+      jp.seed = kp.seed;
+      jp.ld32 = kp.ld32;
+      jp.nblocks = nfull;
+      jp.c = kp.cls_c[0];
+      jp.has_class = kp.nclasses > 0 ? 1u : 0u;
+      const uint64_t g1 = std::min<uint64_t>(nfull, (uint64_t)st->cu_count * wpc);
+      js = jit_launch(fn, jp, (uint32_t)g1, lds_bytes, stream);
+    }
+    if (js == QCF_OK) {
+      if (nfull == kp.nblocks) return QCF_OK;
+      KParams tail = kp;
+      tail.blk0 = nfull;
+      return launch_pf<kNative>(tail, lds_bytes, 1, stream);
+    }
+    const char *req = getenv("QCF_REQUIRE_JIT");
+    if (req && req[0] == '1') return js;
+    clear_error();
   }
   if (masks && !exp) return launch_pf<kReplay>(kp, lds_bytes, (uint32_t)grid, stream);
   if (exp) return launch_pf<kExport>(kp, lds_bytes, (uint32_t)grid, stream);

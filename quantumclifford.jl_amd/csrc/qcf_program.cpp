@@ -282,10 +282,9 @@ struct Compiler {
     s.kind = 1;
     s.rslot = rs;
     rs += 2;
-    if (c3 > 0) {
-      const double two31 = 2147483648.0;
-      s.t1 = c1 >= c3 ? (1u << 31) : (uint32_t)std::floor(c1 / c3 * two31);
-      s.t2 = c2 >= c3 ? (1u << 31) : (uint32_t)std::floor(c2 / c3 * two31);
+    if (c3 > 0) {  // 2^15-scaled: X if u < t1, Z if u < t2, else Y
+      s.t1 = c1 >= c3 ? (1u << 15) : (uint32_t)std::floor(c1 / c3 * 32768.0);
+      s.t2 = c2 >= c3 ? (1u << 15) : (uint32_t)std::floor(c2 / c3 * 32768.0);
     }
     s.bitops = bitops;
     const uint32_t id = (uint32_t)P->sites.size();
@@ -410,7 +409,7 @@ struct Compiler {
         const Site &s = P->sites[B[k].c];
         const uint32_t pb = B[k].op == I_NOISE2 ? B[k].b : 0u;
         const uint32_t pa = B[k].op == I_NOISE2 ? B[k].a : (B[k].a | (B[k].b << 16));
-        out.push_back(Instr{I_SITE, pa, pb, s.rslot, s.kind, s.accept, s.t1, s.t2});
+        out.push_back(Instr{I_SITE, pa, pb, s.rslot, s.t1 | (s.t2 << 16), s.accept, 0, 0});
       }
       i = j;
     }
@@ -527,6 +526,25 @@ qcf_program::~qcf_program() {
       if (p) (void)hipFree(p);
     (void)hipSetDevice(cur);
   }
+  for (auto &j : jit) {
+    if (!j.module) continue;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(j.device);
+    (void)hipModuleUnload((hipModule_t)j.module);
+    (void)hipSetDevice(cur);
+  }
+}
+
+std::vector<qcf::Instr> qcf::variant_code(const qcf_program *P, int variant) {
+  std::vector<Instr> code;
+  const auto &pro = (variant & 1) ? P->prologue_fresh : P->prologue_inplace;
+  code.insert(code.end(), pro.begin(), pro.end());
+  while (code.size() % kChunk) code.push_back(Instr{I_END, 0, 0, 0, 0, 0, 0, 0});
+  code.insert(code.end(), P->body.begin(), P->body.end());
+  if (variant & 2) code.insert(code.end(), P->epilogue_store.begin(), P->epilogue_store.end());
+  if (code.empty()) code.push_back(Instr{I_END, 0, 0, 0, 0, 0, 0, 0});
+  return code;
 }
 
 extern "C" int qcf_program_build(const qcf_op *ops, size_t nops,

@@ -33,8 +33,8 @@ enum InstrOp : uint32_t {
   I_NOISEG,     // G = op>>8 noise sites with consecutive ordinals starting at a;
                 // their data are the next G instructions (I_SITE), always in
                 // the same 64-instruction chunk
-  I_SITE,       // site data: a = x1|z1<<16, b = x2|z2<<16, c = replay plane,
-                // d = kind (1 or 2), e = accept, f = t1, g = t2
+  I_SITE,       // site data: a = x1|z1<<16, b = x2|z2<<16 (0: one qubit),
+                // c = replay plane, d = t1|t2<<16, e = accept (see qcf_device.h)
   I_MEAS,       // v = P[a]; rec[b] = v; if c != none: P[c] = v
   I_XSTORE2,    // v = P[a] ^ P[b]; rec[c] = v; if d != none: P[d] = v
   I_RESETZ,     // P[a] = 0; P[b] ^= draw(c)           (sMRZ reset part)
@@ -70,7 +70,7 @@ struct Site {
   uint32_t ordinal;  // index of this site within its class
   uint32_t accept;   // thinning threshold in [0, 2^31]
   uint32_t rslot;    // first replay plane (body relative)
-  uint32_t t1, t2;   // 2^31-scaled cumulative thresholds
+  uint32_t t1, t2;   // 2^15-scaled cumulative thresholds
   uint32_t kind;     // 1 or 2 qubits
   uint32_t bitops;   // accounting weight
 };
@@ -80,6 +80,13 @@ struct ClassInfo {
   float c;       // 1 / log2(1 - p)  (<= 0)
 };
 
+}  // namespace qcf
+
+struct qcf_program;
+namespace qcf {
+// Device code of one launch variant: bit 0 = fresh prologue (else in-place
+// loads), bit 1 = frame store epilogue; the body starts on a chunk boundary.
+std::vector<Instr> variant_code(const qcf_program *P, int variant);
 }  // namespace qcf
 
 // Host-side compiled program (opaque to C callers).
@@ -103,5 +110,14 @@ struct qcf_program {
     uint32_t ninstr[4] = {0, 0, 0, 0};
   };
   std::vector<DeviceCopy> dev;
+  // kernels specialised to this program by hiprtc, per device and variant
+  struct JitKernel {
+    int device = -1;
+    int variant = -1;
+    void *module = nullptr;   // hipModule_t
+    void *function = nullptr; // hipFunction_t
+    bool failed = false;
+  };
+  std::vector<JitKernel> jit;
   ~qcf_program();
 };

@@ -85,6 +85,8 @@ _SIGNATURES = {
     "qcf_autotune": (_I, [_P, _SZ, _P, C.POINTER(Tune), _P]),
     "qcf_debug_philox": (_I, [_P, _P, _SZ, _P]),
     "qcf_debug_wave_scan": (_I, [_P, _P, _SZ, _P]),
+    "qcf_debug_jit_source": (_I, [_P, _I, _P, _SZ, C.POINTER(_SZ)]),
+    "qcf_debug_jit_compile": (_I, [_P, _I, C.POINTER(_SZ)]),
     "qcf_record_to_bool": (_I, [_P, _SZ, _U64, _U32, _P, _P]),
     "qcf_record_from_bool": (_I, [_P, _U64, _U32, _P, _SZ, _P]),
     "qcf_frame_to_reference": (_I, [_P, _SZ, _U64, _U32, _P, _P]),
