@@ -13,10 +13,12 @@
 #include <hip/hiprtc.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <sstream>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "qcf_common.hpp"
@@ -233,7 +235,16 @@ int compile(const std::string &src, std::vector<char> *code, std::string *log) {
 
 namespace {
 std::mutex g_jit_mu;
+constexpr size_t kCodeCacheMax = 32;
+std::unordered_map<std::string, std::vector<char>> g_code_cache;  // source -> code object
 }  // namespace
+
+// Programs longer than this many device instructions stay on the interpreter:
+// compile time grows faster than linearly with the straight-line code.
+size_t jit_max_instr() {
+  const char *e = getenv("QCF_JIT_MAX_INSTR");
+  return e ? (size_t)strtoull(e, nullptr, 10) : (size_t)12000;
+}
 
 bool jit_disabled() {
   const char *e = getenv("QCF_NO_JIT");
@@ -256,9 +267,25 @@ int jit_get(qcf_program *P, int variant, void **function) {
   jk.variant = variant;
   std::string src, log;
   std::vector<char> bin;
-  int st = generate(P, variant, &src) ? QCF_OK
-                                      : set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
-  if (st == QCF_OK) st = compile(src, &bin, &log);
+  int st = QCF_OK;
+  if (variant_code(P, variant).size() > jit_max_instr())
+    st = set_error(QCF_ERR_UNSUPPORTED, "program longer than QCF_JIT_MAX_INSTR instructions");
+  else if (!generate(P, variant, &src))
+    st = set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
+  if (st == QCF_OK) {
+    // programs with the same source (the same circuit built twice) share one
+    // compilation
+    auto it = g_code_cache.find(src);
+    if (it != g_code_cache.end()) {
+      bin = it->second;
+    } else {
+      st = compile(src, &bin, &log);
+      if (st == QCF_OK) {
+        if (g_code_cache.size() >= kCodeCacheMax) g_code_cache.clear();
+        g_code_cache.emplace(src, bin);
+      }
+    }
+  }
   if (st == QCF_OK) {
     hipModule_t m;
     hipFunction_t f;

@@ -1,0 +1,93 @@
+"""Per-circuit specialised kernels (compiled at run time) against the
+interpreter: for native launches both draw the same events, so records and
+frames must agree bit for bit -- including a partial last tile, which the
+interpreter runs after the specialised kernel has done the complete tiles.
+The interpreter itself is pinned to the oracle by the export/replay tests."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import qcf
+from qcf import circuits
+from test_gpu_frames import random_circuit
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(monkeypatch, run):
+    monkeypatch.delenv("QCF_NO_JIT", raising=False)
+    monkeypatch.setenv("QCF_REQUIRE_JIT", "1")
+    a = run()
+    torch.cuda.synchronize()
+    monkeypatch.delenv("QCF_REQUIRE_JIT")
+    monkeypatch.setenv("QCF_NO_JIT", "1")
+    b = run()
+    torch.cuda.synchronize()
+    monkeypatch.delenv("QCF_NO_JIT")
+    return a, b
+
+
+@pytest.mark.parametrize("d,rounds,p,shots", [(3, 3, 1e-2, 3 * 2048 + 77),
+                                               (5, 25, 1e-3, 40 * 2048 + 5),
+                                               (5, 5, 3e-2, 4096)])
+def test_surface_code_specialised_equals_interpreter(gpu, monkeypatch, d, rounds, p, shots):
+    prog = qcf.Program(circuits.rotated_surface_code(d, rounds, p).ops)
+
+    def run():
+        f = qcf.pftrajectories(prog, shots, seed=1000 + d)
+        return f.record_host(), f.frame_host()
+
+    (ra, fa), (rb, fb) = _both(monkeypatch, run)
+    assert np.array_equal(ra, rb)
+    assert np.array_equal(fa, fb)
+    assert oracle.unpack_shots(ra, shots).any()
+
+
+def test_record_only_launch_specialised_equals_interpreter(gpu, monkeypatch):
+    prog = qcf.Program(circuits.repetition_code(5, 5, 2e-2).ops)
+
+    def run():
+        return qcf.pftrajectories(prog, 5 * 2048 + 3, seed=4, keep_frame=False).record_host()
+
+    a, b = _both(monkeypatch, run)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("shots", [2048, 2 * 2048 + 1])
+def test_random_circuits_specialised_equals_interpreter(gpu, monkeypatch, shots):
+    """Every gate, measurement, reset, classical XOR and noise kind."""
+    rng = np.random.default_rng(500 + shots)
+    for trial in range(4):
+        n, nbits = int(rng.integers(2, 9)), int(rng.integers(1, 12))
+        circ = random_circuit(rng, n, nbits, 150)
+        prog = qcf.Program(circ, nqubits=n, nbits=nbits)
+
+        def run():
+            f = qcf.pftrajectories(prog, shots, seed=trial)
+            return f.record_host(), f.frame_host()
+
+        (ra, fa), (rb, fb) = _both(monkeypatch, run)
+        assert np.array_equal(ra, rb), trial
+        assert np.array_equal(fa, fb), trial
+
+
+def test_in_place_specialised_equals_interpreter(gpu, monkeypatch):
+    """pftrajectories(state, circuit) on existing frames (the in-place variant)."""
+    rng = np.random.default_rng(3)
+    n, nbits, shots = 6, 10, 3 * 2048 + 9
+    circ = random_circuit(rng, n, nbits, 120)
+    prog = qcf.Program(circ, nqubits=n, nbits=nbits)
+    fb = oracle.pack_shots(rng.integers(0, 2, (2 * (n + 1), shots)).astype(np.uint8))
+    mb = oracle.pack_shots(rng.integers(0, 2, (nbits, shots)).astype(np.uint8))
+
+    def run():
+        f = qcf.PauliFrame(shots, n, nbits, init=False)
+        f.frame.copy_(torch.from_numpy(fb.view(np.int64).reshape(-1)))
+        f.record.copy_(torch.from_numpy(mb.view(np.int64).reshape(-1)))
+        qcf.pftrajectories_state(f, prog, seed=77)
+        return f.record_host(), f.frame_host()
+
+    (ra, fa), (rb, fb2) = _both(monkeypatch, run)
+    assert np.array_equal(ra, rb)
+    assert np.array_equal(fa, fb2)
