@@ -104,8 +104,10 @@ __device__ __forceinline__ EvState ev_init() {
 // Draw the next 64 events: one geometric gap per lane (failures before the
 // next event, floor(log2(u) * c) with c = 1/log2(1-p)) and a wave prefix sum.
 // Gaps beyond 2^24 trials advance without an event (memorylessness keeps the
-// process exact).  Specialised kernels keep one out-of-line copy of it.
-#ifdef QCF_RTC
+// process exact).  Specialised kernels with planes in LDS keep one
+// out-of-line copy of it; with planes in registers it is inlined (a call would
+// have to save the live planes).
+#if defined(QCF_RTC) && !defined(QCF_INLINE_EVENTS)
 #define QCF_GEN_BATCH_ATTR __attribute__((noinline))
 #else
 #define QCF_GEN_BATCH_ATTR __forceinline__

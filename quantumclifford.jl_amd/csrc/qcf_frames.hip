@@ -546,30 +546,28 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
     kp.sstride = P->nplanes * 64u;
   }
   // Native launches of on-chip frames run the kernel specialised to this
-  // program on every complete 2048-shot tile; the interpreter (the general
-  // path) takes the partial last tile, so the specialised code has no bounds
-  // checks.  Both draw the same events for the same shots.
+  // program (the partial last tile included); the interpreter is the general
+  // path.  Both draw the same events for the same shots.
   if (!global && !masks && !exp && !jit_disabled()) {
-    const uint32_t nfull = (uint32_t)(shots / kBlockShots);
-    void *fn = nullptr;
-    int js = nfull ? jit_get(P, variant, &fn) : QCF_OK;
-    if (js == QCF_OK && nfull) {
+    JitKernelInfo jk;
+    int js = jit_get(P, variant, &jk);
+    if (js == QCF_OK) {
       JitParams jp{};
       jp.rec = kp.rec;
       jp.frame = kp.frame;
       jp.seed = kp.seed;
       jp.ld32 = kp.ld32;
-      jp.nblocks = nfull;
+      jp.nblocks = kp.nblocks;
+      jp.nw32 = kp.nw32;
+      jp.nwbuf = kp.nwbuf;
+      jp.lastmask = kp.lastmask;
       jp.c = kp.cls_c[0];
       jp.has_class = kp.nclasses > 0 ? 1u : 0u;
-      const uint64_t g1 = std::min<uint64_t>(nfull, (uint64_t)st->cu_count * wpc);
-      js = jit_launch(fn, jp, (uint32_t)g1, lds_bytes, stream);
-    }
-    if (js == QCF_OK) {
-      if (nfull == kp.nblocks) return QCF_OK;
-      KParams tail = kp;
-      tail.blk0 = nfull;
-      return launch_pf<kNative>(tail, lds_bytes, 1, stream);
+      jp.nfull = (uint32_t)(shots / kBlockShots);
+      const uint32_t jwpc = waves_per_cu ? waves_per_cu : jk.blocks_per_cu;
+      const uint64_t g1 = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * jwpc);
+      js = jit_launch(jk.function, jp, (uint32_t)g1, jk.lds_bytes, stream);
+      if (js == QCF_OK) return QCF_OK;
     }
     const char *req = getenv("QCF_REQUIRE_JIT");
     if (req && req[0] == '1') return js;

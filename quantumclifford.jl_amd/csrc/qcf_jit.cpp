@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -40,7 +41,7 @@ struct KJ {
   uint64_t ld32;
   uint32_t nblocks, nw32, nwbuf, lastmask;
   float c;
-  uint32_t has_class, pad0, pad1;
+  uint32_t has_class, nfull, pad0;
 };
 struct FlipLDS {
   uint32_t *L;
@@ -51,160 +52,426 @@ struct FlipLDS {
 };
 )QCFJIT";
 
-namespace {
 std::string plane(uint32_t p) { return "L[" + std::to_string(p * 64u) + "u + l]"; }
-std::string rec(uint32_t col) { return "P.rec[(uint64_t)" + std::to_string(col) + "u * P.ld32 + gw]"; }
-}  // namespace
+
+// Fields of an instruction that name record columns (shifted from one loop
+// iteration to the next), and fields the specialised kernel ignores (replay
+// plane numbers).  Everything else must repeat exactly.
+struct Roles {
+  uint8_t col;     // bit k: word k (a=0 .. g=6) is a record column
+  uint8_t ignore;  // bit k: word k is not used
+};
+
+Roles roles(uint32_t op) {
+  switch (op & 0xFFu) {
+    case I_MEAS: return {1u << 1, 0};
+    case I_XSTORE2: return {1u << 2, 0};
+    case I_RESETZ: return {0, 1u << 2};
+    case I_ACC_REC:
+    case I_ACCX_REC:
+    case I_ACC_STORE:
+    case I_REC_ZERO: return {1u << 0, 0};
+    case I_INIT: return {0, 1u << 2};
+    case I_MR: return {1u << 3, (1u << 2) | (1u << 5) | (1u << 6)};
+    case I_MRX: return {(1u << 3) | (1u << 6), 1u << 2};
+    case I_SITE: return {0, (1u << 2) | (1u << 5) | (1u << 6)};
+    case I_NOISEG: return {0, 0x7Fu};  // the ordinal (word a) is checked separately
+    default: return {0, 0};
+  }
+}
+
+// Does b repeat a with record columns shifted by dcol and (for a noise group)
+// the first ordinal shifted by dord?  dcol / dord are fixed by the first pair
+// that determines them (*have_col / *have_ord).
+bool repeats(const Instr &a, const Instr &b, int64_t *dcol, bool *have_col, int64_t *dord,
+             bool *have_ord) {
+  if (a.op != b.op) return false;
+  const Roles r = roles(a.op);
+  const uint32_t *wa = &a.a, *wb = &b.a;
+  for (int k = 0; k < 7; ++k) {
+    const uint32_t bit = 1u << k;
+    if (r.ignore & bit) continue;
+    if (r.col & bit) {
+      if ((wa[k] == kNone) != (wb[k] == kNone)) return false;
+      if (wa[k] == kNone) continue;
+      const int64_t d = (int64_t)wb[k] - (int64_t)wa[k];
+      if (!*have_col) {
+        *dcol = d;
+        *have_col = true;
+      } else if (d != *dcol) {
+        return false;
+      }
+      continue;
+    }
+    if (wa[k] != wb[k]) return false;
+  }
+  if ((a.op & 0xFFu) == I_NOISEG) {
+    const int64_t d = (int64_t)b.a - (int64_t)a.a;
+    if (!*have_ord) {
+      *dord = d;
+      *have_ord = true;
+    } else if (d != *dord) {
+      return false;
+    }
+  }
+  return true;
+}
+
+uint64_t shape(const Instr &in) {
+  const Roles r = roles(in.op);
+  uint64_t h = 1469598103934665603ull ^ in.op;
+  const uint32_t *w = &in.a;
+  for (int k = 0; k < 7; ++k) {
+    const uint32_t bit = 1u << k;
+    const uint32_t v = (r.ignore & bit) ? 0u : (r.col & bit) ? (w[k] == kNone ? 1u : 2u) : w[k];
+    h = (h ^ v) * 1099511628211ull;
+  }
+  return h;
+}
 
 }  // namespace
+
+// Repeated stretch of the instruction stream (the rounds of a memory
+// experiment): code[s, s + R*L) is R copies of code[s, s + L), copy k with
+// record columns + k*dcol and noise ordinals + k*dord.
+struct Loop {
+  size_t s = 0, L = 0, R = 0;
+  int64_t dcol = 0, dord = 0;
+};
+
+Loop find_loop(const std::vector<Instr> &code) {
+  Loop best;
+  const size_t n = code.size();
+  if (n < 64) return best;
+  std::vector<uint64_t> h(n);
+  for (size_t i = 0; i < n; ++i) h[i] = shape(code[i]);
+  // candidate periods: distances from an anchor in the middle to the next
+  // instructions of the same shape
+  std::vector<size_t> cands;
+  for (size_t anchor : {n / 2, n / 3, (2 * n) / 3}) {
+    size_t found = 0;
+    for (size_t j = anchor + 1; j < n && found < 6; ++j)
+      if (h[j] == h[anchor]) {
+        cands.push_back(j - anchor);
+        ++found;
+      }
+  }
+  std::sort(cands.begin(), cands.end());
+  cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
+  for (size_t L : cands) {
+    if (L < 8 || 2 * L > n) continue;
+    // longest run of i with code[i + L] repeating code[i] under one shift
+    size_t i = 0;
+    while (i + L < n) {
+      if (h[i] != h[i + L]) {
+        ++i;
+        continue;
+      }
+      int64_t dcol = 0, dord = 0;
+      bool hc = false, ho = false;
+      size_t j = i;
+      while (j + L < n && h[j] == h[j + L] && repeats(code[j], code[j + L], &dcol, &hc, &dord, &ho))
+        ++j;
+      const size_t run = j - i;
+      const size_t R = (run + L) / L;
+      if (R >= 2 && (R - 1) * L > (best.R > 1 ? (best.R - 1) * best.L : 0)) {
+        best.s = i;
+        best.L = L;
+        best.R = R;
+        best.dcol = hc ? dcol : 0;
+        best.dord = ho ? dord : 0;
+      }
+      i = j > i ? j : i + 1;
+    }
+  }
+  return best;
+}
 
 // Write the kernel source of one variant; returns false for programs the
 // generator does not handle (the interpreter runs those).
-bool generate(const qcf_program *P, int variant, std::string *out) {
-  const std::vector<Instr> code = variant_code(P, variant);
-  std::ostringstream tab, body;
-  std::vector<std::pair<uint32_t, uint32_t>> groups;  // (first site, count)
-  uint32_t nsite = 0;
-  for (size_t i = 0; i < code.size(); ++i) {
-    const Instr &in = code[i];
-    if ((in.op & 0xFFu) != I_NOISEG) continue;
-    const uint32_t G = in.op >> 8;
-    groups.push_back({nsite, G});
-    for (uint32_t k = 1; k <= G; ++k) {
-      const Instr &s = code[i + k];
-      tab << s.a << "u," << s.b << "u," << s.d << "u," << s.e << "u,\n";
-    }
-    nsite += G;
-    i += G;
+//
+// The stream is first folded: a stretch that repeats (the rounds of a memory
+// experiment) is emitted once inside a loop over its copies, so the kernel
+// stays small enough for the instruction cache.
+//
+// Two storage modes.  Programs with at most jit_reg_planes() planes keep every
+// plane in a register (r<k>): gates are plain VALU XORs and the only LDS is a
+// small buffer of noise slots -- the event pass XORs each event's bit into
+// the slots of its site, and the site then takes its slots back with one
+// exchange-with-zero per plane (so the buffer is zero again for the next
+// group).  Longer programs keep the planes in LDS (L[p*64+l]) as the
+// interpreter does and the events XOR straight into them.
+bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds_bytes) {
+  std::vector<Instr> code;
+  for (const Instr &in : variant_code(P, variant))
+    if ((in.op & 0xFFu) != I_END) code.push_back(in);  // chunk padding
+  Loop lp = jit_no_loops() ? Loop{} : find_loop(code);
+  if (lp.R < 2) lp = Loop{0, 0, 0, 0, 0};
+  // emitted ranges: [0, s), loop body [s, s+L), [s+R*L, n)
+  struct Range {
+    size_t b, e;
+    bool loop;
+  };
+  std::vector<Range> ranges;
+  if (lp.R >= 2) {
+    ranges = {{0, lp.s, false}, {lp.s, lp.s + lp.L, true}, {lp.s + lp.R * lp.L, code.size(), false}};
+  } else {
+    ranges = {{0, code.size(), false}};
   }
+  const bool regs = P->nplanes <= jit_reg_planes();
+  constexpr uint32_t kSlots = 64;  // noise slots per event pass in register mode
+  struct Group {
+    uint32_t site0, count, ord0, slots;
+  };
+  std::ostringstream tab;
+  std::vector<Group> groups;
+  std::vector<size_t> range_groups;      // first group of each range
+  std::vector<const Instr *> sites;
+  std::vector<uint32_t> site_slot;  // first slot of each site (register mode)
+  uint32_t max_slots = 0;
+  for (const Range &rg : ranges) {
+    range_groups.push_back(groups.size());
+    for (size_t i = rg.b; i < rg.e; ++i) {
+      const Instr &in = code[i];
+      if ((in.op & 0xFFu) != I_NOISEG) continue;
+      const uint32_t G = in.op >> 8;
+      for (uint32_t k = 0; k < G;) {
+        Group g{(uint32_t)sites.size(), 0u, in.a + k, 0u};
+        while (k < G) {
+          const Instr &s = code[i + 1 + k];
+          const uint32_t need = s.b ? 4u : 2u;
+          if (regs && g.slots + need > kSlots) break;
+          if (regs) {
+            const uint32_t b0 = g.slots;
+            tab << (b0 | (b0 + 1u) << 16) << "u,"
+                << (s.b ? ((b0 + 2u) | (b0 + 3u) << 16) : 0u) << "u," << s.d << "u," << s.e
+                << "u,\n";
+          } else {
+            tab << s.a << "u," << s.b << "u," << s.d << "u," << s.e << "u,\n";
+          }
+          sites.push_back(&s);
+          site_slot.push_back(g.slots);
+          g.slots += need;
+          ++g.count;
+          ++k;
+        }
+        max_slots = std::max(max_slots, g.slots);
+        groups.push_back(g);
+      }
+      i += G;
+    }
+  }
+  range_groups.push_back(groups.size());
+  const uint32_t nsite = (uint32_t)sites.size();
+  auto PL = [&](uint32_t p) {
+    return regs ? "r" + std::to_string(p) : plane(p);
+  };
+  auto XOR = [&](uint32_t p, const std::string &v) {
+    return regs ? PL(p) + " ^= " + v + ";" : "atomicXor(&" + plane(p) + ", " + v + ");";
+  };
+  auto slot = [](uint32_t k) { return "L[" + std::to_string(k * 64u) + "u + l]"; };
   auto load_group = [&](size_t g, const char *pfx) {
     std::ostringstream o;
-    const uint32_t off = groups[g].first, G = groups[g].second;
+    const uint32_t off = groups[g].site0, G = groups[g].count;
     o << "    { const uint32_t *q = QS + 4u * (" << off << "u + (l < " << G << "u ? l : " << G - 1
       << "u));\n      " << pfx << "0 = q[0]; " << pfx << "1 = q[1]; " << pfx << "2 = q[2]; "
       << pfx << "3 = q[3]; }\n";
     return o.str();
   };
-  size_t gi = 0;
-  if (!groups.empty()) body << "    uint32_t t0, t1, t2, t3;\n" << load_group(0, "t");
-  for (size_t i = 0; i < code.size(); ++i) {
-    const Instr &in = code[i];
-    const uint32_t op = in.op & 0xFFu;
-    const uint32_t a = in.a, b = in.b, c = in.c;
-    switch (op) {
-      case I_END:
-        break;
-      case I_XORN: {
-        const uint32_t n = in.op >> 8;
-        const uint32_t *w = &in.a;
-        body << "    {";
-        for (uint32_t k = 0; k < n; ++k)
-          body << " const uint32_t v" << k << " = " << plane(w[k] >> 16) << ";";
-        for (uint32_t k = 0; k < n; ++k)
-          body << " atomicXor(&" << plane(w[k] & 0xFFFFu) << ", v" << k << ");";
-        body << " }\n";
-        break;
-      }
-      case I_NOISEG: {
-        const uint32_t G = in.op >> 8;
-        body << "    {\n";
-        if (gi + 1 < groups.size()) body << "    uint32_t n0, n1, n2, n3;\n" << load_group(gi + 1, "n");
-        body << "    if (P.has_class) qcfd::group_events(ev, " << a << "u, " << G
-             << "u, 0u, t0, t1, t2, t3, 0u, P.seed, P.c, blk, l, flip);\n";
-        if (gi + 1 < groups.size()) body << "    t0 = n0; t1 = n1; t2 = n2; t3 = n3;\n";
-        body << "    }\n";
-        ++gi;
-        i += G;
-        break;
-      }
-      case I_MR:
-      case I_MRX: {
-        const uint32_t col = in.d, keep = in.e;
-        body << "    { const uint32_t v = " << plane(a) << ";";
-        if (col != kNone) body << " " << rec(col) << " = v;";
-        if (op == I_MRX)
-          body << " " << rec(in.g) << " = v ^ " << plane(in.f) << ";";
-        if (keep != kNone) body << " " << plane(keep) << " = v;";
-        body << " " << plane(a) << " = 0u; atomicXor(&" << plane(b) << ", g.next()); }\n";
-        break;
-      }
-      case I_XSTORE2:
-        body << "    { const uint32_t v = " << plane(a) << " ^ " << plane(b) << "; " << rec(c)
-             << " = v;";
-        if (in.d != kNone) body << " " << plane(in.d) << " = v;";
-        body << " }\n";
-        break;
-      case I_MEAS:
-        body << "    { const uint32_t v = " << plane(a) << "; " << rec(b) << " = v;";
-        if (c != kNone) body << " " << plane(c) << " = v;";
-        body << " }\n";
-        break;
-      case I_RESETZ:
-        body << "    " << plane(a) << " = 0u; atomicXor(&" << plane(b) << ", g.next());\n";
-        break;
-      case I_ACC_LDS:
-        body << "    acc = " << plane(a) << ";\n";
-        break;
-      case I_ACCX_LDS:
-        body << "    acc ^= " << plane(a) << ";\n";
-        break;
-      case I_ACC_REC:
-        body << "    acc = " << rec(a) << ";\n";
-        break;
-      case I_ACCX_REC:
-        body << "    acc ^= " << rec(a) << ";\n";
-        break;
-      case I_ACC_STORE:
-        body << "    " << rec(a) << " = acc;";
-        if (b != kNone) body << " " << plane(b) << " = acc;";
-        body << "\n";
-        break;
-      case I_REC_ZERO:
-        body << "    " << rec(a) << " = 0u;\n";
-        break;
-      case I_INIT:
-        body << "    " << plane(a) << " = 0u; " << plane(b) << " = g.next();\n";
-        break;
-      case I_LOAD:
-        body << "    " << plane(a) << " = P.frame[(uint64_t)" << b << "u * P.ld32 + gw];\n";
-        break;
-      case I_STORE:
-        body << "    P.frame[(uint64_t)" << b << "u * P.ld32 + gw] = " << plane(a) << ";\n";
-        break;
-      case I_ZERO:
-        body << "    " << plane(a) << " = 0u;\n";
-        break;
-      case I_LIN1:
-        body << "    { const uint32_t x = " << plane(a) << ", z = " << plane(b) << "; " << plane(a)
-             << " = " << ((c & 1u) ? "x" : "0u") << " ^ " << ((c & 2u) ? "z" : "0u") << "; "
-             << plane(b) << " = " << ((c & 4u) ? "x" : "0u") << " ^ " << ((c & 8u) ? "z" : "0u")
-             << "; }\n";
-        break;
-      default:
-        return false;
-    }
+  auto emit = [&](bool tail, std::ostringstream &body) -> bool {
+  if (regs) {
+    body << "    uint32_t r0 = 0u";
+    for (uint32_t p = 1; p < P->nplanes; ++p) body << ", r" << p << " = 0u";
+    body << ";\n";
   }
+  if (!groups.empty()) body << "    uint32_t t0 = 0u, t1 = 0u, t2 = 0u, t3 = 0u;\n";
+  for (size_t ri = 0; ri < ranges.size(); ++ri) {
+    const Range &rg = ranges[ri];
+    if (rg.b == rg.e) continue;
+    const bool in_loop = rg.loop;
+    auto at = [&](const char *arr, uint32_t col, bool shifted) {
+      return std::string(arr) + "[(uint64_t)(" + std::to_string(col) +
+             (shifted ? "u + cb" : "u") + ") * ld + gw]";
+    };
+    // the partial tile: lanes past the record read zeros and store nothing,
+    // bits past the last shot are stored as zeros
+    auto st_rec = [&](uint32_t col, const std::string &v) {
+      const std::string m = at("P.rec", col, in_loop);
+      return tail ? "if (live) " + m + " = (" + v + ") & lm;" : m + " = " + v + ";";
+    };
+    auto ld_rec = [&](uint32_t col) {
+      const std::string m = at("P.rec", col, in_loop);
+      return tail ? "(live ? " + m + " : 0u)" : m;
+    };
+    if (in_loop)
+      body << "    #pragma nounroll\n    for (uint32_t it = 0; it < " << lp.R
+           << "u; ++it) {\n    const uint32_t cb = it * " << (uint32_t)lp.dcol
+           << "u, ob = it * " << (uint32_t)lp.dord << "u;\n    (void)cb; (void)ob;\n";
+    size_t gi = range_groups[ri];
+    const size_t gend = range_groups[ri + 1];
+    if (gi < gend) body << load_group(gi, "t");
+    for (size_t i = rg.b; i < rg.e; ++i) {
+      const Instr &in = code[i];
+      const uint32_t op = in.op & 0xFFu;
+      const uint32_t a = in.a, b = in.b, c = in.c;
+      switch (op) {
+        case I_XORN: {
+          const uint32_t n = in.op >> 8;
+          const uint32_t *w = &in.a;
+          body << "    {";
+          for (uint32_t k = 0; k < n; ++k)
+            body << " const uint32_t v" << k << " = " << PL(w[k] >> 16) << ";";
+          for (uint32_t k = 0; k < n; ++k)
+            body << " " << XOR(w[k] & 0xFFFFu, "v" + std::to_string(k));
+          body << " }\n";
+          break;
+        }
+        case I_NOISEG: {
+          const uint32_t G = in.op >> 8;
+          for (uint32_t done = 0; done < G; ++gi) {
+            const Group &g = groups[gi];
+            body << "    {\n";
+            if (gi + 1 < gend) body << "    uint32_t n0, n1, n2, n3;\n" << load_group(gi + 1, "n");
+            body << "    if (P.has_class) qcfd::group_events(ev, " << g.ord0
+                 << (in_loop ? "u + ob, " : "u, ") << g.count
+                 << "u, 0u, t0, t1, t2, t3, 0u, P.seed, P.c, blk, l, flip);\n";
+            if (gi + 1 < gend) body << "    t0 = n0; t1 = n1; t2 = n2; t3 = n3;\n";
+            if (regs) {
+              for (uint32_t k = 0; k < g.count; ++k) {
+                const Instr &s = *sites[g.site0 + k];
+                const uint32_t s0 = site_slot[g.site0 + k];
+                const uint32_t pl[4] = {s.a & 0xFFFFu, s.a >> 16, s.b & 0xFFFFu, s.b >> 16};
+                const uint32_t np = s.b ? 4u : 2u;
+                body << "   ";
+                for (uint32_t j = 0; j < np; ++j)
+                  body << " " << PL(pl[j]) << " ^= atomicExch(&" << slot(s0 + j) << ", 0u);";
+                // keeps the compiler from reassociating XOR chains across the
+                // noise (which stretches live ranges far past the register file)
+                body << " asm volatile(\"\" :";
+                for (uint32_t j = 0; j < np; ++j)
+                  body << (j ? ", " : " ") << "\"+v\"(" << PL(pl[j]) << ")";
+                body << ");\n";
+              }
+            }
+            body << "    }\n";
+            done += g.count;
+          }
+          i += G;
+          break;
+        }
+        case I_MR:
+        case I_MRX: {
+          const uint32_t col = in.d, keep = in.e;
+          body << "    { const uint32_t v = " << PL(a) << ";";
+          if (col != kNone) body << " " << st_rec(col, "v");
+          if (op == I_MRX) body << " " << st_rec(in.g, "v ^ " + PL(in.f));
+          if (keep != kNone) body << " " << PL(keep) << " = v;";
+          body << " " << PL(a) << " = 0u; " << XOR(b, "g.next()") << " }\n";
+          break;
+        }
+        case I_XSTORE2:
+          body << "    { const uint32_t v = " << PL(a) << " ^ " << PL(b) << "; "
+               << st_rec(c, "v");
+          if (in.d != kNone) body << " " << PL(in.d) << " = v;";
+          body << " }\n";
+          break;
+        case I_MEAS:
+          body << "    { const uint32_t v = " << PL(a) << "; " << st_rec(b, "v");
+          if (c != kNone) body << " " << PL(c) << " = v;";
+          body << " }\n";
+          break;
+        case I_RESETZ:
+          body << "    " << PL(a) << " = 0u; " << XOR(b, "g.next()") << "\n";
+          break;
+        case I_ACC_LDS:
+          body << "    acc = " << PL(a) << ";\n";
+          break;
+        case I_ACCX_LDS:
+          body << "    acc ^= " << PL(a) << ";\n";
+          break;
+        case I_ACC_REC:
+          body << "    acc = " << ld_rec(a) << ";\n";
+          break;
+        case I_ACCX_REC:
+          body << "    acc ^= " << ld_rec(a) << ";\n";
+          break;
+        case I_ACC_STORE:
+          body << "    " << st_rec(a, "acc");
+          if (b != kNone) body << " " << PL(b) << " = acc;";
+          body << "\n";
+          break;
+        case I_REC_ZERO:
+          body << "    " << st_rec(a, "0u") << "\n";
+          break;
+        case I_INIT:
+          body << "    " << PL(a) << " = 0u; " << PL(b) << " = g.next();\n";
+          break;
+        case I_LOAD:
+          body << "    " << PL(a) << " = "
+               << (tail ? "(live ? " + at("P.frame", b, false) + " : 0u)" : at("P.frame", b, false))
+               << ";\n";
+          break;
+        case I_STORE:
+          body << "    " << (tail ? "if (live) " : "") << at("P.frame", b, false) << " = " << PL(a)
+               << (tail ? " & lm;\n" : ";\n");
+          break;
+        case I_ZERO:
+          body << "    " << PL(a) << " = 0u;\n";
+          break;
+        case I_LIN1:
+          body << "    { const uint32_t x = " << PL(a) << ", z = " << PL(b) << "; " << PL(a)
+               << " = " << ((c & 1u) ? "x" : "0u") << " ^ " << ((c & 2u) ? "z" : "0u") << "; "
+               << PL(b) << " = " << ((c & 4u) ? "x" : "0u") << " ^ " << ((c & 8u) ? "z" : "0u")
+               << "; }\n";
+          break;
+        default:
+          return false;
+      }
+    }
+    if (in_loop) body << "    }\n";
+  }
+    return true;
+  };
+  std::ostringstream full, part;
+  if (!emit(false, full) || !emit(true, part)) return false;
   std::ostringstream src;
-  src << "#define QCF_RTC 1\n" << kDeviceSrc << kKernelPrelude;
+  src << "#define QCF_RTC 1\n";
+  if (regs) src << "#define QCF_INLINE_EVENTS 1\n";
+  src << "// folded: prologue " << lp.s << ", loop " << lp.R << " x " << lp.L << ", of "
+      << code.size() << " instructions\n";
+  src << kDeviceSrc << kKernelPrelude;
   src << "__constant__ uint32_t QS[" << std::max<uint32_t>(4, 4 * nsite) << "] = {\n"
       << (nsite ? tab.str() : std::string("0u,0u,0u,0u")) << "};\n";
+  src << "extern \"C\" __global__ void __launch_bounds__(64) qcf_jit_kernel(KJ P) {\n"
+      << "  extern __shared__ uint32_t L[];\n"
+      << "  const uint32_t l = threadIdx.x;\n"
+      << "  const FlipLDS flip{L};\n";
+  if (regs)  // the slot buffer starts at zero and every pass leaves it at zero
+    src << "  for (uint32_t k = 0; k < " << std::max<uint32_t>(max_slots, 1u)
+        << "u; ++k) L[k * 64u + l] = 0u;\n";
   src << R"QCFJIT(
-extern "C" __global__ void __launch_bounds__(64) qcf_jit_kernel(KJ P) {
-  extern __shared__ uint32_t L[];
-  const uint32_t l = threadIdx.x;
-  const FlipLDS flip{L};
-  // complete tiles only: every lane's 32 shots exist
   for (uint32_t blk = blockIdx.x; blk < P.nblocks; blk += gridDim.x) {
     const uint32_t gw = blk * 64u + l;
+    // an opaque per-tile copy of the plane stride keeps the compiler from
+    // hoisting every record address out of the tile loop
+    uint64_t ld = P.ld32;
+    asm volatile("" : "+s"(ld));
     qcfd::Xoshiro g = qcfd::gauge_stream(P.seed, gw);
     qcfd::EvState ev = qcfd::ev_init();
     if (P.has_class) ev = qcfd::gen_batch(ev, P.seed, P.c, blk, l);
     uint32_t acc = 0u;
     (void)acc;
 )QCFJIT";
-  src << body.str();
+  src << "    if (blk < P.nfull) {  // every lane's 32 shots exist\n" << full.str()
+      << "    } else {\n"
+      << "    const bool live = gw < P.nwbuf;\n"
+      << "    const uint32_t lm = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1u ? P.lastmask : "
+         "0xFFFFFFFFu);\n"
+      << part.str() << "    }\n";
   src << "  }\n}\n";
   *out = src.str();
+  if (lds_bytes)
+    *lds_bytes = regs ? std::max<uint32_t>(max_slots, 1u) * 256u : P->nplanes * 256u;
   return true;
 }
 
@@ -239,6 +506,12 @@ constexpr size_t kCodeCacheMax = 32;
 std::unordered_map<std::string, std::vector<char>> g_code_cache;  // source -> code object
 }  // namespace
 
+// Programs with at most this many planes keep them in registers.
+uint32_t jit_reg_planes() {
+  const char *e = getenv("QCF_JIT_REG_PLANES");
+  return e ? (uint32_t)strtoul(e, nullptr, 10) : 192u;
+}
+
 // Programs longer than this many device instructions stay on the interpreter:
 // compile time grows faster than linearly with the straight-line code.
 size_t jit_max_instr() {
@@ -246,20 +519,28 @@ size_t jit_max_instr() {
   return e ? (size_t)strtoull(e, nullptr, 10) : (size_t)12000;
 }
 
+// QCF_JIT_NO_LOOPS=1 emits the whole stream straight-line.
+bool jit_no_loops() {
+  const char *e = getenv("QCF_JIT_NO_LOOPS");
+  return e && e[0] == '1';
+}
+
 bool jit_disabled() {
   const char *e = getenv("QCF_NO_JIT");
   return e && e[0] == '1';
 }
 
-int jit_get(qcf_program *P, int variant, void **function) {
-  *function = nullptr;
+int jit_get(qcf_program *P, int variant, JitKernelInfo *info) {
+  *info = JitKernelInfo{};
   int dev = 0;
   QCF_HIP_CHECK(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lock(g_jit_mu);
   for (auto &j : P->jit)
     if (j.device == dev && j.variant == variant) {
       if (j.failed) return set_error(QCF_ERR_UNSUPPORTED, "specialised kernel unavailable");
-      *function = j.function;
+      info->function = j.function;
+      info->lds_bytes = j.lds_bytes;
+      info->blocks_per_cu = j.blocks_per_cu;
       return QCF_OK;
     }
   qcf_program::JitKernel jk;
@@ -270,7 +551,7 @@ int jit_get(qcf_program *P, int variant, void **function) {
   int st = QCF_OK;
   if (variant_code(P, variant).size() > jit_max_instr())
     st = set_error(QCF_ERR_UNSUPPORTED, "program longer than QCF_JIT_MAX_INSTR instructions");
-  else if (!generate(P, variant, &src))
+  else if (!generate(P, variant, &src, &jk.lds_bytes))
     st = set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   if (st == QCF_OK) {
     // programs with the same source (the same circuit built twice) share one
@@ -289,19 +570,28 @@ int jit_get(qcf_program *P, int variant, void **function) {
   if (st == QCF_OK) {
     hipModule_t m;
     hipFunction_t f;
+    int nb = 0;
     if (hipModuleLoadData(&m, bin.data()) != hipSuccess) {
       st = set_error(QCF_ERR_HIP, "hipModuleLoadData failed");
     } else if (hipModuleGetFunction(&f, m, "qcf_jit_kernel") != hipSuccess) {
       (void)hipModuleUnload(m);
       st = set_error(QCF_ERR_HIP, "hipModuleGetFunction failed");
+    } else if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, 64, jk.lds_bytes) !=
+                   hipSuccess ||
+               nb < 1) {
+      (void)hipModuleUnload(m);
+      st = set_error(QCF_ERR_HIP, "specialised kernel does not fit a compute unit");
     } else {
       jk.module = m;
       jk.function = f;
+      jk.blocks_per_cu = (uint32_t)nb;
     }
   }
   jk.failed = st != QCF_OK;
   P->jit.push_back(jk);
-  *function = jk.function;
+  info->function = jk.function;
+  info->lds_bytes = jk.lds_bytes;
+  info->blocks_per_cu = jk.blocks_per_cu;
   return st;
 }
 
@@ -325,7 +615,7 @@ extern "C" int qcf_debug_jit_source(const qcf_program *prog, int variant, char *
   qcf::clear_error();
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src;
-  if (!qcf::generate(prog, variant & 3, &src))
+  if (!qcf::generate(prog, variant & 3, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   if (needed) *needed = src.size() + 1;
   if (buf && cap) {
@@ -341,7 +631,7 @@ extern "C" int qcf_debug_jit_compile(const qcf_program *prog, int variant, size_
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src, log;
   std::vector<char> bin;
-  if (!qcf::generate(prog, variant & 3, &src))
+  if (!qcf::generate(prog, variant & 3, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   QCF_TRY(qcf::compile(src, &bin, &log));
   if (code_bytes) *code_bytes = bin.size();

@@ -17,15 +17,27 @@ struct JitParams {
   uint64_t ld32;
   uint32_t nblocks, nw32, nwbuf, lastmask;
   float c;
-  uint32_t has_class, pad0, pad1;
+  uint32_t has_class, nfull, pad0;
 };
 
 // QCF_NO_JIT=1 keeps every launch on the interpreter.
 bool jit_disabled();
 
+// Largest plane count kept in registers (QCF_JIT_REG_PLANES, default 192) and
+// longest program specialised (QCF_JIT_MAX_INSTR, default 12000).
+uint32_t jit_reg_planes();
+bool jit_no_loops();
+size_t jit_max_instr();
+
+struct JitKernelInfo {
+  void *function = nullptr;     // hipFunction_t
+  uint32_t lds_bytes = 0;       // dynamic LDS per wave
+  uint32_t blocks_per_cu = 0;   // resident waves per compute unit
+};
+
 // The compiled kernel of `variant` for the current device (compiled on first
 // use, cached in the program).  Returns an error if it cannot be built.
-int jit_get(qcf_program *P, int variant, void **function);
+int jit_get(qcf_program *P, int variant, JitKernelInfo *info);
 
 int jit_launch(void *function, const JitParams &p, uint32_t grid, uint32_t lds_bytes,
                hipStream_t stream);

@@ -494,6 +494,91 @@ struct Compiler {
   }
 };
 
+// Classical slots (planes >= base) read and written by one instruction.
+// Every instruction reads all of its slots before it writes any.
+void slot_fields(Instr &in, uint32_t base, std::vector<uint32_t *> *reads,
+                 std::vector<uint32_t *> *defs) {
+  auto rd = [&](uint32_t &f) {
+    if (f != kNone && f >= base) reads->push_back(&f);
+  };
+  auto df = [&](uint32_t &f) {
+    if (f != kNone && f >= base) defs->push_back(&f);
+  };
+  switch (in.op & 0xFFu) {
+    case I_MEAS: rd(in.a); df(in.c); break;
+    case I_XSTORE2: rd(in.a); rd(in.b); df(in.d); break;
+    case I_ACC_LDS:
+    case I_ACCX_LDS: rd(in.a); break;
+    case I_ACC_STORE: df(in.b); break;
+    case I_MR: df(in.e); break;
+    case I_MRX: rd(in.f); df(in.e); break;
+    default: break;
+  }
+}
+
+// Reassign the classical slots of the body from the live ranges of their
+// values: at each instruction the slots whose values are read for the last
+// time are freed before the values it defines take slots (most recently freed
+// first).  A measurement that replaces its own previous-round value in a
+// detector then lands in the same slot every round, so the rounds of a memory
+// experiment compile to identical instruction blocks.  Returns the slot count.
+uint32_t renumber_slots(std::vector<Instr> &body, uint32_t base) {
+  std::vector<int64_t> cur;      // old slot -> current value
+  std::vector<size_t> last;      // value -> index of its last read (or def)
+  std::vector<uint32_t *> reads, defs;
+  auto val_of = [&](uint32_t slot) -> int64_t & {
+    if (cur.size() <= slot - base) cur.resize(slot - base + 1, -1);
+    return cur[slot - base];
+  };
+  for (size_t i = 0; i < body.size(); ++i) {
+    reads.clear();
+    defs.clear();
+    slot_fields(body[i], base, &reads, &defs);
+    for (uint32_t *f : reads) {
+      const int64_t v = val_of(*f);
+      if (v >= 0) last[v] = i;
+    }
+    for (uint32_t *f : defs) {
+      val_of(*f) = (int64_t)last.size();
+      last.push_back(i);
+    }
+  }
+  std::fill(cur.begin(), cur.end(), -1);
+  std::vector<uint32_t> slot_of(last.size(), kNone), free_list;
+  uint32_t next = base;
+  int64_t nval = 0;
+  std::vector<int64_t> freeing;
+  for (size_t i = 0; i < body.size(); ++i) {
+    reads.clear();
+    defs.clear();
+    slot_fields(body[i], base, &reads, &defs);
+    freeing.clear();
+    for (uint32_t *f : reads) {
+      const int64_t v = val_of(*f);
+      if (v < 0) continue;  // read of a slot never written: leave as is
+      *f = slot_of[v];
+      if (last[v] == i && std::find(freeing.begin(), freeing.end(), v) == freeing.end())
+        freeing.push_back(v);
+    }
+    for (int64_t v : freeing) free_list.push_back(slot_of[v]);
+    for (uint32_t *f : defs) {
+      const int64_t v = nval++;
+      val_of(*f) = v;
+      uint32_t s;
+      if (!free_list.empty()) {
+        s = free_list.back();
+        free_list.pop_back();
+      } else {
+        s = next++;
+      }
+      slot_of[v] = s;
+      *f = s;
+      if (last[v] == i) free_list.push_back(s);  // never read
+    }
+  }
+  return next;
+}
+
 int op_qubit_check(const qcf_op &op, int nqargs, uint32_t nq) {
   if (op.q1 == 0 || (nqargs == 2 && op.q2 == 0))
     return set_error(QCF_ERR_ARGUMENT,
@@ -767,8 +852,8 @@ extern "C" int qcf_program_build(const qcf_op *ops, size_t nops,
     return st;
   }
   C.peephole();
+  P->nplanes = renumber_slots(P->body, 2 * nq);
   C.group_sites();
-  P->nplanes = C.next_slot;
   if (P->nplanes > 0xFFFF) {
     delete P;
     return set_error(QCF_ERR_UNSUPPORTED, "circuit needs %u on-chip planes", P->nplanes);

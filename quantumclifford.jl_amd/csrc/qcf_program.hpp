@@ -116,6 +116,8 @@ struct qcf_program {
     int variant = -1;
     void *module = nullptr;   // hipModule_t
     void *function = nullptr; // hipFunction_t
+    uint32_t lds_bytes = 0;
+    uint32_t blocks_per_cu = 0;
     bool failed = false;
   };
   std::vector<JitKernel> jit;

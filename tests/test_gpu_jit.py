@@ -1,7 +1,7 @@
 """Per-circuit specialised kernels (compiled at run time) against the
 interpreter: for native launches both draw the same events, so records and
-frames must agree bit for bit -- including a partial last tile, which the
-interpreter runs after the specialised kernel has done the complete tiles.
+frames must agree bit for bit -- including the partial last tile, where lanes
+past the last shot store nothing and bits past it are stored as zeros.
 The interpreter itself is pinned to the oracle by the export/replay tests."""
 import numpy as np
 import pytest
@@ -44,11 +44,12 @@ def test_surface_code_specialised_equals_interpreter(gpu, monkeypatch, d, rounds
     assert oracle.unpack_shots(ra, shots).any()
 
 
-def test_record_only_launch_specialised_equals_interpreter(gpu, monkeypatch):
+@pytest.mark.parametrize("shots", [1, 100, 2047, 5 * 2048 + 3])
+def test_record_only_launch_specialised_equals_interpreter(gpu, monkeypatch, shots):
     prog = qcf.Program(circuits.repetition_code(5, 5, 2e-2).ops)
 
     def run():
-        return qcf.pftrajectories(prog, 5 * 2048 + 3, seed=4, keep_frame=False).record_host()
+        return qcf.pftrajectories(prog, shots, seed=4, keep_frame=False).record_host()
 
     a, b = _both(monkeypatch, run)
     assert np.array_equal(a, b)
