@@ -88,35 +88,31 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
 // Noise events of one tile.  Events occur at rate p (one class for the whole
 // program) along the sequence (site ordinal, shot in tile): position
 // ordinal*2048 + shot.  Lane i holds the i-th event of the current batch of 64.
+// Each lane draws its gaps and Pauli choices from its own xoshiro128** stream,
+// seeded once per tile from Philox(tile, lane).
 struct EvState {
   uint32_t pos;    // event position (per lane)
   uint32_t upau;   // uniform for the Pauli choice
   uint32_t uacc;   // 31-bit uniform for thinning; 0xFFFFFFFF = null event
   uint32_t last;   // position held by lane 63 (wave uniform)
-  uint32_t batch;  // batches drawn so far (wave uniform)
+  Xoshiro rng;     // per-lane event stream
 };
 
-__device__ __forceinline__ EvState ev_init() {
-  EvState e{0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u};
+__device__ __forceinline__ EvState ev_init(uint64_t seed, uint32_t tile, uint32_t lane) {
+  const u4 r = philox(u4{tile, lane, 0u, kStreamEvent}, (uint32_t)seed, (uint32_t)(seed >> 32));
+  EvState e{0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu, Xoshiro{r.x, r.y, r.z, r.w | 1u}};
   return e;
 }
 
 // Draw the next 64 events: one geometric gap per lane (failures before the
 // next event, floor(log2(u) * c) with c = 1/log2(1-p)) and a wave prefix sum.
 // Gaps beyond 2^24 trials advance without an event (memorylessness keeps the
-// process exact).  Specialised kernels with planes in LDS keep one
-// out-of-line copy of it; with planes in registers it is inlined (a call would
-// have to save the live planes).
-#if defined(QCF_RTC) && !defined(QCF_INLINE_EVENTS)
-#define QCF_GEN_BATCH_ATTR __attribute__((noinline))
-#else
-#define QCF_GEN_BATCH_ATTR __forceinline__
-#endif
-__device__ QCF_GEN_BATCH_ATTR EvState gen_batch(EvState ev, uint64_t seed, float c,
-                                                uint32_t tile, uint32_t lane) {
-  const u4 r = philox(u4{tile, ev.batch, lane, kStreamEvent}, (uint32_t)seed,
-                      (uint32_t)(seed >> 32));
-  const float uf = ((float)(r.x >> 8) + 0.5f) * (1.0f / 16777216.0f);
+// process exact).
+__device__ __forceinline__ EvState gen_batch(EvState ev, float c) {
+  const uint32_t r0 = ev.rng.next();
+  const uint32_t r1 = ev.rng.next();
+  const uint32_t r2 = ev.rng.next();
+  const float uf = ((float)(r0 >> 8) + 0.5f) * (1.0f / 16777216.0f);
   const float g = __log2f(uf) * c;
   uint32_t incr;
   bool null_ev;
@@ -129,9 +125,8 @@ __device__ QCF_GEN_BATCH_ATTR EvState gen_batch(EvState ev, uint64_t seed, float
   }
   ev.pos = ev.last + wave_inclusive_scan(incr);
   ev.last = readlane(ev.pos, 63);
-  ev.upau = r.y;
-  ev.uacc = null_ev ? 0xFFFFFFFFu : (r.z >> 1);
-  ev.batch += 1;
+  ev.upau = r1;
+  ev.uacc = null_ev ? 0xFFFFFFFFu : (r2 >> 1);
   return ev;
 }
 
@@ -160,9 +155,8 @@ __device__ __forceinline__ uint32_t event_flips(uint32_t u, uint32_t w1, uint32_
 template <class Flip>
 __device__ __forceinline__ void group_events(EvState &ev, uint32_t ord0, uint32_t G,
                                              uint32_t base, uint32_t e0, uint32_t e1,
-                                             uint32_t e2, uint32_t e3, uint32_t pay,
-                                             uint64_t seed, float c, uint32_t tile,
-                                             uint32_t lane, Flip flip) {
+                                             uint32_t e2, uint32_t e3, uint32_t pay, float c,
+                                             Flip flip) {
   const uint32_t lo = ord0 * kTileShots;
   const uint32_t span = G * kTileShots;
   while (true) {
@@ -185,7 +179,7 @@ __device__ __forceinline__ void group_events(EvState &ev, uint32_t ord0, uint32_
       }
     }
     if (ev.last >= lo + span) break;
-    ev = gen_batch(ev, seed, c, tile, lane);
+    ev = gen_batch(ev, c);
   }
 }
 

@@ -104,8 +104,8 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
   const uint64_t goff = gw;
   qcfd::Xoshiro gauge{0, 0, 0, 1};
   if (MODE != kReplay) gauge = qcfd::gauge_stream(P.seed, gw);
-  EvState ev0 = qcfd::ev_init();
-  if (MODE != kReplay && P.nclasses > 0) ev0 = qcfd::gen_batch(ev0, P.seed, P.cls_c[0], blk, lane);
+  EvState ev0 = qcfd::ev_init(P.seed, blk, lane);
+  if (MODE != kReplay && P.nclasses > 0) ev0 = qcfd::gen_batch(ev0, P.cls_c[0]);
   // Every global load in the instruction loop is waited for on the spot, so
   // no load is ever pending where control flow merges (which would make the
   // compiler wait for all outstanding record stores there).
@@ -191,7 +191,7 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
             export_flip<MODE>(P, P.body_rbase + rs + j, blk * 64u + tl, bit);
           };
           qcfd::group_events(ev0, readlane(cA.y, k), G, k + 1, cA.y, cA.z, cB.x, cB.y, cA.w,
-                             P.seed, P.cls_c[0], blk, lane, flip);
+                             P.cls_c[0], flip);
         }
         k += G;
         continue;
@@ -565,7 +565,12 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
       jp.has_class = kp.nclasses > 0 ? 1u : 0u;
       jp.nfull = (uint32_t)(shots / kBlockShots);
       const uint32_t jwpc = waves_per_cu ? waves_per_cu : jk.blocks_per_cu;
-      const uint64_t g1 = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * jwpc);
+      uint64_t g1 = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * jwpc);
+      const uint32_t nx = jit_xcds();
+      if (nx > 1 && g1 >= nx) {
+        g1 -= g1 % nx;
+        jp.xcds = nx;
+      }
       js = jit_launch(jk.function, jp, (uint32_t)g1, jk.lds_bytes, stream);
       if (js == QCF_OK) return QCF_OK;
     }

@@ -41,7 +41,7 @@ struct KJ {
   uint64_t ld32;
   uint32_t nblocks, nw32, nwbuf, lastmask;
   float c;
-  uint32_t has_class, nfull, pad0;
+  uint32_t has_class, nfull, xcds;
 };
 struct FlipLDS {
   uint32_t *L;
@@ -219,7 +219,9 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     ranges = {{0, code.size(), false}};
   }
   const bool regs = P->nplanes <= jit_reg_planes();
+  const bool nt = jit_nt_stores();
   constexpr uint32_t kSlots = 64;  // noise slots per event pass in register mode
+  constexpr size_t kBarrierOps = 30;  // operands per compiler barrier
   struct Group {
     uint32_t site0, count, ord0, slots;
   };
@@ -297,7 +299,10 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     // bits past the last shot are stored as zeros
     auto st_rec = [&](uint32_t col, const std::string &v) {
       const std::string m = at("P.rec", col, in_loop);
-      return tail ? "if (live) " + m + " = (" + v + ") & lm;" : m + " = " + v + ";";
+      const std::string val = tail ? "(" + v + ") & lm" : v;
+      const std::string st = nt ? "__builtin_nontemporal_store(" + val + ", &" + m + ");"
+                                : m + " = " + val + ";";
+      return tail ? "if (live) " + st : st;
     };
     auto ld_rec = [&](uint32_t col) {
       const std::string m = at("P.rec", col, in_loop);
@@ -334,22 +339,30 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
             if (gi + 1 < gend) body << "    uint32_t n0, n1, n2, n3;\n" << load_group(gi + 1, "n");
             body << "    if (P.has_class) qcfd::group_events(ev, " << g.ord0
                  << (in_loop ? "u + ob, " : "u, ") << g.count
-                 << "u, 0u, t0, t1, t2, t3, 0u, P.seed, P.c, blk, l, flip);\n";
+                 << "u, 0u, t0, t1, t2, t3, 0u, P.c, flip);\n";
             if (gi + 1 < gend) body << "    t0 = n0; t1 = n1; t2 = n2; t3 = n3;\n";
             if (regs) {
+              std::vector<uint32_t> touched;
               for (uint32_t k = 0; k < g.count; ++k) {
                 const Instr &s = *sites[g.site0 + k];
                 const uint32_t s0 = site_slot[g.site0 + k];
                 const uint32_t pl[4] = {s.a & 0xFFFFu, s.a >> 16, s.b & 0xFFFFu, s.b >> 16};
                 const uint32_t np = s.b ? 4u : 2u;
                 body << "   ";
-                for (uint32_t j = 0; j < np; ++j)
+                for (uint32_t j = 0; j < np; ++j) {
                   body << " " << PL(pl[j]) << " ^= atomicExch(&" << slot(s0 + j) << ", 0u);";
-                // keeps the compiler from reassociating XOR chains across the
-                // noise (which stretches live ranges far past the register file)
-                body << " asm volatile(\"\" :";
-                for (uint32_t j = 0; j < np; ++j)
-                  body << (j ? ", " : " ") << "\"+v\"(" << PL(pl[j]) << ")";
+                  if (std::find(touched.begin(), touched.end(), pl[j]) == touched.end())
+                    touched.push_back(pl[j]);
+                }
+                body << "\n";
+              }
+              // one barrier per pass keeps the compiler from reassociating XOR
+              // chains across the noise (which stretches live ranges past the
+              // register file) while leaving the exchanges free to overlap
+              for (size_t k0 = 0; k0 < touched.size(); k0 += kBarrierOps) {
+                body << "    asm volatile(\"\" :";
+                for (size_t k = k0; k < std::min(touched.size(), k0 + kBarrierOps); ++k)
+                  body << (k > k0 ? ", " : " ") << "\"+v\"(" << PL(touched[k]) << ")";
                 body << ");\n";
               }
             }
@@ -436,7 +449,6 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   if (!emit(false, full) || !emit(true, part)) return false;
   std::ostringstream src;
   src << "#define QCF_RTC 1\n";
-  if (regs) src << "#define QCF_INLINE_EVENTS 1\n";
   src << "// folded: prologue " << lp.s << ", loop " << lp.R << " x " << lp.L << ", of "
       << code.size() << " instructions\n";
   src << kDeviceSrc << kKernelPrelude;
@@ -449,16 +461,25 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   if (regs)  // the slot buffer starts at zero and every pass leaves it at zero
     src << "  for (uint32_t k = 0; k < " << std::max<uint32_t>(max_slots, 1u)
         << "u; ++k) L[k * 64u + l] = 0u;\n";
+  // Tiles are dealt out per XCD: with X = P.xcds > 1 the grid is a multiple of
+  // X, workgroup i runs on XCD i mod X, and XCD x takes the x-th contiguous
+  // run of tiles, so each XCD's L2 writes whole contiguous stretches of every
+  // record column.
   src << R"QCFJIT(
-  for (uint32_t blk = blockIdx.x; blk < P.nblocks; blk += gridDim.x) {
+  const uint32_t nx = P.xcds ? P.xcds : 1u;
+  const uint32_t per = gridDim.x / nx, xi = blockIdx.x % nx, wi = blockIdx.x / nx;
+  const uint32_t span = (P.nblocks + nx - 1u) / nx;
+  for (uint32_t t = wi; t < span; t += per) {
+    const uint32_t blk = xi * span + t;
+    if (blk >= P.nblocks) break;
     const uint32_t gw = blk * 64u + l;
     // an opaque per-tile copy of the plane stride keeps the compiler from
     // hoisting every record address out of the tile loop
     uint64_t ld = P.ld32;
     asm volatile("" : "+s"(ld));
     qcfd::Xoshiro g = qcfd::gauge_stream(P.seed, gw);
-    qcfd::EvState ev = qcfd::ev_init();
-    if (P.has_class) ev = qcfd::gen_batch(ev, P.seed, P.c, blk, l);
+    qcfd::EvState ev = qcfd::ev_init(P.seed, blk, l);
+    if (P.has_class) ev = qcfd::gen_batch(ev, P.c);
     uint32_t acc = 0u;
     (void)acc;
 )QCFJIT";
@@ -517,6 +538,18 @@ uint32_t jit_reg_planes() {
 size_t jit_max_instr() {
   const char *e = getenv("QCF_JIT_MAX_INSTR");
   return e ? (size_t)strtoull(e, nullptr, 10) : (size_t)12000;
+}
+
+// QCF_JIT_NT=1 writes the record with nontemporal stores.
+bool jit_nt_stores() {
+  const char *e = getenv("QCF_JIT_NT");
+  return e && e[0] == '1';
+}
+
+// XCDs the grid is dealt over (QCF_XCDS, default 8; 1 = plain grid stride).
+uint32_t jit_xcds() {
+  const char *e = getenv("QCF_XCDS");
+  return e ? (uint32_t)strtoul(e, nullptr, 10) : 8u;
 }
 
 // QCF_JIT_NO_LOOPS=1 emits the whole stream straight-line.

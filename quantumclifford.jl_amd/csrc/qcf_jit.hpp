@@ -17,7 +17,7 @@ struct JitParams {
   uint64_t ld32;
   uint32_t nblocks, nw32, nwbuf, lastmask;
   float c;
-  uint32_t has_class, nfull, pad0;
+  uint32_t has_class, nfull, xcds;
 };
 
 // QCF_NO_JIT=1 keeps every launch on the interpreter.
@@ -27,6 +27,8 @@ bool jit_disabled();
 // longest program specialised (QCF_JIT_MAX_INSTR, default 12000).
 uint32_t jit_reg_planes();
 bool jit_no_loops();
+bool jit_nt_stores();
+uint32_t jit_xcds();
 size_t jit_max_instr();
 
 struct JitKernelInfo {
