@@ -305,6 +305,55 @@ int qcf_frame_to_reference(const uint64_t *frame, size_t ld, uint64_t shots,
 int qcf_frame_from_reference(const uint64_t *xzs, uint64_t shots, uint32_t nq,
                              uint64_t *frame, size_t ld, void *stream);
 
+/* ---- matching decoder (host) ------------------------------------------- */
+
+/*
+ * Circuit-level detector error model of a compiled program: every noise site
+ * and Pauli choice is one fault, pushed through the program with all gauge
+ * draws zero; faults with the same detector set and observable mask merge
+ * (rates combined as independent flips).  det_bits / obs_bits are 1-based
+ * record bits (detectors and observables written by ClassicalXOR or
+ * measurements).  Replaces the syndrome/fault bookkeeping that the reference
+ * hands to PyMatching (ext/QuantumCliffordPyQDecodersExt/
+ * QuantumCliffordPyQDecodersExt.jl:83-130); no reference counterpart for
+ * the model itself.  At most 64 observables.
+ */
+typedef struct qcf_dem qcf_dem;
+int qcf_dem_build(const qcf_program *prog, const uint32_t *det_bits, size_t ndet,
+                  const uint32_t *obs_bits, size_t nobs, qcf_dem **out);
+int qcf_dem_free(qcf_dem *dem);
+int qcf_dem_size(const qcf_dem *dem, size_t *nmech, uint32_t *ndet, uint32_t *nobs);
+int qcf_dem_mechanism(const qcf_dem *dem, size_t i, uint32_t *dets, size_t cap,
+                      size_t *ndets, uint64_t *obs, double *p);
+
+/*
+ * Minimum-weight perfect-matching decoder over an error model (the
+ * AbstractSyndromeDecoder contract of src/ecc/decoder_pipeline.jl:1-24:
+ * decode / batchdecode).  Mechanisms flipping one or two detectors are
+ * edges (weight log((1-p)/p)); larger ones are split into existing edges
+ * whose observables add up, else dropped (counted by qcf_matcher_info).
+ *   qcf_matcher_decode: syndromes shots x ndet bytes (row-major) ->
+ *     predicted observable flips shots x nobs bytes, and optionally the
+ *     guessed mechanisms shots x nmech bytes (the `guesses` rows that
+ *     evaluate_guesses, src/ecc/decoder_pipeline.jl:172-189, multiplies by
+ *     the faults matrix).
+ *   qcf_matcher_decode_packed: detector bits packed 64 shots per word,
+ *     ndet planes of ld words (the record layout of qcf_pf_sample).
+ * nthreads <= 0 uses up to 16 host threads.  Host memory only.
+ */
+typedef struct qcf_matcher qcf_matcher;
+int qcf_matcher_create(const qcf_dem *dem, qcf_matcher **out);
+int qcf_matcher_free(qcf_matcher *m);
+int qcf_matcher_info(const qcf_matcher *m, size_t *nedges, uint32_t *dropped);
+int qcf_matcher_decode(const qcf_matcher *m, const uint8_t *syndromes, uint64_t shots,
+                       uint8_t *obs_out, uint8_t *mech_out, int nthreads);
+int qcf_matcher_decode_packed(const qcf_matcher *m, const uint64_t *dets, size_t ld,
+                              uint64_t shots, uint8_t *obs_out, int nthreads);
+
+/* Maximum-weight matching of a dense graph (n x n int64 weights, <= 0 = no
+ * edge; mate[i] = -1 if unmatched).  Test entry for the blossom solver. */
+int qcf_debug_max_weight_matching(int n, const int64_t *w, int32_t *mate);
+
 /* ---- shared accounting ------------------------------------------------- */
 
 /* Frame bit-ops per shot of a compiled circuit (CNOT 2, H 2, S 1, SWAP 4,

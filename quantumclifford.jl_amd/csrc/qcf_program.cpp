@@ -287,6 +287,9 @@ struct Compiler {
       s.t2 = c2 >= c3 ? (1u << 15) : (uint32_t)std::floor(c2 / c3 * 32768.0);
     }
     s.bitops = bitops;
+    s.c1 = c1;
+    s.c2 = c2;
+    s.c3 = c3;
     const uint32_t id = (uint32_t)P->sites.size();
     P->sites.push_back(s);
     site_pe.push_back(c3);
@@ -302,6 +305,7 @@ struct Compiler {
     s.rslot = rs;
     rs += 4;
     s.bitops = 4;
+    s.c3 = e;
     const uint32_t id = (uint32_t)P->sites.size();
     P->sites.push_back(s);
     site_pe.push_back(e);

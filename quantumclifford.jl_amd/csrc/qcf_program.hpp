@@ -73,6 +73,9 @@ struct Site {
   uint32_t t1, t2;   // 2^15-scaled cumulative thresholds
   uint32_t kind;     // 1 or 2 qubits
   uint32_t bitops;   // accounting weight
+  // exact rates (host only): one qubit X = c1, Z = c2 - c1, Y = c3 - c2;
+  // two qubits: any non-identity Pauli with total rate c3
+  double c1, c2, c3;
 };
 
 struct ClassInfo {

@@ -14,6 +14,6 @@ from .pauli import (P, PauliOperator, Tableau, mul_left_, mul_left_row_, mul_lef
 from .frames import (PauliFrame, Program, alloc_masks, frame_from_reference,
                      frame_to_reference, initZ_, lower_circuit, measurements,
                      pfmeasurements, pftrajectories, pftrajectories_state, seed_)
-from . import batch, circuits, randoms
+from . import batch, circuits, decoders, randoms
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -92,6 +92,17 @@ _SIGNATURES = {
     "qcf_frame_to_reference": (_I, [_P, _SZ, _U64, _U32, _P, _P]),
     "qcf_frame_from_reference": (_I, [_P, _U64, _U32, _P, _SZ, _P]),
     "qcf_program_bitops": (C.c_double, [_P]),
+    "qcf_dem_build": (_I, [_P, _P, _SZ, _P, _SZ, C.POINTER(_P)]),
+    "qcf_dem_free": (_I, [_P]),
+    "qcf_dem_size": (_I, [_P, C.POINTER(_SZ), C.POINTER(_U32), C.POINTER(_U32)]),
+    "qcf_dem_mechanism": (_I, [_P, _SZ, _P, _SZ, C.POINTER(_SZ), C.POINTER(_U64),
+                               C.POINTER(C.c_double)]),
+    "qcf_matcher_create": (_I, [_P, C.POINTER(_P)]),
+    "qcf_matcher_free": (_I, [_P]),
+    "qcf_matcher_info": (_I, [_P, C.POINTER(_SZ), C.POINTER(_U32)]),
+    "qcf_matcher_decode": (_I, [_P, _P, _U64, _P, _P, _I]),
+    "qcf_matcher_decode_packed": (_I, [_P, _P, _SZ, _U64, _P, _I]),
+    "qcf_debug_max_weight_matching": (_I, [_I, _P, _P]),
 }
 
 _lib = None
