@@ -354,6 +354,47 @@ int qcf_matcher_decode_packed(const qcf_matcher *m, const uint64_t *dets, size_t
  * edge; mate[i] = -1 if unmatched).  Test entry for the blossom solver. */
 int qcf_debug_max_weight_matching(int n, const int64_t *w, int32_t *mate);
 
+/* ---- neural decoder (device) ------------------------------------------ */
+
+/*
+ * Perceptron decoder ndet -> h1 -> h2 -> h3 -> 1 (GELU; BF16 weights and
+ * activations, FP32 accumulation on the matrix cores, FP32 master weights
+ * with Adam).  Inputs are gathered straight from a packed record (the layout
+ * qcf_pf_sample writes): det_bits / obs_bit are 1-based record bits.  No
+ * reference counterpart; it serves the AbstractSyndromeDecoder contract
+ * (src/ecc/decoder_pipeline.jl:1-24) from the host mirror.
+ *   widths and batches: multiples of 128; the input is zero-padded to a
+ *     multiple of 128.
+ *   parameters (fp32, flat): W1[h1][kin] b1 W2[h2][h1] b2 W3[h3][h2] b3
+ *     w4[h3] b4, kin = ndet rounded up to 128.  set_params resets Adam.
+ *   train_step: gathers the first `batch` shots of `rec`, forward, binary
+ *     cross-entropy on the logit, backward, one Adam step; adds the batch
+ *     mean loss to *loss_accum (device float, may be NULL).
+ *   gradients: the same forward/backward without the update, copied to host.
+ *   infer: logits (device float[shots]) for all shots of `rec`.
+ * All device pointers; `stream` is a hipStream_t (NULL = default stream).
+ */
+typedef struct qcf_mlp qcf_mlp;
+int qcf_mlp_create(const uint32_t *det_bits, uint32_t ndet, uint32_t obs_bit, uint32_t h1,
+                   uint32_t h2, uint32_t h3, uint32_t batch, uint32_t infer_batch,
+                   qcf_mlp **out);
+int qcf_mlp_free(qcf_mlp *m);
+int qcf_mlp_nparams(const qcf_mlp *m, size_t *n);
+int qcf_mlp_set_params(qcf_mlp *m, const float *host, void *stream);
+int qcf_mlp_get_params(const qcf_mlp *m, float *host, void *stream);
+int qcf_mlp_train_step(qcf_mlp *m, const uint64_t *rec, size_t ld, float lr, float beta1,
+                       float beta2, float eps, float *loss_accum, void *stream);
+int qcf_mlp_gradients(qcf_mlp *m, const uint64_t *rec, size_t ld, float *grad_host,
+                      float *loss_host, void *stream);
+int qcf_mlp_infer(qcf_mlp *m, const uint64_t *rec, size_t ld, uint64_t shots, float *logits,
+                  void *stream);
+
+/* BF16 GEMM of the decoder (FP32 out C[M][N]): layout 0 = A[M][K] B[N][K],
+ * 1 = A[M][K] B[K][N], 2 = A[K][M] B[K][N].  M, N multiples of 128, K of 32.
+ * Test entry for the matrix-core kernel. */
+int qcf_debug_gemm_bf16(int layout, int M, int N, int K, const void *A, const void *B,
+                        float *C, void *stream);
+
 /* ---- shared accounting ------------------------------------------------- */
 
 /* Frame bit-ops per shot of a compiled circuit (CNOT 2, H 2, S 1, SWAP 4,

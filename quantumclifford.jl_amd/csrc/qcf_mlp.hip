@@ -1,0 +1,615 @@
+// Neural syndrome decoder: a 4-layer perceptron (detectors -> h1 -> h2 -> h3
+// -> 1 logit of the observable flip, GELU between layers) trained on
+// syndromes that the frame sampler draws on the device.  BF16 weights and
+// activations, FP32 accumulation, FP32 master weights with Adam.
+//
+// No reference counterpart: the reference decodes through
+// AbstractSyndromeDecoder (src/ecc/decoder_pipeline.jl:1-24) with table,
+// belief-propagation and matching back ends; this decoder plugs into the same
+// contract from the host side.
+//
+// Every dense contraction runs in one hand-written GEMM kernel on the gfx950
+// matrix cores (v_mfma_f32_32x32x16_bf16): 128x128 output tiles per
+// 256-thread workgroup (four waves, 64x64 each as 2x2 MFMA tiles), K in steps
+// of 32 through double-buffered LDS.  Operands are read either K-contiguous
+// or transposed (M- or N-contiguous), which covers the forward pass
+// (X W^T), the input gradient (dY W) and the weight gradient (dY^T X)
+// without transposing anything in memory.  Epilogues fuse bias + GELU
+// (forward) and the GELU derivative (backward).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/qcf.h"
+#include "qcf_common.hpp"
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace qcf {
+namespace {
+
+constexpr int kTile = 128;  // output tile edge
+constexpr int kBK = 32;     // K per LDS stage
+constexpr int kLdsRow = kBK + 8;  // padded LDS row (80 bytes)
+
+enum Epi : int {
+  kEpiStoreF32 = 0,   // C (fp32) = acc
+  kEpiBiasGelu = 1,   // pre = acc + bias (fp32); act = gelu(pre) (bf16)
+  kEpiDGelu = 2,      // out = acc * gelu'(pre_in) (bf16)
+  kEpiBiasGeluInfer = 3,  // act = gelu(acc + bias) (bf16), no pre-activation kept
+};
+
+struct GemmArgs {
+  const bf16 *A, *B;
+  int M, N, K, lda, ldb, ldc;
+  float *Cf;           // kEpiStoreF32 output / kEpiBiasGelu pre-activation
+  bf16 *Cb;            // bf16 output
+  const float *bias;   // per column
+  const float *pre_in; // kEpiDGelu
+};
+
+__device__ __forceinline__ float gelu(float x) {
+  return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
+}
+__device__ __forceinline__ float dgelu(float x) {
+  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+  const float pdf = 0.39894228040143268f * __expf(-0.5f * x * x);
+  return cdf + x * pdf;
+}
+
+// One operand tile (128 rows of the output dimension x 32 of K) moves from
+// global memory to LDS as [row][k].  K-contiguous operands move 16 bytes per
+// thread-chunk; transposed ones are read along the output dimension and
+// scattered into the [row][k] layout.
+template <bool KMAJOR>
+__device__ __forceinline__ void load_tile(const bf16 *G, int ld, int r0, int k0, int t,
+                                          uint4 (&regs)[2]) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int c = t + h * 256;
+    if (KMAJOR) {
+      const int row = c >> 2, kc = (c & 3) * 8;
+      regs[h] = *(const uint4 *)(G + (size_t)(r0 + row) * ld + k0 + kc);
+    } else {
+      const int k = c >> 4, rc = (c & 15) * 8;
+      regs[h] = *(const uint4 *)(G + (size_t)(k0 + k) * ld + r0 + rc);
+    }
+  }
+}
+
+template <bool KMAJOR>
+__device__ __forceinline__ void store_tile(bf16 (*S)[kLdsRow], int t, const uint4 (&regs)[2]) {
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int c = t + h * 256;
+    if (KMAJOR) {
+      const int row = c >> 2, kc = (c & 3) * 8;
+      *(uint4 *)&S[row][kc] = regs[h];
+    } else {
+      const int k = c >> 4, rc = (c & 15) * 8;
+      const bf16 *v = (const bf16 *)&regs[h];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) S[rc + j][k] = v[j];
+    }
+  }
+}
+
+// C[m][n] = sum_k A(m,k) B(k,n) with A(m,k) = A[m*lda+k] (A_KMAJOR) or
+// A[k*lda+m], and B(k,n) = B[n*ldb+k] (B_KMAJOR) or B[k*ldb+n].
+// Requires M, N multiples of 128 and K a multiple of 32 (checked on the host).
+template <bool A_KMAJOR, bool B_KMAJOR, int EPI>
+__global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16 As[2][kTile][kLdsRow];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[2][kTile][kLdsRow];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = w >> 1, wn = w & 1;
+  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+  uint4 ra[2], rb[2];
+  load_tile<A_KMAJOR>(g.A, g.lda, m0, 0, t, ra);
+  load_tile<B_KMAJOR>(g.B, g.ldb, n0, 0, t, rb);
+  store_tile<A_KMAJOR>(As[0], t, ra);
+  store_tile<B_KMAJOR>(Bs[0], t, rb);
+  __syncthreads();
+  const int KT = g.K / kBK;
+  for (int kt = 0; kt < KT; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < KT) {
+      load_tile<A_KMAJOR>(g.A, g.lda, m0, (kt + 1) * kBK, t, ra);
+      load_tile<B_KMAJOR>(g.B, g.ldb, n0, (kt + 1) * kBK, t, rb);
+    }
+#pragma unroll
+    for (int kk = 0; kk < kBK; kk += 16) {
+      bf16x8 a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        a[i] = *(const bf16x8 *)&As[cur][wm * 64 + i * 32 + (lane & 31)][kk + 8 * (lane >> 5)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        b[j] = *(const bf16x8 *)&Bs[cur][wn * 64 + j * 32 + (lane & 31)][kk + 8 * (lane >> 5)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (kt + 1 < KT) {
+      store_tile<A_KMAJOR>(As[cur ^ 1], t, ra);
+      store_tile<B_KMAJOR>(Bs[cur ^ 1], t, rb);
+    }
+    __syncthreads();
+  }
+  // epilogue: lane holds column (lane & 31) of each 32x32 tile, rows
+  // 8*(r/4) + 4*(lane/32) + r%4
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = n0 + wn * 64 + j * 32 + (lane & 31);
+      const float bn = (EPI == kEpiBiasGelu || EPI == kEpiBiasGeluInfer) ? g.bias[n] : 0.0f;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const size_t o = (size_t)m * g.ldc + n;
+        const float v = acc[i][j][r];
+        if (EPI == kEpiStoreF32) {
+          g.Cf[o] = v;
+        } else if (EPI == kEpiBiasGelu) {
+          const float p = v + bn;
+          g.Cf[o] = p;
+          g.Cb[o] = (bf16)gelu(p);
+        } else if (EPI == kEpiBiasGeluInfer) {
+          g.Cb[o] = (bf16)gelu(v + bn);
+        } else {
+          g.Cb[o] = (bf16)(v * dgelu(g.pre_in[o]));
+        }
+      }
+    }
+}
+
+template <bool AK, bool BK, int EPI>
+int launch_gemm(const GemmArgs &g, hipStream_t s) {
+  if (g.M % kTile || g.N % kTile || g.K % kBK || g.M <= 0 || g.N <= 0 || g.K <= 0)
+    return set_error(QCF_ERR_DIMENSION, "gemm: M=%d N=%d K=%d must be multiples of %d/%d/%d",
+                     g.M, g.N, g.K, kTile, kTile, kBK);
+  hipLaunchKernelGGL((gemm_bf16<AK, BK, EPI>), dim3(g.N / kTile, g.M / kTile), dim3(256), 0, s,
+                     g);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+// Syndromes of `rows` shots starting at shot `s0` of a packed record: x[m][j]
+// = detector j of shot m (0 past the detectors and past the shots), y[m] =
+// observable.
+__global__ void gather_inputs(const uint64_t *rec, size_t ld, uint64_t s0, uint64_t nshots,
+                              const uint32_t *det_cols, uint32_t ndet, uint32_t kin,
+                              uint32_t obs_col, int with_y, bf16 *x, float *y, uint32_t rows) {
+  const uint32_t m = blockIdx.x;
+  if (m >= rows) return;
+  const uint64_t s = s0 + m;
+  const bool live = s < nshots;
+  const uint64_t wi = s >> 6, bit = s & 63;
+  for (uint32_t j = threadIdx.x; j < kin; j += blockDim.x) {
+    float v = 0.0f;
+    if (live && j < ndet) v = (float)((rec[(size_t)det_cols[j] * ld + wi] >> bit) & 1u);
+    x[(size_t)m * kin + j] = (bf16)v;
+  }
+  if (with_y && threadIdx.x == 0)
+    y[m] = live ? (float)((rec[(size_t)obs_col * ld + wi] >> bit) & 1u) : 0.0f;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// Output layer h3 -> 1: z[m] = a3[m] . w4 + b4, one wave per row.  With
+// labels: loss += mean(softplus(z) - y z), dz = (sigmoid(z) - y) / rows.
+__global__ void head_forward(const bf16 *a3, const bf16 *w4, const float *b4, uint32_t h3,
+                             uint32_t rows, float *z, const float *y, float *dz, float *loss,
+                             float inv_rows) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t m = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (m >= rows) return;
+  float s = 0.0f;
+  for (uint32_t k = lane; k < h3; k += 64) s += (float)a3[(size_t)m * h3 + k] * (float)w4[k];
+  s = wave_sum(s) + b4[0];
+  if (lane == 0) {
+    z[m] = s;
+    if (y) {
+      const float yy = y[m];
+      const float sp = s > 0 ? s + log1pf(__expf(-s)) : log1pf(__expf(s));
+      const float sg = 1.0f / (1.0f + __expf(-s));
+      dz[m] = (sg - yy) * inv_rows;
+      if (loss) atomicAdd(loss, (sp - yy * s) * inv_rows);
+    }
+  }
+}
+
+// d3[m][k] = dz[m] * w4[k] * gelu'(pre3[m][k])
+__global__ void head_backward(const float *dz, const bf16 *w4, const float *pre3, uint32_t h3,
+                              uint32_t rows, bf16 *d3) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)rows * h3) return;
+  const uint32_t m = (uint32_t)(i / h3), k = (uint32_t)(i % h3);
+  d3[i] = (bf16)(dz[m] * (float)w4[k] * dgelu(pre3[i]));
+}
+
+// Column sums out[n] = sum_m D[m][n] * (scale ? scale[m] : 1), in two fixed
+// passes (partials per 128-row chunk, then their sum) so the result does not
+// depend on scheduling.
+__global__ void colsum_partial(const bf16 *D, const float *scale, uint32_t rows, uint32_t cols,
+                               float *part) {
+  const uint32_t n = blockIdx.x * 64 + (threadIdx.x & 63);
+  const uint32_t r0 = blockIdx.y * 128;
+  __shared__ float sh[4][64];
+  float s = 0.0f;
+  if (n < cols)
+    for (uint32_t m = r0 + (threadIdx.x >> 6); m < min(rows, r0 + 128); m += 4)
+      s += (float)D[(size_t)m * cols + n] * (scale ? scale[m] : 1.0f);
+  sh[threadIdx.x >> 6][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (threadIdx.x < 64 && n < cols)
+    part[(size_t)blockIdx.y * cols + n] = sh[0][threadIdx.x] + sh[1][threadIdx.x] +
+                                          sh[2][threadIdx.x] + sh[3][threadIdx.x];
+}
+
+__global__ void colsum_final(const float *part, uint32_t nchunks, uint32_t cols, float *out) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= cols) return;
+  float s = 0.0f;
+  for (uint32_t c = 0; c < nchunks; ++c) s += part[(size_t)c * cols + n];
+  out[n] = s;
+}
+
+__global__ void vec_sum(const float *v, uint32_t n, float *out) {
+  __shared__ float sh[256];
+  float s = 0.0f;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) s += v[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// Adam (Kingma & Ba 2015) on the fp32 master weights; refreshes the bf16 copy.
+__global__ void adam_step(float *w, float *mo, float *ve, const float *g, bf16 *wb, size_t n,
+                          float lr, float b1, float b2, float eps, float c1, float c2) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float m = b1 * mo[i] + (1.0f - b1) * gi;
+  const float v = b2 * ve[i] + (1.0f - b2) * gi * gi;
+  mo[i] = m;
+  ve[i] = v;
+  const float x = w[i] - lr * (m * c1) / (sqrtf(v * c2) + eps);
+  w[i] = x;
+  wb[i] = (bf16)x;
+}
+
+__global__ void to_bf16(const float *w, bf16 *wb, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) wb[i] = (bf16)w[i];
+}
+
+}  // namespace
+}  // namespace qcf
+
+// Parameter layout (fp32, flat): W1[h1][kin] b1[h1] W2[h2][h1] b2[h2]
+// W3[h3][h2] b3[h3] w4[h3] b4[1]; kin = ndet rounded up to 128 (the padding
+// columns of W1 see zero inputs).
+struct qcf_mlp {
+  int device = 0;
+  uint32_t ndet = 0, kin = 0, h1 = 0, h2 = 0, h3 = 0, batch = 0, ibatch = 0, obs_col = 0;
+  size_t off[8] = {};
+  size_t nparam = 0;
+  uint64_t step = 0;
+  uint32_t *det_cols = nullptr;
+  float *w = nullptr, *mo = nullptr, *ve = nullptr, *g = nullptr;
+  bf16 *wb = nullptr;
+  // activations, sized for max(batch, ibatch) rows
+  bf16 *x = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr;
+  bf16 *d1 = nullptr, *d2 = nullptr, *d3 = nullptr;
+  float *pre1 = nullptr, *pre2 = nullptr, *pre3 = nullptr;
+  float *y = nullptr, *z = nullptr, *dz = nullptr, *part = nullptr, *lossbuf = nullptr;
+  std::vector<void *> allocs;
+  ~qcf_mlp() {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device);
+    for (void *p : allocs) (void)hipFree(p);
+    (void)hipSetDevice(cur);
+  }
+};
+
+namespace qcf {
+namespace {
+
+template <class T>
+int dev_alloc(qcf_mlp *M, T **p, size_t n) {
+  void *q = nullptr;
+  if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
+    return set_error(QCF_ERR_ALLOC, "hipMalloc of %zu bytes failed", n * sizeof(T));
+  M->allocs.push_back(q);
+  *p = (T *)q;
+  return QCF_OK;
+}
+
+int forward(qcf_mlp *M, uint32_t rows, bool train, hipStream_t s) {
+  GemmArgs g{};
+  const bf16 *W1 = M->wb + M->off[0], *W2 = M->wb + M->off[2], *W3 = M->wb + M->off[4];
+  const float *b1 = M->w + M->off[1], *b2 = M->w + M->off[3], *b3 = M->w + M->off[5];
+  auto layer = [&](const bf16 *X, int kin, const bf16 *W, const float *b, int nout, float *pre,
+                   bf16 *act) {
+    g = GemmArgs{X, W, (int)rows, nout, kin, kin, kin, nout, pre, act, b, nullptr};
+    return train ? launch_gemm<true, true, kEpiBiasGelu>(g, s)
+                 : launch_gemm<true, true, kEpiBiasGeluInfer>(g, s);
+  };
+  QCF_TRY(layer(M->x, M->kin, W1, b1, M->h1, M->pre1, M->a1));
+  QCF_TRY(layer(M->a1, M->h1, W2, b2, M->h2, M->pre2, M->a2));
+  QCF_TRY(layer(M->a2, M->h2, W3, b3, M->h3, M->pre3, M->a3));
+  return QCF_OK;
+}
+
+int colsum(qcf_mlp *M, const bf16 *D, const float *scale, uint32_t rows, uint32_t cols,
+           float *out, hipStream_t s) {
+  const uint32_t nch = (rows + 127) / 128;
+  hipLaunchKernelGGL(colsum_partial, dim3((cols + 63) / 64, nch), dim3(256), 0, s, D, scale,
+                     rows, cols, M->part);
+  hipLaunchKernelGGL(colsum_final, dim3((cols + 255) / 256), dim3(256), 0, s, M->part, nch, cols,
+                     out);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+// Forward + backward of one batch already gathered into x / y; gradients in g.
+int fwd_bwd(qcf_mlp *M, float *loss, hipStream_t s) {
+  const uint32_t B = M->batch;
+  QCF_TRY(forward(M, B, true, s));
+  const bf16 *w4 = M->wb + M->off[6];
+  hipLaunchKernelGGL(head_forward, dim3(B / 4), dim3(256), 0, s, M->a3, w4, M->w + M->off[7],
+                     M->h3, B, M->z, M->y, M->dz, loss, 1.0f / (float)B);
+  hipLaunchKernelGGL(head_backward, dim3((unsigned)(((size_t)B * M->h3 + 255) / 256)), dim3(256),
+                     0, s, M->dz, w4, M->pre3, M->h3, B, M->d3);
+  QCF_HIP_CHECK(hipGetLastError());
+  float *G = M->g;
+  // output layer: grad w4 = a3^T dz, grad b4 = sum dz
+  QCF_TRY(colsum(M, M->a3, M->dz, B, M->h3, G + M->off[6], s));
+  hipLaunchKernelGGL(vec_sum, dim3(1), dim3(256), 0, s, M->dz, B, G + M->off[7]);
+  // layer 3: grad W3 = d3^T a2, grad b3 = colsum d3, d2 = (d3 W3) * gelu'(pre2)
+  const bf16 *W2 = M->wb + M->off[2], *W3 = M->wb + M->off[4];
+  GemmArgs g{};
+  g = GemmArgs{M->d3, M->a2, (int)M->h3, (int)M->h2, (int)B, (int)M->h3, (int)M->h2, (int)M->h2,
+               G + M->off[4], nullptr, nullptr, nullptr};
+  QCF_TRY((launch_gemm<false, false, kEpiStoreF32>(g, s)));
+  QCF_TRY(colsum(M, M->d3, nullptr, B, M->h3, G + M->off[5], s));
+  g = GemmArgs{M->d3, W3, (int)B, (int)M->h2, (int)M->h3, (int)M->h3, (int)M->h2, (int)M->h2,
+               nullptr, M->d2, nullptr, M->pre2};
+  QCF_TRY((launch_gemm<true, false, kEpiDGelu>(g, s)));
+  // layer 2
+  g = GemmArgs{M->d2, M->a1, (int)M->h2, (int)M->h1, (int)B, (int)M->h2, (int)M->h1, (int)M->h1,
+               G + M->off[2], nullptr, nullptr, nullptr};
+  QCF_TRY((launch_gemm<false, false, kEpiStoreF32>(g, s)));
+  QCF_TRY(colsum(M, M->d2, nullptr, B, M->h2, G + M->off[3], s));
+  g = GemmArgs{M->d2, W2, (int)B, (int)M->h1, (int)M->h2, (int)M->h2, (int)M->h1, (int)M->h1,
+               nullptr, M->d1, nullptr, M->pre1};
+  QCF_TRY((launch_gemm<true, false, kEpiDGelu>(g, s)));
+  // layer 1 (no input gradient)
+  g = GemmArgs{M->d1, M->x, (int)M->h1, (int)M->kin, (int)B, (int)M->h1, (int)M->kin,
+               (int)M->kin, G + M->off[0], nullptr, nullptr, nullptr};
+  QCF_TRY((launch_gemm<false, false, kEpiStoreF32>(g, s)));
+  QCF_TRY(colsum(M, M->d1, nullptr, B, M->h1, G + M->off[1], s));
+  return QCF_OK;
+}
+
+int gather(qcf_mlp *M, const uint64_t *rec, size_t ld, uint64_t s0, uint64_t nshots,
+           uint32_t rows, bool with_y, hipStream_t s) {
+  hipLaunchKernelGGL(gather_inputs, dim3(rows), dim3(128), 0, s, rec, ld, s0, nshots,
+                     M->det_cols, M->ndet, M->kin, M->obs_col, with_y ? 1 : 0, M->x, M->y, rows);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+}  // namespace
+}  // namespace qcf
+
+using namespace qcf;
+
+extern "C" int qcf_mlp_create(const uint32_t *det_bits, uint32_t ndet, uint32_t obs_bit,
+                              uint32_t h1, uint32_t h2, uint32_t h3, uint32_t batch,
+                              uint32_t infer_batch, qcf_mlp **out) {
+  clear_error();
+  if (!out || (ndet && !det_bits)) return set_error(QCF_ERR_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (ndet == 0 || obs_bit == 0) return set_error(QCF_ERR_ARGUMENT, "need detectors and an observable");
+  for (uint32_t v : {h1, h2, h3, batch, infer_batch})
+    if (v == 0 || v % 128)
+      return set_error(QCF_ERR_ARGUMENT, "layer widths and batches must be multiples of 128 (got %u)", v);
+  for (uint32_t j = 0; j < ndet; ++j)
+    if (det_bits[j] == 0) return set_error(QCF_ERR_ARGUMENT, "record bits are 1-based");
+  qcf_mlp *M = new qcf_mlp();
+  (void)hipGetDevice(&M->device);
+  M->ndet = ndet;
+  M->kin = (ndet + 127) / 128 * 128;
+  M->h1 = h1;
+  M->h2 = h2;
+  M->h3 = h3;
+  M->batch = batch;
+  M->ibatch = infer_batch;
+  M->obs_col = obs_bit - 1;
+  size_t o = 0;
+  const size_t sizes[8] = {(size_t)h1 * M->kin, h1, (size_t)h2 * h1, h2, (size_t)h3 * h2, h3, h3, 1};
+  for (int i = 0; i < 8; ++i) {
+    M->off[i] = o;
+    o += sizes[i];
+  }
+  M->nparam = o;
+  const size_t R = std::max(batch, infer_batch);
+  int st = QCF_OK;
+  auto A = [&](auto **p, size_t n) {
+    if (st == QCF_OK) st = dev_alloc(M, p, n);
+  };
+  A(&M->det_cols, ndet);
+  A(&M->w, o);
+  A(&M->mo, o);
+  A(&M->ve, o);
+  A(&M->g, o);
+  A(&M->wb, o);
+  A(&M->x, R * M->kin);
+  A(&M->a1, R * h1);
+  A(&M->a2, R * h2);
+  A(&M->a3, R * h3);
+  A(&M->d1, (size_t)batch * h1);
+  A(&M->d2, (size_t)batch * h2);
+  A(&M->d3, (size_t)batch * h3);
+  A(&M->pre1, (size_t)batch * h1);
+  A(&M->pre2, (size_t)batch * h2);
+  A(&M->pre3, (size_t)batch * h3);
+  A(&M->y, R);
+  A(&M->z, R);
+  A(&M->dz, R);
+  A(&M->lossbuf, 1);
+  A(&M->part, (size_t)((batch + 127) / 128) * std::max(std::max(h1, h2), std::max(h3, M->kin)));
+  if (st == QCF_OK) {
+    std::vector<uint32_t> cols(ndet);
+    for (uint32_t j = 0; j < ndet; ++j) cols[j] = det_bits[j] - 1;
+    if (hipMemcpy(M->det_cols, cols.data(), ndet * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(M->w, 0, o * 4) != hipSuccess || hipMemset(M->wb, 0, o * 2) != hipSuccess ||
+        hipMemset(M->mo, 0, o * 4) != hipSuccess || hipMemset(M->ve, 0, o * 4) != hipSuccess)
+      st = set_error(QCF_ERR_HIP, "initialising the decoder buffers failed");
+  }
+  if (st != QCF_OK) {
+    delete M;
+    return st;
+  }
+  *out = M;
+  return QCF_OK;
+}
+
+extern "C" int qcf_mlp_free(qcf_mlp *m) {
+  delete m;
+  return QCF_OK;
+}
+
+extern "C" int qcf_mlp_nparams(const qcf_mlp *m, size_t *n) {
+  clear_error();
+  if (!m || !n) return set_error(QCF_ERR_ARGUMENT, "null argument");
+  *n = m->nparam;
+  return QCF_OK;
+}
+
+extern "C" int qcf_mlp_set_params(qcf_mlp *m, const float *host, void *stream) {
+  clear_error();
+  if (!m || !host) return set_error(QCF_ERR_ARGUMENT, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  QCF_HIP_CHECK(hipMemcpyAsync(m->w, host, m->nparam * 4, hipMemcpyHostToDevice, s));
+  QCF_HIP_CHECK(hipMemsetAsync(m->mo, 0, m->nparam * 4, s));
+  QCF_HIP_CHECK(hipMemsetAsync(m->ve, 0, m->nparam * 4, s));
+  hipLaunchKernelGGL(to_bf16, dim3((unsigned)((m->nparam + 255) / 256)), dim3(256), 0, s, m->w,
+                     m->wb, m->nparam);
+  QCF_HIP_CHECK(hipGetLastError());
+  QCF_HIP_CHECK(hipStreamSynchronize(s));
+  m->step = 0;
+  return QCF_OK;
+}
+
+extern "C" int qcf_mlp_get_params(const qcf_mlp *m, float *host, void *stream) {
+  clear_error();
+  if (!m || !host) return set_error(QCF_ERR_ARGUMENT, "null argument");
+  QCF_HIP_CHECK(hipMemcpyAsync(host, m->w, m->nparam * 4, hipMemcpyDeviceToHost,
+                               (hipStream_t)stream));
+  QCF_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return QCF_OK;
+}
+
+extern "C" int qcf_mlp_gradients(qcf_mlp *m, const uint64_t *rec, size_t ld, float *grad_host,
+                                 float *loss_host, void *stream) {
+  clear_error();
+  if (!m || !rec || !grad_host || !loss_host) return set_error(QCF_ERR_ARGUMENT, "null argument");
+  if (ld * 64 < m->batch) return set_error(QCF_ERR_DIMENSION, "record holds fewer shots than a batch");
+  hipStream_t s = (hipStream_t)stream;
+  QCF_TRY(gather(m, rec, ld, 0, m->batch, m->batch, true, s));
+  float *loss = m->lossbuf;
+  QCF_HIP_CHECK(hipMemsetAsync(loss, 0, 4, s));
+  QCF_TRY(fwd_bwd(m, loss, s));
+  QCF_HIP_CHECK(hipMemcpyAsync(grad_host, m->g, m->nparam * 4, hipMemcpyDeviceToHost, s));
+  QCF_HIP_CHECK(hipMemcpyAsync(loss_host, loss, 4, hipMemcpyDeviceToHost, s));
+  QCF_HIP_CHECK(hipStreamSynchronize(s));
+  return QCF_OK;
+}
+
+extern "C" int qcf_mlp_train_step(qcf_mlp *m, const uint64_t *rec, size_t ld, float lr,
+                                  float beta1, float beta2, float eps, float *loss_accum,
+                                  void *stream) {
+  clear_error();
+  if (!m || !rec) return set_error(QCF_ERR_ARGUMENT, "null argument");
+  if (ld * 64 < m->batch) return set_error(QCF_ERR_DIMENSION, "record holds fewer shots than a batch");
+  hipStream_t s = (hipStream_t)stream;
+  QCF_TRY(gather(m, rec, ld, 0, m->batch, m->batch, true, s));
+  QCF_TRY(fwd_bwd(m, loss_accum, s));
+  m->step += 1;
+  const float c1 = 1.0f / (1.0f - std::pow(beta1, (float)m->step));
+  const float c2 = 1.0f / (1.0f - std::pow(beta2, (float)m->step));
+  hipLaunchKernelGGL(adam_step, dim3((unsigned)((m->nparam + 255) / 256)), dim3(256), 0, s, m->w,
+                     m->mo, m->ve, m->g, m->wb, m->nparam, lr, beta1, beta2, eps, c1, c2);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+extern "C" int qcf_mlp_infer(qcf_mlp *m, const uint64_t *rec, size_t ld, uint64_t shots,
+                             float *logits, void *stream) {
+  clear_error();
+  if (!m || (shots && (!rec || !logits))) return set_error(QCF_ERR_ARGUMENT, "null argument");
+  if (ld * 64 < shots) return set_error(QCF_ERR_DIMENSION, "ld %zu words < %llu shots", ld,
+                                         (unsigned long long)shots);
+  hipStream_t s = (hipStream_t)stream;
+  for (uint64_t s0 = 0; s0 < shots; s0 += m->ibatch) {
+    const uint32_t rows = (uint32_t)std::min<uint64_t>(m->ibatch, shots - s0);
+    const uint32_t rows128 = (rows + 127) / 128 * 128;
+    QCF_TRY(gather(m, rec, ld, s0, shots, rows128, false, s));
+    QCF_TRY(forward(m, rows128, false, s));
+    hipLaunchKernelGGL(head_forward, dim3(rows128 / 4), dim3(256), 0, s, m->a3,
+                       m->wb + m->off[6], m->w + m->off[7], m->h3, rows, logits + s0,
+                       (const float *)nullptr, (float *)nullptr, (float *)nullptr, 0.0f);
+    QCF_HIP_CHECK(hipGetLastError());
+  }
+  return QCF_OK;
+}
+
+extern "C" int qcf_debug_gemm_bf16(int layout, int M, int N, int K, const void *A, const void *B,
+                                   float *C, void *stream) {
+  clear_error();
+  GemmArgs g{(const bf16 *)A, (const bf16 *)B, M, N, K, 0, 0, N, C, nullptr, nullptr, nullptr};
+  hipStream_t s = (hipStream_t)stream;
+  switch (layout) {
+    case 0:  // A[M][K], B[N][K]
+      g.lda = K;
+      g.ldb = K;
+      return launch_gemm<true, true, kEpiStoreF32>(g, s);
+    case 1:  // A[M][K], B[K][N]
+      g.lda = K;
+      g.ldb = N;
+      return launch_gemm<true, false, kEpiStoreF32>(g, s);
+    case 2:  // A[K][M], B[K][N]
+      g.lda = M;
+      g.ldb = N;
+      return launch_gemm<false, false, kEpiStoreF32>(g, s);
+    default:
+      return set_error(QCF_ERR_ARGUMENT, "layout %d", layout);
+  }
+}

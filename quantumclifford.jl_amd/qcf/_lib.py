@@ -103,6 +103,16 @@ _SIGNATURES = {
     "qcf_matcher_decode": (_I, [_P, _P, _U64, _P, _P, _I]),
     "qcf_matcher_decode_packed": (_I, [_P, _P, _SZ, _U64, _P, _I]),
     "qcf_debug_max_weight_matching": (_I, [_I, _P, _P]),
+    "qcf_mlp_create": (_I, [_P, _U32, _U32, _U32, _U32, _U32, _U32, _U32, C.POINTER(_P)]),
+    "qcf_mlp_free": (_I, [_P]),
+    "qcf_mlp_nparams": (_I, [_P, C.POINTER(_SZ)]),
+    "qcf_mlp_set_params": (_I, [_P, _P, _P]),
+    "qcf_mlp_get_params": (_I, [_P, _P, _P]),
+    "qcf_mlp_train_step": (_I, [_P, _P, _SZ, C.c_float, C.c_float, C.c_float, C.c_float, _P,
+                                _P]),
+    "qcf_mlp_gradients": (_I, [_P, _P, _SZ, _P, _P, _P]),
+    "qcf_mlp_infer": (_I, [_P, _P, _SZ, _U64, _P, _P]),
+    "qcf_debug_gemm_bf16": (_I, [_I, _I, _I, _I, _P, _P, _P, _P]),
 }
 
 _lib = None
