@@ -367,9 +367,10 @@ int qcf_debug_max_weight_matching(int n, const int64_t *w, int32_t *mate);
  *     multiple of 128.
  *   parameters (fp32, flat): W1[h1][kin] b1 W2[h2][h1] b2 W3[h3][h2] b3
  *     w4[h3] b4, kin = ndet rounded up to 128.  set_params resets Adam.
- *   train_step: gathers the first `batch` shots of `rec`, forward, binary
- *     cross-entropy on the logit, backward, one Adam step; adds the batch
- *     mean loss to *loss_accum (device float, may be NULL).
+ *   train_step: gathers shots shot0 .. shot0+batch-1 of `rec` (shot0 a
+ *     multiple of 64), forward, binary cross-entropy on the logit, backward,
+ *     one Adam step; adds the batch mean loss to *loss_accum (device float,
+ *     may be NULL).
  *   gradients: the same forward/backward without the update, copied to host.
  *   infer: logits (device float[shots]) for all shots of `rec`.
  * All device pointers; `stream` is a hipStream_t (NULL = default stream).
@@ -382,10 +383,10 @@ int qcf_mlp_free(qcf_mlp *m);
 int qcf_mlp_nparams(const qcf_mlp *m, size_t *n);
 int qcf_mlp_set_params(qcf_mlp *m, const float *host, void *stream);
 int qcf_mlp_get_params(const qcf_mlp *m, float *host, void *stream);
-int qcf_mlp_train_step(qcf_mlp *m, const uint64_t *rec, size_t ld, float lr, float beta1,
-                       float beta2, float eps, float *loss_accum, void *stream);
-int qcf_mlp_gradients(qcf_mlp *m, const uint64_t *rec, size_t ld, float *grad_host,
-                      float *loss_host, void *stream);
+int qcf_mlp_train_step(qcf_mlp *m, const uint64_t *rec, size_t ld, uint64_t shot0, float lr,
+                       float beta1, float beta2, float eps, float *loss_accum, void *stream);
+int qcf_mlp_gradients(qcf_mlp *m, const uint64_t *rec, size_t ld, uint64_t shot0,
+                      float *grad_host, float *loss_host, void *stream);
 int qcf_mlp_infer(qcf_mlp *m, const uint64_t *rec, size_t ld, uint64_t shots, float *logits,
                   void *stream);
 

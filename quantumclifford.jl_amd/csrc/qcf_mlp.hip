@@ -189,24 +189,49 @@ int launch_gemm(const GemmArgs &g, hipStream_t s) {
   return QCF_OK;
 }
 
-// Syndromes of `rows` shots starting at shot `s0` of a packed record: x[m][j]
-// = detector j of shot m (0 past the detectors and past the shots), y[m] =
-// observable.
+// Syndromes of `rows` shots starting at shot `s0` (a multiple of 64) of a
+// packed record: x[m][j] = detector j of shot m (0 past the detectors and past
+// the shots), optionally the transpose xT[j][m], and y[m] = observable.  One
+// workgroup per 64 shots: each detector word is read once.
 __global__ void gather_inputs(const uint64_t *rec, size_t ld, uint64_t s0, uint64_t nshots,
                               const uint32_t *det_cols, uint32_t ndet, uint32_t kin,
-                              uint32_t obs_col, int with_y, bf16 *x, float *y, uint32_t rows) {
-  const uint32_t m = blockIdx.x;
-  if (m >= rows) return;
-  const uint64_t s = s0 + m;
-  const bool live = s < nshots;
-  const uint64_t wi = s >> 6, bit = s & 63;
+                              uint32_t obs_col, float *y, uint16_t *x, uint16_t *xT,
+                              uint32_t rows) {
+  const uint32_t m0 = blockIdx.x * 64;
+  const uint64_t first = s0 + m0, wi = first >> 6;
+  const uint64_t left = nshots > first ? nshots - first : 0;
+  const uint64_t valid = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
+  constexpr uint16_t kOne = 0x3F80;  // bf16 1.0
   for (uint32_t j = threadIdx.x; j < kin; j += blockDim.x) {
-    float v = 0.0f;
-    if (live && j < ndet) v = (float)((rec[(size_t)det_cols[j] * ld + wi] >> bit) & 1u);
-    x[(size_t)m * kin + j] = (bf16)v;
+    uint64_t w = 0;
+    if (j < ndet && valid) w = rec[(size_t)det_cols[j] * ld + wi] & valid;
+    for (uint32_t b = 0; b < 64; ++b) x[(size_t)(m0 + b) * kin + j] = ((w >> b) & 1u) ? kOne : 0;
+    if (xT) {
+      uint4 *dst = (uint4 *)(xT + (size_t)j * rows + m0);
+      for (uint32_t q = 0; q < 8; ++q) {
+        uint32_t v[4];
+        for (uint32_t h = 0; h < 4; ++h) {
+          const uint32_t b = q * 8 + h * 2;
+          v[h] = (((w >> b) & 1u) ? kOne : 0u) | ((((w >> (b + 1)) & 1u) ? kOne : 0u) << 16);
+        }
+        dst[q] = uint4{v[0], v[1], v[2], v[3]};
+      }
+    }
   }
-  if (with_y && threadIdx.x == 0)
-    y[m] = live ? (float)((rec[(size_t)obs_col * ld + wi] >> bit) & 1u) : 0.0f;
+  if (y && threadIdx.x < 64) {
+    const uint64_t o = valid ? rec[(size_t)obs_col * ld + wi] & valid : 0;
+    y[m0 + threadIdx.x] = (float)((o >> threadIdx.x) & 1u);
+  }
+}
+
+// dst[c][r] = src[r][c] for a rows x cols bf16 matrix (both multiples of 64).
+__global__ void transpose_bf16(const uint16_t *src, uint16_t *dst, uint32_t rows, uint32_t cols) {
+  __shared__ uint16_t t[64][66];
+  const uint32_t r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (uint32_t i = ty; i < 64; i += 4) t[i][tx] = src[(size_t)(r0 + i) * cols + c0 + tx];
+  __syncthreads();
+  for (uint32_t i = ty; i < 64; i += 4) dst[(size_t)(c0 + i) * rows + r0 + tx] = t[tx][i];
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -325,6 +350,7 @@ struct qcf_mlp {
   // activations, sized for max(batch, ibatch) rows
   bf16 *x = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr;
   bf16 *d1 = nullptr, *d2 = nullptr, *d3 = nullptr;
+  bf16 *xT = nullptr, *a1T = nullptr, *a2T = nullptr, *dT = nullptr;  // batch-minor copies
   float *pre1 = nullptr, *pre2 = nullptr, *pre3 = nullptr;
   float *y = nullptr, *z = nullptr, *dz = nullptr, *part = nullptr, *lossbuf = nullptr;
   std::vector<void *> allocs;
@@ -377,7 +403,17 @@ int colsum(qcf_mlp *M, const bf16 *D, const float *scale, uint32_t rows, uint32_
   return QCF_OK;
 }
 
-// Forward + backward of one batch already gathered into x / y; gradients in g.
+int transpose(const bf16 *src, bf16 *dst, uint32_t rows, uint32_t cols, hipStream_t s) {
+  hipLaunchKernelGGL(transpose_bf16, dim3(cols / 64, rows / 64), dim3(256), 0, s,
+                     (const uint16_t *)src, (uint16_t *)dst, rows, cols);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+// Forward + backward of one batch already gathered into x / xT / y;
+// gradients in g.  Weight gradients sum over the batch, so they contract two
+// batch-minor operands (transposed copies of the activations and of the
+// back-propagated pre-activation gradients) with K-contiguous reads.
 int fwd_bwd(qcf_mlp *M, float *loss, hipStream_t s) {
   const uint32_t B = M->batch;
   QCF_TRY(forward(M, B, true, s));
@@ -387,40 +423,43 @@ int fwd_bwd(qcf_mlp *M, float *loss, hipStream_t s) {
   hipLaunchKernelGGL(head_backward, dim3((unsigned)(((size_t)B * M->h3 + 255) / 256)), dim3(256),
                      0, s, M->dz, w4, M->pre3, M->h3, B, M->d3);
   QCF_HIP_CHECK(hipGetLastError());
+  QCF_TRY(transpose(M->a1, M->a1T, B, M->h1, s));
+  QCF_TRY(transpose(M->a2, M->a2T, B, M->h2, s));
   float *G = M->g;
   // output layer: grad w4 = a3^T dz, grad b4 = sum dz
   QCF_TRY(colsum(M, M->a3, M->dz, B, M->h3, G + M->off[6], s));
   hipLaunchKernelGGL(vec_sum, dim3(1), dim3(256), 0, s, M->dz, B, G + M->off[7]);
-  // layer 3: grad W3 = d3^T a2, grad b3 = colsum d3, d2 = (d3 W3) * gelu'(pre2)
   const bf16 *W2 = M->wb + M->off[2], *W3 = M->wb + M->off[4];
-  GemmArgs g{};
-  g = GemmArgs{M->d3, M->a2, (int)M->h3, (int)M->h2, (int)B, (int)M->h3, (int)M->h2, (int)M->h2,
-               G + M->off[4], nullptr, nullptr, nullptr};
-  QCF_TRY((launch_gemm<false, false, kEpiStoreF32>(g, s)));
-  QCF_TRY(colsum(M, M->d3, nullptr, B, M->h3, G + M->off[5], s));
-  g = GemmArgs{M->d3, W3, (int)B, (int)M->h2, (int)M->h3, (int)M->h3, (int)M->h2, (int)M->h2,
-               nullptr, M->d2, nullptr, M->pre2};
-  QCF_TRY((launch_gemm<true, false, kEpiDGelu>(g, s)));
-  // layer 2
-  g = GemmArgs{M->d2, M->a1, (int)M->h2, (int)M->h1, (int)B, (int)M->h2, (int)M->h1, (int)M->h1,
-               G + M->off[2], nullptr, nullptr, nullptr};
-  QCF_TRY((launch_gemm<false, false, kEpiStoreF32>(g, s)));
-  QCF_TRY(colsum(M, M->d2, nullptr, B, M->h2, G + M->off[3], s));
-  g = GemmArgs{M->d2, W2, (int)B, (int)M->h1, (int)M->h2, (int)M->h2, (int)M->h1, (int)M->h1,
-               nullptr, M->d1, nullptr, M->pre1};
-  QCF_TRY((launch_gemm<true, false, kEpiDGelu>(g, s)));
-  // layer 1 (no input gradient)
-  g = GemmArgs{M->d1, M->x, (int)M->h1, (int)M->kin, (int)B, (int)M->h1, (int)M->kin,
-               (int)M->kin, G + M->off[0], nullptr, nullptr, nullptr};
-  QCF_TRY((launch_gemm<false, false, kEpiStoreF32>(g, s)));
-  QCF_TRY(colsum(M, M->d1, nullptr, B, M->h1, G + M->off[1], s));
+  // one layer back: grad W = dT aT^T, grad b = colsum d, and (unless first)
+  // d_prev = (d W) * gelu'(pre_prev)
+  auto back = [&](const bf16 *d, uint32_t hout, const bf16 *aT, uint32_t hin, float *gW,
+                  float *gb, const bf16 *W, const float *pre_prev, bf16 *d_prev) -> int {
+    QCF_TRY(transpose(d, M->dT, B, hout, s));
+    GemmArgs g{M->dT, aT, (int)hout, (int)hin, (int)B, (int)B, (int)B, (int)hin,
+               gW, nullptr, nullptr, nullptr};
+    QCF_TRY((launch_gemm<true, true, kEpiStoreF32>(g, s)));
+    QCF_TRY(colsum(M, d, nullptr, B, hout, gb, s));
+    if (d_prev) {
+      g = GemmArgs{d, W, (int)B, (int)hin, (int)hout, (int)hout, (int)hin, (int)hin,
+                   nullptr, d_prev, nullptr, pre_prev};
+      QCF_TRY((launch_gemm<true, false, kEpiDGelu>(g, s)));
+    }
+    return QCF_OK;
+  };
+  QCF_TRY(back(M->d3, M->h3, M->a2T, M->h2, G + M->off[4], G + M->off[5], W3, M->pre2, M->d2));
+  QCF_TRY(back(M->d2, M->h2, M->a1T, M->h1, G + M->off[2], G + M->off[3], W2, M->pre1, M->d1));
+  QCF_TRY(back(M->d1, M->h1, M->xT, M->kin, G + M->off[0], G + M->off[1], nullptr, nullptr,
+               nullptr));
   return QCF_OK;
 }
 
 int gather(qcf_mlp *M, const uint64_t *rec, size_t ld, uint64_t s0, uint64_t nshots,
-           uint32_t rows, bool with_y, hipStream_t s) {
-  hipLaunchKernelGGL(gather_inputs, dim3(rows), dim3(128), 0, s, rec, ld, s0, nshots,
-                     M->det_cols, M->ndet, M->kin, M->obs_col, with_y ? 1 : 0, M->x, M->y, rows);
+           uint32_t rows, bool train, hipStream_t s) {
+  if (s0 % 64) return set_error(QCF_ERR_ARGUMENT, "first shot %llu is not a multiple of 64",
+                                (unsigned long long)s0);
+  hipLaunchKernelGGL(gather_inputs, dim3(rows / 64), dim3(256), 0, s, rec, ld, s0, nshots,
+                     M->det_cols, M->ndet, M->kin, M->obs_col, train ? M->y : nullptr,
+                     (uint16_t *)M->x, train ? (uint16_t *)M->xT : nullptr, rows);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
 }
@@ -477,6 +516,10 @@ extern "C" int qcf_mlp_create(const uint32_t *det_bits, uint32_t ndet, uint32_t 
   A(&M->d1, (size_t)batch * h1);
   A(&M->d2, (size_t)batch * h2);
   A(&M->d3, (size_t)batch * h3);
+  A(&M->xT, (size_t)batch * M->kin);
+  A(&M->a1T, (size_t)batch * h1);
+  A(&M->a2T, (size_t)batch * h2);
+  A(&M->dT, (size_t)batch * std::max(std::max(h1, h2), h3));
   A(&M->pre1, (size_t)batch * h1);
   A(&M->pre2, (size_t)batch * h2);
   A(&M->pre3, (size_t)batch * h3);
@@ -537,13 +580,14 @@ extern "C" int qcf_mlp_get_params(const qcf_mlp *m, float *host, void *stream) {
   return QCF_OK;
 }
 
-extern "C" int qcf_mlp_gradients(qcf_mlp *m, const uint64_t *rec, size_t ld, float *grad_host,
-                                 float *loss_host, void *stream) {
+extern "C" int qcf_mlp_gradients(qcf_mlp *m, const uint64_t *rec, size_t ld, uint64_t shot0,
+                                 float *grad_host, float *loss_host, void *stream) {
   clear_error();
   if (!m || !rec || !grad_host || !loss_host) return set_error(QCF_ERR_ARGUMENT, "null argument");
-  if (ld * 64 < m->batch) return set_error(QCF_ERR_DIMENSION, "record holds fewer shots than a batch");
+  if (ld * 64 < shot0 + m->batch)
+    return set_error(QCF_ERR_DIMENSION, "record holds fewer than shot0 + batch shots");
   hipStream_t s = (hipStream_t)stream;
-  QCF_TRY(gather(m, rec, ld, 0, m->batch, m->batch, true, s));
+  QCF_TRY(gather(m, rec, ld, shot0, shot0 + m->batch, m->batch, true, s));
   float *loss = m->lossbuf;
   QCF_HIP_CHECK(hipMemsetAsync(loss, 0, 4, s));
   QCF_TRY(fwd_bwd(m, loss, s));
@@ -553,14 +597,15 @@ extern "C" int qcf_mlp_gradients(qcf_mlp *m, const uint64_t *rec, size_t ld, flo
   return QCF_OK;
 }
 
-extern "C" int qcf_mlp_train_step(qcf_mlp *m, const uint64_t *rec, size_t ld, float lr,
-                                  float beta1, float beta2, float eps, float *loss_accum,
-                                  void *stream) {
+extern "C" int qcf_mlp_train_step(qcf_mlp *m, const uint64_t *rec, size_t ld, uint64_t shot0,
+                                  float lr, float beta1, float beta2, float eps,
+                                  float *loss_accum, void *stream) {
   clear_error();
   if (!m || !rec) return set_error(QCF_ERR_ARGUMENT, "null argument");
-  if (ld * 64 < m->batch) return set_error(QCF_ERR_DIMENSION, "record holds fewer shots than a batch");
+  if (ld * 64 < shot0 + m->batch)
+    return set_error(QCF_ERR_DIMENSION, "record holds fewer than shot0 + batch shots");
   hipStream_t s = (hipStream_t)stream;
-  QCF_TRY(gather(m, rec, ld, 0, m->batch, m->batch, true, s));
+  QCF_TRY(gather(m, rec, ld, shot0, shot0 + m->batch, m->batch, true, s));
   QCF_TRY(fwd_bwd(m, loss_accum, s));
   m->step += 1;
   const float c1 = 1.0f / (1.0f - std::pow(beta1, (float)m->step));

@@ -108,9 +108,9 @@ _SIGNATURES = {
     "qcf_mlp_nparams": (_I, [_P, C.POINTER(_SZ)]),
     "qcf_mlp_set_params": (_I, [_P, _P, _P]),
     "qcf_mlp_get_params": (_I, [_P, _P, _P]),
-    "qcf_mlp_train_step": (_I, [_P, _P, _SZ, C.c_float, C.c_float, C.c_float, C.c_float, _P,
-                                _P]),
-    "qcf_mlp_gradients": (_I, [_P, _P, _SZ, _P, _P, _P]),
+    "qcf_mlp_train_step": (_I, [_P, _P, _SZ, _U64, C.c_float, C.c_float, C.c_float,
+                                C.c_float, _P, _P]),
+    "qcf_mlp_gradients": (_I, [_P, _P, _SZ, _U64, _P, _P, _P]),
     "qcf_mlp_infer": (_I, [_P, _P, _SZ, _U64, _P, _P]),
     "qcf_debug_gemm_bf16": (_I, [_I, _I, _I, _I, _P, _P, _P, _P]),
 }

@@ -90,18 +90,18 @@ class MLPDecoder:
         _lib.call("qcf_mlp_get_params", self.handle, out.ctypes.data_as(C.c_void_p), _stream())
         return out
 
-    def gradients(self, record: torch.Tensor, ld: int):
-        """(loss, flat gradient) of the first `batch` shots of a packed record."""
+    def gradients(self, record: torch.Tensor, ld: int, shot0: int = 0):
+        """(loss, flat gradient) of shots shot0 .. shot0+batch-1 of a packed record."""
         g = np.empty(self.nparams, dtype=np.float32)
         loss = np.empty(1, dtype=np.float32)
-        _lib.call("qcf_mlp_gradients", self.handle, _ptr(record), ld,
+        _lib.call("qcf_mlp_gradients", self.handle, _ptr(record), ld, shot0,
                   g.ctypes.data_as(C.c_void_p), loss.ctypes.data_as(C.c_void_p), _stream())
         return float(loss[0]), g
 
     def train_step(self, record: torch.Tensor, ld: int, lr: float = 3e-4, beta1: float = 0.9,
-                   beta2: float = 0.999, eps: float = 1e-8) -> None:
-        _lib.call("qcf_mlp_train_step", self.handle, _ptr(record), ld, lr, beta1, beta2, eps,
-                  _ptr(self._loss), _stream())
+                   beta2: float = 0.999, eps: float = 1e-8, shot0: int = 0) -> None:
+        _lib.call("qcf_mlp_train_step", self.handle, _ptr(record), ld, shot0, lr, beta1, beta2,
+                  eps, _ptr(self._loss), _stream())
 
     def logits(self, record: torch.Tensor, ld: int, shots: int) -> torch.Tensor:
         out = torch.empty(max(1, shots), dtype=torch.float32, device=record.device)
@@ -116,25 +116,30 @@ class MLPDecoder:
 
 
 def train(decoder: MLPDecoder, make_circuit, steps: int, p_range=(5e-4, 5e-3), lr: float = 3e-4,
-          seed: int = 0, n_rates: int = 16, log_every: int = 0, log=print):
-    """Train on `steps` batches.  Each batch samples `decoder.batch` shots on
-    the device from one of `n_rates` programs whose noise rates are log-spaced
-    over p_range, picked uniformly per batch (programs that differ only in
-    their rate share one compiled sampling kernel)."""
+          seed: int = 0, n_rates: int = 16, log_every: int = 0, log=print,
+          batches_per_sample: int = 16):
+    """Train on `steps` batches.  The frame sampler draws `batches_per_sample`
+    batches per launch on the device from one of `n_rates` programs whose
+    noise rates are log-spaced over p_range, picked uniformly per launch
+    (programs that differ only in their rate share one compiled sampling
+    kernel); every batch is used once."""
     rng = np.random.default_rng(seed)
     rates = np.exp(np.linspace(np.log(p_range[0]), np.log(p_range[1]), n_rates))
     progs = [Program(make_circuit(float(p)).ops) for p in rates]
     B = decoder.batch
-    ld = words(B)
+    per = max(1, int(batches_per_sample))
+    ld = words(B * per)
     rec = torch.empty(max(1, progs[0].nbits) * ld, dtype=torch.int64, device="cuda")
     st = _stream()
     hist = []
     t0 = time.perf_counter()
     for step in range(steps):
-        prog = progs[int(rng.integers(n_rates))]
-        _lib.call("qcf_pf_sample", prog.handle, B, int(rng.integers(0, 2**62)), None, _ptr(rec),
-                  None, ld, 1, st)
-        decoder.train_step(rec, ld, lr=lr)
+        k = step % per
+        if k == 0:
+            prog = progs[int(rng.integers(n_rates))]
+            _lib.call("qcf_pf_sample", prog.handle, B * per, int(rng.integers(0, 2**62)), None,
+                      _ptr(rec), None, ld, 1, st)
+        decoder.train_step(rec, ld, lr=lr, shot0=k * B)
         if log_every and (step + 1) % log_every == 0:
             loss = decoder.take_loss() / log_every
             hist.append((step + 1, loss))

@@ -79,9 +79,11 @@ def test_gradients_match_oracle(gpu, d, rounds, hidden, batch):
     c = circuits.rotated_surface_code(d, rounds, 4e-3)
     dec = mlp.MLPDecoder(c.det_bits, c.obs_bits[0], hidden=hidden, batch=batch,
                          infer_batch=batch, seed=3)
-    rec, ld = sample_record(c, batch, seed=17)
-    loss, g = dec.gradients(rec, ld)
-    X, y = host_inputs(rec, ld, c.det_bits, c.obs_bits[0], batch, dec.kin)
+    # the second batch of a two-batch record (shot offset)
+    rec, ld = sample_record(c, 2 * batch, seed=17)
+    loss, g = dec.gradients(rec, ld, shot0=batch)
+    X, y = host_inputs(rec, ld, c.det_bits, c.obs_bits[0], 2 * batch, dec.kin)
+    X, y = X[batch:], y[batch:]
     params = dec.params()
     oloss, og = om.loss_and_grads(params, X, y, dec.dims, emulate=True)
     assert loss == pytest.approx(oloss, rel=2e-4)
