@@ -127,7 +127,7 @@ def bench_batched(ncirc=512, seed=0, dmax=15):
     ms_s = time_launches(seq, 2, 1)
     return dict(ncirc=ncirc, total_shots=total, ms_batched=ms_b, ms_sequential=ms_s,
                 shots_s_batched=total / (ms_b * 1e-3), shots_s_sequential=total / (ms_s * 1e-3),
-                ratio=ms_s / ms_b, tune=[tune.waves_per_cu, tune.tile_blocks], dmax=dmax)
+                ratio=ms_s / ms_b, tune=[tune.waves_per_cu, tune.tile_blocks, tune.lds_cap], dmax=dmax)
 
 
 def cpu_baseline(target_s=12.0):

@@ -227,6 +227,9 @@ int qcf_pf_sample(const qcf_program *prog, uint64_t shots, uint64_t seed,
 typedef struct qcf_tune {
   uint32_t waves_per_cu; /* resident single-wave workgroups per CU (grid size) */
   uint32_t tile_blocks;  /* 2048-shot tiles per work item of the device queue  */
+  uint32_t lds_cap;      /* bytes of LDS a circuit's frame may take (0 = all the
+                            device allows); larger frames live in global memory,
+                            which keeps more waves resident                     */
 } qcf_tune;
 
 /*

@@ -818,6 +818,7 @@ extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
   // circuits whose frame does not fit in LDS keep their planes in a
   // batch-owned global buffer: one slot of the largest such frame per wave
   const bool force_gl = force_global_frames();
+  if (tune && tune->lds_cap) lds_max = std::min<int>(lds_max, (int)tune->lds_cap);
   uint32_t gl_planes = 0;
   for (size_t k = 0; k < n; ++k) {
     const uint32_t lb = progs[k]->nplanes * 256u;
@@ -916,7 +917,7 @@ extern "C" int qcf_autotune(const qcf_program *const *progs, size_t n,
                             const uint64_t *shots, qcf_tune *out, void *stream) {
   clear_error();
   if (!out) return set_error(QCF_ERR_ARGUMENT, "null output pointer");
-  *out = qcf_tune{0, 0};
+  *out = qcf_tune{0, 0, 0};
   if (n == 0) return QCF_OK;
   hipStream_t s = (hipStream_t)stream;
   // scratch records so the caller's buffers are untouched
@@ -929,20 +930,24 @@ extern "C" int qcf_autotune(const qcf_program *const *progs, size_t n,
     const size_t bytes = std::max<size_t>(1, progs[k]->nbits) * lds[k] * 8;
     if (hipMalloc(&bufs[k], bytes) != hipSuccess) st = set_error(QCF_ERR_ALLOC, "autotune scratch");
   }
-  qcf_batch *B = nullptr;
-  if (st == QCF_OK) st = qcf_batch_create(progs, n, shots, seeds.data(), bufs.data(), lds.data(), nullptr, &B);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (st == QCF_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     st = set_error(QCF_ERR_HIP, "autotune events");
   float best = 1e30f;
-  qcf_tune best_t{0, 0};
-  const uint32_t lds_cap = B ? std::max<uint32_t>(1, (160u * 1024u) / std::max(1u, B->lds_bytes)) : 1;
-  const uint32_t wpcs[] = {1, 2, 4, 8, 16};
-  const uint32_t tbs[] = {1, 2, 4, 8};
-  for (uint32_t wpc : wpcs) {
-    if (st != QCF_OK || wpc > lds_cap) break;
+  qcf_tune best_t{0, 0, 0};
+  // frames above the LDS cap go to global memory: fewer bytes of LDS per
+  // wave, more resident waves; the grid takes as many waves as LDS allows
+  const uint32_t caps[] = {0u, 72u * 1024u, 36u * 1024u, 14u * 1024u};
+  const uint32_t tbs[] = {1, 4};
+  for (uint32_t cap : caps) {
+    if (st != QCF_OK) break;
+    qcf_batch *B = nullptr;
+    qcf_tune t0{0, 1, cap};
+    if ((st = qcf_batch_create(progs, n, shots, seeds.data(), bufs.data(), lds.data(), &t0, &B)) !=
+        QCF_OK)
+      break;
     for (uint32_t tb : tbs) {
-      qcf_tune t{wpc, tb};
+      qcf_tune t{0, tb, cap};
       if ((st = qcf::set_tune(B, &t)) != QCF_OK) break;
       if ((st = qcf::batch_launch(B, s)) != QCF_OK) break;  // warm
       if (hipEventRecord(e0, s) != hipSuccess) { st = set_error(QCF_ERR_HIP, "event"); break; }
@@ -958,10 +963,10 @@ extern "C" int qcf_autotune(const qcf_program *const *progs, size_t n,
         best_t = t;
       }
     }
+    qcf_batch_free(B);
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  if (B) qcf_batch_free(B);
   for (auto *p : bufs)
     if (p) (void)hipFree(p);
   if (st == QCF_OK) *out = best_t;

@@ -56,7 +56,8 @@ class ProgramInfo(C.Structure):
 
 
 class Tune(C.Structure):
-    _fields_ = [("waves_per_cu", C.c_uint32), ("tile_blocks", C.c_uint32)]
+    _fields_ = [("waves_per_cu", C.c_uint32), ("tile_blocks", C.c_uint32),
+                ("lds_cap", C.c_uint32)]
 
 
 _P = C.c_void_p

@@ -7,6 +7,7 @@ from typing import Sequence
 import numpy as np
 import torch
 
+from . import _lib as _LIB
 from ._lib import Tune, call
 from .frames import Program, words
 
@@ -52,11 +53,11 @@ class Batch:
         return self.records[k].cpu().numpy().view(np.uint64).reshape(-1, max(1, words(s)))[: p.nbits]
 
     def __del__(self):
-        from . import _lib
         h = getattr(self, "handle", None)
-        if h and h.value and _lib._lib is not None:
-            _lib._lib.qcf_batch_free(h)
-            self.handle = None
+        lib_mod = _LIB
+        if h and h.value and lib_mod is not None and lib_mod._lib is not None:
+            lib_mod._lib.qcf_batch_free(h)
+        self.handle = None
 
 
 def autotune(progs: Sequence[Program], shots: Sequence[int]) -> Tune:
