@@ -459,6 +459,7 @@ int launch_pf(const KParams &kp, uint32_t lds_bytes, uint32_t grid, hipStream_t 
 // Frames too large for LDS live in a per-wave region of this device buffer
 // (grown on demand; only single-circuit launches use it, batches own theirs).
 constexpr uint32_t kGlobalWavesPerCU = 8;
+constexpr uint32_t kGlobalJitWavesPerCU = 16;  // specialised kernels, planes in global memory
 struct FrameScratch {
   void *ptr = nullptr;
   size_t bytes = 0;
@@ -548,9 +549,12 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
   // Native launches of on-chip frames run the kernel specialised to this
   // program (the partial last tile included); the interpreter is the general
   // path.  Both draw the same events for the same shots.
-  if (!global && !masks && !exp && !jit_disabled()) {
+  // Frames in global memory stay on the interpreter unless QCF_JIT_GLOBAL=1:
+  // both are bound by L2 latency there and the specialised code measured no
+  // faster for d >= 9 while costing seconds of compilation.
+  if (!masks && !exp && !jit_disabled() && (!global || jit_global_enabled())) {
     JitKernelInfo jk;
-    int js = jit_get(P, variant, &jk);
+    int js = jit_get(P, variant | (global ? 4 : 0), &jk);
     if (js == QCF_OK) {
       JitParams jp{};
       jp.rec = kp.rec;
@@ -564,14 +568,19 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
       jp.c = kp.cls_c[0];
       jp.has_class = kp.nclasses > 0 ? 1u : 0u;
       jp.nfull = (uint32_t)(shots / kBlockShots);
-      const uint32_t jwpc = waves_per_cu ? waves_per_cu : jk.blocks_per_cu;
+      uint32_t jwpc = waves_per_cu ? waves_per_cu : jk.blocks_per_cu;
+      if (global) jwpc = std::min<uint32_t>(jwpc, kGlobalJitWavesPerCU);
       uint64_t g1 = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * jwpc);
       const uint32_t nx = jit_xcds();
       if (nx > 1 && g1 >= nx) {
         g1 -= g1 % nx;
         jp.xcds = nx;
       }
-      js = jit_launch(jk.function, jp, (uint32_t)g1, jk.lds_bytes, stream);
+      if (global) {
+        js = frame_scratch(g1 * (size_t)P->nplanes * 256u, &jp.scratch);
+        jp.sstride = (uint64_t)P->nplanes * 64u;
+      }
+      if (js == QCF_OK) js = jit_launch(jk.function, jp, (uint32_t)g1, jk.lds_bytes, stream);
       if (js == QCF_OK) return QCF_OK;
     }
     const char *req = getenv("QCF_REQUIRE_JIT");

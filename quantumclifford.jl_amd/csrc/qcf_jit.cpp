@@ -42,12 +42,32 @@ struct KJ {
   uint32_t nblocks, nw32, nwbuf, lastmask;
   float c;
   uint32_t has_class, nfull, xcds;
+  uint32_t *scratch;  // per-wave planes (global mode)
+  uint64_t sstride;   // words per wave
 };
 struct FlipLDS {
   uint32_t *L;
   __device__ __forceinline__ void operator()(uint32_t p, uint32_t tl, uint32_t bit, uint32_t,
                                              uint32_t) const {
     atomicXor(&L[p * 64u + tl], bit);
+  }
+};
+// planes in global memory: every access at L2 (agent scope) so that the
+// event flips other lanes make are seen by the next read
+__device__ __forceinline__ uint32_t gld(const uint32_t *a) {
+  return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gst(uint32_t *a, uint32_t v) {
+  __hip_atomic_store(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gxr(uint32_t *a, uint32_t v) {
+  __hip_atomic_fetch_xor(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+struct FlipG {
+  uint32_t *F;
+  __device__ __forceinline__ void operator()(uint32_t p, uint32_t tl, uint32_t bit, uint32_t,
+                                             uint32_t) const {
+    gxr(F + p * 64u + tl, bit);
   }
 };
 )QCFJIT";
@@ -187,6 +207,16 @@ Loop find_loop(const std::vector<Instr> &code) {
   return best;
 }
 
+// Instructions a specialised kernel spells out: the stream with its repeated
+// stretch written once.
+size_t emitted_size(const qcf_program *P, int variant) {
+  std::vector<Instr> code;
+  for (const Instr &in : variant_code(P, variant))
+    if ((in.op & 0xFFu) != I_END) code.push_back(in);
+  const Loop lp = jit_no_loops() ? Loop{} : find_loop(code);
+  return lp.R >= 2 ? code.size() - (lp.R - 1) * lp.L : code.size();
+}
+
 // Write the kernel source of one variant; returns false for programs the
 // generator does not handle (the interpreter runs those).
 //
@@ -218,7 +248,10 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   } else {
     ranges = {{0, code.size(), false}};
   }
-  const bool regs = P->nplanes <= jit_reg_planes();
+  // planes in global memory when the launcher found the frame too large for
+  // LDS (variant bit 2), else registers for small programs, else LDS
+  const bool glob = (variant & 4) != 0;
+  const bool regs = !glob && P->nplanes <= jit_reg_planes();
   const bool nt = jit_nt_stores();
   constexpr uint32_t kSlots = 64;  // noise slots per event pass in register mode
   constexpr size_t kBarrierOps = 30;  // operands per compiler barrier
@@ -265,11 +298,21 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   }
   range_groups.push_back(groups.size());
   const uint32_t nsite = (uint32_t)sites.size();
-  auto PL = [&](uint32_t p) {
-    return regs ? "r" + std::to_string(p) : plane(p);
+  auto gaddr = [](uint32_t p) { return "F + (" + std::to_string(p * 64u) + "u + l)"; };
+  // plane access: read expression, write and XOR statements
+  auto PL = [&](uint32_t p) -> std::string {
+    if (regs) return "r" + std::to_string(p);
+    if (glob) return "gld(" + gaddr(p) + ")";
+    return plane(p);
   };
-  auto XOR = [&](uint32_t p, const std::string &v) {
-    return regs ? PL(p) + " ^= " + v + ";" : "atomicXor(&" + plane(p) + ", " + v + ");";
+  auto WR = [&](uint32_t p, const std::string &v) -> std::string {
+    if (glob) return "gst(" + gaddr(p) + ", " + v + ");";
+    return PL(p) + " = " + v + ";";
+  };
+  auto XOR = [&](uint32_t p, const std::string &v) -> std::string {
+    if (regs) return PL(p) + " ^= " + v + ";";
+    if (glob) return "gxr(" + gaddr(p) + ", " + v + ");";
+    return "atomicXor(&" + plane(p) + ", " + v + ");";
   };
   auto slot = [](uint32_t k) { return "L[" + std::to_string(k * 64u) + "u + l]"; };
   auto load_group = [&](size_t g, const char *pfx) {
@@ -378,23 +421,23 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
           body << "    { const uint32_t v = " << PL(a) << ";";
           if (col != kNone) body << " " << st_rec(col, "v");
           if (op == I_MRX) body << " " << st_rec(in.g, "v ^ " + PL(in.f));
-          if (keep != kNone) body << " " << PL(keep) << " = v;";
-          body << " " << PL(a) << " = 0u; " << XOR(b, "g.next()") << " }\n";
+          if (keep != kNone) body << " " << WR(keep, "v");
+          body << " " << WR(a, "0u") << " " << XOR(b, "g.next()") << " }\n";
           break;
         }
         case I_XSTORE2:
           body << "    { const uint32_t v = " << PL(a) << " ^ " << PL(b) << "; "
                << st_rec(c, "v");
-          if (in.d != kNone) body << " " << PL(in.d) << " = v;";
+          if (in.d != kNone) body << " " << WR(in.d, "v");
           body << " }\n";
           break;
         case I_MEAS:
           body << "    { const uint32_t v = " << PL(a) << "; " << st_rec(b, "v");
-          if (c != kNone) body << " " << PL(c) << " = v;";
+          if (c != kNone) body << " " << WR(c, "v");
           body << " }\n";
           break;
         case I_RESETZ:
-          body << "    " << PL(a) << " = 0u; " << XOR(b, "g.next()") << "\n";
+          body << "    " << WR(a, "0u") << " " << XOR(b, "g.next()") << "\n";
           break;
         case I_ACC_LDS:
           body << "    acc = " << PL(a) << ";\n";
@@ -410,32 +453,34 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
           break;
         case I_ACC_STORE:
           body << "    " << st_rec(a, "acc");
-          if (b != kNone) body << " " << PL(b) << " = acc;";
+          if (b != kNone) body << " " << WR(b, "acc");
           body << "\n";
           break;
         case I_REC_ZERO:
           body << "    " << st_rec(a, "0u") << "\n";
           break;
         case I_INIT:
-          body << "    " << PL(a) << " = 0u; " << PL(b) << " = g.next();\n";
+          body << "    " << WR(a, "0u") << " " << WR(b, "g.next()") << "\n";
           break;
         case I_LOAD:
-          body << "    " << PL(a) << " = "
-               << (tail ? "(live ? " + at("P.frame", b, false) + " : 0u)" : at("P.frame", b, false))
-               << ";\n";
+          body << "    "
+               << WR(a, tail ? "(live ? " + at("P.frame", b, false) + " : 0u)"
+                             : at("P.frame", b, false))
+               << "\n";
           break;
         case I_STORE:
           body << "    " << (tail ? "if (live) " : "") << at("P.frame", b, false) << " = " << PL(a)
                << (tail ? " & lm;\n" : ";\n");
           break;
         case I_ZERO:
-          body << "    " << PL(a) << " = 0u;\n";
+          body << "    " << WR(a, "0u") << "\n";
           break;
         case I_LIN1:
-          body << "    { const uint32_t x = " << PL(a) << ", z = " << PL(b) << "; " << PL(a)
-               << " = " << ((c & 1u) ? "x" : "0u") << " ^ " << ((c & 2u) ? "z" : "0u") << "; "
-               << PL(b) << " = " << ((c & 4u) ? "x" : "0u") << " ^ " << ((c & 8u) ? "z" : "0u")
-               << "; }\n";
+          body << "    { const uint32_t x = " << PL(a) << ", z = " << PL(b) << "; "
+               << WR(a, std::string((c & 1u) ? "x" : "0u") + " ^ " + ((c & 2u) ? "z" : "0u"))
+               << " "
+               << WR(b, std::string((c & 4u) ? "x" : "0u") + " ^ " + ((c & 8u) ? "z" : "0u"))
+               << " }\n";
           break;
         default:
           return false;
@@ -457,7 +502,9 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   src << "extern \"C\" __global__ void __launch_bounds__(64) qcf_jit_kernel(KJ P) {\n"
       << "  extern __shared__ uint32_t L[];\n"
       << "  const uint32_t l = threadIdx.x;\n"
-      << "  const FlipLDS flip{L};\n";
+      << (glob ? "  uint32_t *const F = P.scratch + (uint64_t)blockIdx.x * P.sstride;\n"
+                 "  (void)L;\n  const FlipG flip{F};\n"
+               : "  const FlipLDS flip{L};\n");
   if (regs)  // the slot buffer starts at zero and every pass leaves it at zero
     src << "  for (uint32_t k = 0; k < " << std::max<uint32_t>(max_slots, 1u)
         << "u; ++k) L[k * 64u + l] = 0u;\n";
@@ -492,7 +539,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   src << "  }\n}\n";
   *out = src.str();
   if (lds_bytes)
-    *lds_bytes = regs ? std::max<uint32_t>(max_slots, 1u) * 256u : P->nplanes * 256u;
+    *lds_bytes = glob ? 0u : regs ? std::max<uint32_t>(max_slots, 1u) * 256u : P->nplanes * 256u;
   return true;
 }
 
@@ -540,6 +587,12 @@ size_t jit_max_instr() {
   return e ? (size_t)strtoull(e, nullptr, 10) : (size_t)12000;
 }
 
+// QCF_JIT_GLOBAL=1 also specialises programs whose frames live in global memory.
+bool jit_global_enabled() {
+  const char *e = getenv("QCF_JIT_GLOBAL");
+  return e && e[0] == '1';
+}
+
 // QCF_JIT_NT=1 writes the record with nontemporal stores.
 bool jit_nt_stores() {
   const char *e = getenv("QCF_JIT_NT");
@@ -582,7 +635,7 @@ int jit_get(qcf_program *P, int variant, JitKernelInfo *info) {
   std::string src, log;
   std::vector<char> bin;
   int st = QCF_OK;
-  if (variant_code(P, variant).size() > jit_max_instr())
+  if (emitted_size(P, variant) > jit_max_instr())
     st = set_error(QCF_ERR_UNSUPPORTED, "program longer than QCF_JIT_MAX_INSTR instructions");
   else if (!generate(P, variant, &src, &jk.lds_bytes))
     st = set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
@@ -648,7 +701,7 @@ extern "C" int qcf_debug_jit_source(const qcf_program *prog, int variant, char *
   qcf::clear_error();
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src;
-  if (!qcf::generate(prog, variant & 3, &src, nullptr))
+  if (!qcf::generate(prog, variant & 7, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   if (needed) *needed = src.size() + 1;
   if (buf && cap) {
@@ -664,7 +717,7 @@ extern "C" int qcf_debug_jit_compile(const qcf_program *prog, int variant, size_
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src, log;
   std::vector<char> bin;
-  if (!qcf::generate(prog, variant & 3, &src, nullptr))
+  if (!qcf::generate(prog, variant & 7, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   QCF_TRY(qcf::compile(src, &bin, &log));
   if (code_bytes) *code_bytes = bin.size();

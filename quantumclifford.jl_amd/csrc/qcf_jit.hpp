@@ -18,6 +18,8 @@ struct JitParams {
   uint32_t nblocks, nw32, nwbuf, lastmask;
   float c;
   uint32_t has_class, nfull, xcds;
+  uint32_t *scratch;  // per-wave planes when the frame does not fit in LDS
+  uint64_t sstride;   // words per wave
 };
 
 // QCF_NO_JIT=1 keeps every launch on the interpreter.
@@ -27,6 +29,7 @@ bool jit_disabled();
 // longest program specialised (QCF_JIT_MAX_INSTR, default 12000).
 uint32_t jit_reg_planes();
 bool jit_no_loops();
+bool jit_global_enabled();
 bool jit_nt_stores();
 uint32_t jit_xcds();
 size_t jit_max_instr();

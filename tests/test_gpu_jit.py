@@ -92,3 +92,23 @@ def test_in_place_specialised_equals_interpreter(gpu, monkeypatch):
     (ra, fa), (rb, fb2) = _both(monkeypatch, run)
     assert np.array_equal(ra, rb)
     assert np.array_equal(fa, fb2)
+
+
+@pytest.mark.parametrize("d,rounds,force", [(7, 3, True), (11, 2, False)])
+def test_global_frames_specialised_equals_interpreter(gpu, monkeypatch, d, rounds, force):
+    """Frames too large for LDS (or forced out of it) keep their planes in
+    global memory in the specialised kernel too (QCF_JIT_GLOBAL=1 opts in)."""
+    monkeypatch.setenv("QCF_JIT_GLOBAL", "1")
+    if force:
+        monkeypatch.setenv("QCF_FORCE_GLOBAL_FRAMES", "1")
+    prog = qcf.Program(circuits.rotated_surface_code(d, rounds, 5e-3).ops)
+    if not force:
+        assert prog.info.lds_bytes > 160 * 1024
+
+    def run():
+        f = qcf.pftrajectories(prog, 3 * 2048 + 5, seed=21)
+        return f.record_host(), f.frame_host()
+
+    (ra, fa), (rb, fb) = _both(monkeypatch, run)
+    assert np.array_equal(ra, rb)
+    assert np.array_equal(fa, fb)
