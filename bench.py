@@ -25,6 +25,11 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per shot of the d=5 x 25 launch from rocprofv3 --pmc (profiles/
+# pf_jit_kernel_d5x25_traffic.md): WRITE_SIZE 1.544e6 KiB and FETCH_SIZE
+# 696 KiB per 10^7-shot dispatch, scaled by what the counters read for a
+# 1 GiB copy (WRITE_SIZE 1.0x, FETCH_SIZE 0.5x of the bytes moved).
+TRAFFIC_BYTES_PER_SHOT = (1.544e6 * 1.0 + 696.4 * 2.0) * 1024 / 1e7
 INT_VALU_PEAK = 256 * 64 * 2.4e9  # int32 lane-ops/s (256 CUs x 64 lanes x 2.4 GHz)
 METRIC = "batched vs per-circuit shots/s, d=5 surface 25 rds; NN vs MWPM LER"
 
@@ -141,16 +146,48 @@ def cpu_baseline(target_s=12.0):
     ops, args = qcf.lower_circuit(c.ops, c.nqubits)
     nq, nbits = c.nqubits + 1, c.nbits
     cores = min(16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
     oracle.pf_run(ops, args, nq, nbits, 4000, None, True, seed=1, nthreads=cores)
-    est = (time.perf_counter() - t0) / 4000
-    shots = int(max(10000, min(2_000_000, target_s / max(est, 1e-9))))
-    t0 = time.perf_counter()
-    oracle.pf_run(ops, args, nq, nbits, shots, None, True, seed=2, nthreads=cores)
-    dt = time.perf_counter() - t0
+    # chunks of 10^6 shots (the oracle keeps one byte per record bit) until
+    # about target_s seconds of CPU work
+    chunk, shots, dt, k = 1_000_000, 0, 0.0, 0
+    while dt < target_s and k < 30:
+        t0 = time.perf_counter()
+        oracle.pf_run(ops, args, nq, nbits, chunk, None, True, seed=2 + k, nthreads=cores)
+        dt += time.perf_counter() - t0
+        shots += chunk
+        k += 1
     return dict(value=shots / dt, unit="shots/s", cores=cores, kind="port",
-                sample=f"{shots} shots of the d=5 x 25-round circuit (p=1e-3), oracle "
-                       f"restatement with OpenMP over {cores} threads, {dt:.1f} s")
+                sample=f"{shots} shots of the d=5 x 25-round circuit (p=1e-3) in chunks of "
+                       f"{chunk}, oracle restatement with OpenMP over {cores} threads, "
+                       f"{dt:.1f} s")
+
+
+def bench_mlp(steps: int, eval_shots: int = 200_000):
+    """Perceptron decoder (600 -> 1024 -> 1024 -> 512 -> 1, BF16 on the matrix
+    cores) trained for `steps` batches of 8192 device-sampled d=5 x 25 shots
+    at p log-spaced in [5e-4, 5e-3]; then logical error rates against the
+    matching decoder at p = 1e-3 with the fallback |logit| < tau -> matching."""
+    from qcf import circuits, decoders, mlp
+
+    def make(p):
+        return circuits.rotated_surface_code(5, 25, p)
+
+    c = make(1e-3)
+    dec = mlp.MLPDecoder(c.det_bits, c.obs_bits[0], batch=8192, infer_batch=65536, seed=0)
+    tr = mlp.train(dec, make, steps=steps, p_range=(5e-4, 5e-3), lr=3e-4, seed=1)
+    # matrix-core work per batch: forward and weight gradients over all three
+    # hidden layers, input gradients of layers 2 and 3 (the 512 -> 1 head is
+    # not on the matrix cores)
+    layers = ((dec.kin, 1024), (1024, 1024), (1024, 512))
+    flops_step = 2 * 8192 * (2 * sum(a * b for a, b in layers) +
+                             sum(a * b for a, b in layers[1:]))
+    matcher = decoders.MatchingDecoder(c.ops, c.det_bits, c.obs_bits)
+    evals = [mlp.evaluate(dec, matcher, c.ops, shots=eval_shots, tau=tau, seed=7,
+                          chunk=eval_shots) for tau in (0.0, 1.0, 2.0, 4.0)]
+    return dict(train_steps=steps, train_s=tr["seconds"], ms_per_step=1e3 * tr["seconds"] / steps,
+                train_tflops=flops_step * steps / tr["seconds"] / 1e12, p_eval=1e-3,
+                evals=[{k: v for k, v in e.items() if k != "shots"} for e in evals],
+                eval_shots=eval_shots)
 
 
 def main():
@@ -162,6 +199,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-dmax", type=int, default=15)
+    ap.add_argument("--mlp-steps", type=int, default=3000)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the frame sampler shards "
@@ -187,6 +225,13 @@ def main():
             log(f"  {extras['batched']}")
         except Exception as e:
             extras["batched"] = {"error": repr(e)}
+        if a.mlp_steps > 0:
+            log(f"perceptron decoder, {a.mlp_steps} training batches ...")
+            try:
+                extras["mlp_decoder"] = bench_mlp(a.mlp_steps)
+                log(f"  {extras['mlp_decoder']}")
+            except Exception as e:
+                extras["mlp_decoder"] = {"error": repr(e)}
     cpu = None
     if not a.no_cpu:
         log("cpu baseline (oracle) ...")
@@ -211,7 +256,8 @@ def main():
                    "shots_per_step": fr["shots"], "record_bits_per_shot": fr["nbits"],
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_BYTES_PER_SHOT * fr["shots"]},
         "int_valu": {"bitops_per_shot": fr["bitops_per_shot"], "bitops_s": fr["bitops_s"],
                      "lane_ops_s": fr["lane_ops_s"], "peak_lane_ops_s": INT_VALU_PEAK,
                      "frac": fr["lane_ops_s"] / INT_VALU_PEAK},

@@ -36,6 +36,7 @@ namespace {
 constexpr int kTile = 128;  // output tile edge
 constexpr int kBK = 32;     // K per LDS stage
 constexpr int kLdsRow = kBK + 8;  // padded LDS row (80 bytes)
+constexpr int kMaxSplitK = 16;
 
 enum Epi : int {
   kEpiStoreF32 = 0,   // C (fp32) = acc
@@ -109,6 +110,9 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  // split K: workgroup z covers K range [z*K/Z, (z+1)*K/Z) and, for the fp32
+  // store, writes its partial sum to slice z of the output
+  const int kz = g.K / (int)gridDim.z, kbase = (int)blockIdx.z * kz;
   f32x16 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -117,17 +121,17 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
   uint4 ra[2], rb[2];
-  load_tile<A_KMAJOR>(g.A, g.lda, m0, 0, t, ra);
-  load_tile<B_KMAJOR>(g.B, g.ldb, n0, 0, t, rb);
+  load_tile<A_KMAJOR>(g.A, g.lda, m0, kbase, t, ra);
+  load_tile<B_KMAJOR>(g.B, g.ldb, n0, kbase, t, rb);
   store_tile<A_KMAJOR>(As[0], t, ra);
   store_tile<B_KMAJOR>(Bs[0], t, rb);
   __syncthreads();
-  const int KT = g.K / kBK;
+  const int KT = kz / kBK;
   for (int kt = 0; kt < KT; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < KT) {
-      load_tile<A_KMAJOR>(g.A, g.lda, m0, (kt + 1) * kBK, t, ra);
-      load_tile<B_KMAJOR>(g.B, g.ldb, n0, (kt + 1) * kBK, t, rb);
+      load_tile<A_KMAJOR>(g.A, g.lda, m0, kbase + (kt + 1) * kBK, t, ra);
+      load_tile<B_KMAJOR>(g.B, g.ldb, n0, kbase + (kt + 1) * kBK, t, rb);
     }
 #pragma unroll
     for (int kk = 0; kk < kBK; kk += 16) {
@@ -164,7 +168,7 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
         const size_t o = (size_t)m * g.ldc + n;
         const float v = acc[i][j][r];
         if (EPI == kEpiStoreF32) {
-          g.Cf[o] = v;
+          g.Cf[o + (size_t)blockIdx.z * g.M * g.ldc] = v;
         } else if (EPI == kEpiBiasGelu) {
           const float p = v + bn;
           g.Cf[o] = p;
@@ -178,13 +182,36 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
     }
 }
 
+// out[i] = sum_z part[z][i] in a fixed order (split-K reduction).
+__global__ void sum_slices(const float *part, uint32_t nz, size_t n, float *out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (uint32_t z = 0; z < nz; ++z) s += part[z * n + i];
+  out[i] = s;
+}
+
+// `splitk` > 1 (fp32 store only) runs that many K slices into `part`
+// (splitk * M * ldc floats) and sums them into C.
 template <bool AK, bool BK, int EPI>
-int launch_gemm(const GemmArgs &g, hipStream_t s) {
+int launch_gemm(const GemmArgs &g, hipStream_t s, int splitk = 1, float *part = nullptr) {
   if (g.M % kTile || g.N % kTile || g.K % kBK || g.M <= 0 || g.N <= 0 || g.K <= 0)
     return set_error(QCF_ERR_DIMENSION, "gemm: M=%d N=%d K=%d must be multiples of %d/%d/%d",
                      g.M, g.N, g.K, kTile, kTile, kBK);
-  hipLaunchKernelGGL((gemm_bf16<AK, BK, EPI>), dim3(g.N / kTile, g.M / kTile), dim3(256), 0, s,
-                     g);
+  if (splitk > 1 && (EPI != kEpiStoreF32 || !part || g.K % (splitk * kBK)))
+    splitk = 1;
+  if (splitk == 1) {
+    hipLaunchKernelGGL((gemm_bf16<AK, BK, EPI>), dim3(g.N / kTile, g.M / kTile, 1), dim3(256), 0,
+                       s, g);
+  } else {
+    GemmArgs gp = g;
+    gp.Cf = part;
+    hipLaunchKernelGGL((gemm_bf16<AK, BK, EPI>), dim3(g.N / kTile, g.M / kTile, splitk),
+                       dim3(256), 0, s, gp);
+    const size_t n = (size_t)g.M * g.ldc;
+    hipLaunchKernelGGL(sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part,
+                       (uint32_t)splitk, n, g.Cf);
+  }
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
 }
@@ -353,6 +380,7 @@ struct qcf_mlp {
   bf16 *xT = nullptr, *a1T = nullptr, *a2T = nullptr, *dT = nullptr;  // batch-minor copies
   float *pre1 = nullptr, *pre2 = nullptr, *pre3 = nullptr;
   float *y = nullptr, *z = nullptr, *dz = nullptr, *part = nullptr, *lossbuf = nullptr;
+  float *kpart = nullptr;  // split-K partial sums of the weight gradients
   std::vector<void *> allocs;
   ~qcf_mlp() {
     int cur = 0;
@@ -437,7 +465,11 @@ int fwd_bwd(qcf_mlp *M, float *loss, hipStream_t s) {
     QCF_TRY(transpose(d, M->dT, B, hout, s));
     GemmArgs g{M->dT, aT, (int)hout, (int)hin, (int)B, (int)B, (int)B, (int)hin,
                gW, nullptr, nullptr, nullptr};
-    QCF_TRY((launch_gemm<true, true, kEpiStoreF32>(g, s)));
+    // few output tiles, long K: split K so that ~512 workgroups run
+    const int tiles = (int)((hout / kTile) * (hin / kTile));
+    int sk = 1;
+    while (sk * tiles < 512 && (int)B % (2 * sk * 256) == 0 && 2 * sk <= kMaxSplitK) sk *= 2;
+    QCF_TRY((launch_gemm<true, true, kEpiStoreF32>(g, s, sk, M->kpart)));
     QCF_TRY(colsum(M, d, nullptr, B, hout, gb, s));
     if (d_prev) {
       g = GemmArgs{d, W, (int)B, (int)hin, (int)hout, (int)hout, (int)hin, (int)hin,
@@ -527,6 +559,7 @@ extern "C" int qcf_mlp_create(const uint32_t *det_bits, uint32_t ndet, uint32_t 
   A(&M->z, R);
   A(&M->dz, R);
   A(&M->lossbuf, 1);
+  A(&M->kpart, (size_t)kMaxSplitK * std::max((size_t)h1 * M->kin, std::max((size_t)h2 * h1, (size_t)h3 * h2)));
   A(&M->part, (size_t)((batch + 127) / 128) * std::max(std::max(h1, h2), std::max(h3, M->kin)));
   if (st == QCF_OK) {
     std::vector<uint32_t> cols(ndet);
