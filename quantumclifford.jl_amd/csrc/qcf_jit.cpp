@@ -251,7 +251,15 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   // planes in global memory when the launcher found the frame too large for
   // LDS (variant bit 2), else registers for small programs, else LDS
   const bool glob = (variant & 4) != 0;
-  const bool regs = !glob && P->nplanes <= jit_reg_planes();
+  // registers pay an LDS exchange per noisy plane per noise pass; LDS planes
+  // pay LDS traffic per gate but take events for free, and measured faster
+  // until events get dense (d=5 x 25: 0.84 vs 0.94 ms at p=1e-3, 2.79 vs
+  // 2.07 ms at p=1e-2), so registers are used for small programs with many
+  // events per tile
+  const double ev_per_tile =
+      P->classes.empty() ? 0.0 : (double)P->sites.size() * kBlockShots * P->classes[0].p;
+  const bool regs = !glob && P->nplanes <= jit_reg_planes() &&
+                    (ev_per_tile >= jit_reg_events() || P->nplanes * 256u > 40u * 1024u);
   const bool nt = jit_nt_stores();
   constexpr uint32_t kSlots = 64;  // noise slots per event pass in register mode
   constexpr size_t kBarrierOps = 30;  // operands per compiler barrier
@@ -573,6 +581,13 @@ std::mutex g_jit_mu;
 constexpr size_t kCodeCacheMax = 32;
 std::unordered_map<std::string, std::vector<char>> g_code_cache;  // source -> code object
 }  // namespace
+
+// Planes go to registers only with at least this many expected noise events
+// per 2048-shot tile (QCF_JIT_REG_EVENTS).
+double jit_reg_events() {
+  const char *e = getenv("QCF_JIT_REG_EVENTS");
+  return e ? strtod(e, nullptr) : 20000.0;
+}
 
 // Programs with at most this many planes keep them in registers.
 uint32_t jit_reg_planes() {

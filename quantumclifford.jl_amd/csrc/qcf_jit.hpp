@@ -28,6 +28,7 @@ bool jit_disabled();
 // Largest plane count kept in registers (QCF_JIT_REG_PLANES, default 192) and
 // longest program specialised (QCF_JIT_MAX_INSTR, default 12000).
 uint32_t jit_reg_planes();
+double jit_reg_events();
 bool jit_no_loops();
 bool jit_global_enabled();
 bool jit_nt_stores();
