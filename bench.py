@@ -162,11 +162,12 @@ def cpu_baseline(target_s=12.0):
                        f"{dt:.1f} s")
 
 
-def bench_mlp(steps: int, eval_shots: int = 200_000):
+def bench_mlp(steps: int, eval_shots: int):
     """Perceptron decoder (600 -> 1024 -> 1024 -> 512 -> 1, BF16 on the matrix
     cores) trained for `steps` batches of 8192 device-sampled d=5 x 25 shots
-    at p log-spaced in [5e-4, 5e-3]; then logical error rates against the
-    matching decoder at p = 1e-3 with the fallback |logit| < tau -> matching."""
+    at p evenly spread over [5e-4, 5e-3]; then, on `eval_shots` fresh shots at
+    each p in {1e-3, 2e-3, 3e-3}, logical error rates against the matching
+    decoder, alone and with the fallback |logit| < tau -> matching."""
     from qcf import circuits, decoders, mlp
 
     def make(p):
@@ -181,13 +182,17 @@ def bench_mlp(steps: int, eval_shots: int = 200_000):
     layers = ((dec.kin, 1024), (1024, 1024), (1024, 512))
     flops_step = 2 * 8192 * (2 * sum(a * b for a, b in layers) +
                              sum(a * b for a, b in layers[1:]))
-    matcher = decoders.MatchingDecoder(c.ops, c.det_bits, c.obs_bits)
-    evals = [mlp.evaluate(dec, matcher, c.ops, shots=eval_shots, tau=tau, seed=7,
-                          chunk=eval_shots) for tau in (0.0, 1.0, 2.0, 4.0)]
-    return dict(train_steps=steps, train_s=tr["seconds"], ms_per_step=1e3 * tr["seconds"] / steps,
-                train_tflops=flops_step * steps / tr["seconds"] / 1e12, p_eval=1e-3,
-                evals=[{k: v for k, v in e.items() if k != "shots"} for e in evals],
-                eval_shots=eval_shots)
+    evals = {}
+    for p in (1e-3, 2e-3, 3e-3):
+        cp = make(p)
+        matcher = decoders.MatchingDecoder(cp.ops, cp.det_bits, cp.obs_bits)
+        rows = mlp.evaluate(dec, matcher, cp.ops, shots=eval_shots, taus=(0.0, 1.0, 2.0, 4.0),
+                            seed=7, chunk=1 << 20)
+        evals[str(p)] = [{k: v for k, v in r.items() if k != "shots"} for r in rows]
+    return dict(train_steps=steps, train_shots=steps * 8192, train_s=tr["seconds"],
+                ms_per_step=1e3 * tr["seconds"] / steps,
+                train_tflops=flops_step * steps / tr["seconds"] / 1e12,
+                eval_shots_per_p=eval_shots, evals=evals)
 
 
 def main():
@@ -199,7 +204,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batched-dmax", type=int, default=15)
-    ap.add_argument("--mlp-steps", type=int, default=3000)
+    ap.add_argument("--mlp-steps", type=int, default=24414)  # 2e8 shots of 8192
+    ap.add_argument("--mlp-eval-shots", type=int, default=10**7)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the frame sampler shards "
@@ -228,7 +234,7 @@ def main():
         if a.mlp_steps > 0:
             log(f"perceptron decoder, {a.mlp_steps} training batches ...")
             try:
-                extras["mlp_decoder"] = bench_mlp(a.mlp_steps)
+                extras["mlp_decoder"] = bench_mlp(a.mlp_steps, a.mlp_eval_shots)
                 log(f"  {extras['mlp_decoder']}")
             except Exception as e:
                 extras["mlp_decoder"] = {"error": repr(e)}

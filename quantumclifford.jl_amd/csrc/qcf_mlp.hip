@@ -35,8 +35,8 @@ namespace {
 
 constexpr int kTile = 128;  // output tile edge
 constexpr int kBK = 32;     // K per LDS stage
-constexpr int kLdsRow = kBK + 8;  // padded LDS row (80 bytes)
 constexpr int kMaxSplitK = 16;
+constexpr bool kWideK = true;  // 64-wide K stages when K allows
 
 enum Epi : int {
   kEpiStoreF32 = 0,   // C (fp32) = acc
@@ -63,18 +63,18 @@ __device__ __forceinline__ float dgelu(float x) {
   return cdf + x * pdf;
 }
 
-// One operand tile (128 rows of the output dimension x 32 of K) moves from
+// One operand tile (128 rows of the output dimension x BK of K) moves from
 // global memory to LDS as [row][k].  K-contiguous operands move 16 bytes per
 // thread-chunk; transposed ones are read along the output dimension and
-// scattered into the [row][k] layout.
-template <bool KMAJOR>
+// scattered into the [row][k] layout.  BK/16 chunks per thread.
+template <bool KMAJOR, int BK>
 __device__ __forceinline__ void load_tile(const bf16 *G, int ld, int r0, int k0, int t,
-                                          uint4 (&regs)[2]) {
+                                          uint4 (&regs)[BK / 16]) {
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < BK / 16; ++h) {
     const int c = t + h * 256;
     if (KMAJOR) {
-      const int row = c >> 2, kc = (c & 3) * 8;
+      const int row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
       regs[h] = *(const uint4 *)(G + (size_t)(r0 + row) * ld + k0 + kc);
     } else {
       const int k = c >> 4, rc = (c & 15) * 8;
@@ -83,13 +83,14 @@ __device__ __forceinline__ void load_tile(const bf16 *G, int ld, int r0, int k0,
   }
 }
 
-template <bool KMAJOR>
-__device__ __forceinline__ void store_tile(bf16 (*S)[kLdsRow], int t, const uint4 (&regs)[2]) {
+template <bool KMAJOR, int BK>
+__device__ __forceinline__ void store_tile(bf16 (*S)[BK + 8], int t,
+                                           const uint4 (&regs)[BK / 16]) {
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < BK / 16; ++h) {
     const int c = t + h * 256;
     if (KMAJOR) {
-      const int row = c >> 2, kc = (c & 3) * 8;
+      const int row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
       *(uint4 *)&S[row][kc] = regs[h];
     } else {
       const int k = c >> 4, rc = (c & 15) * 8;
@@ -103,10 +104,10 @@ __device__ __forceinline__ void store_tile(bf16 (*S)[kLdsRow], int t, const uint
 // C[m][n] = sum_k A(m,k) B(k,n) with A(m,k) = A[m*lda+k] (A_KMAJOR) or
 // A[k*lda+m], and B(k,n) = B[n*ldb+k] (B_KMAJOR) or B[k*ldb+n].
 // Requires M, N multiples of 128 and K a multiple of 32 (checked on the host).
-template <bool A_KMAJOR, bool B_KMAJOR, int EPI>
+template <bool A_KMAJOR, bool B_KMAJOR, int EPI, int BK>
 __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) bf16 As[2][kTile][kLdsRow];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[2][kTile][kLdsRow];
+  __shared__ __attribute__((aligned(16))) bf16 As[2][kTile][BK + 8];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[2][kTile][BK + 8];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
@@ -120,21 +121,21 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-  uint4 ra[2], rb[2];
-  load_tile<A_KMAJOR>(g.A, g.lda, m0, kbase, t, ra);
-  load_tile<B_KMAJOR>(g.B, g.ldb, n0, kbase, t, rb);
-  store_tile<A_KMAJOR>(As[0], t, ra);
-  store_tile<B_KMAJOR>(Bs[0], t, rb);
+  uint4 ra[BK / 16], rb[BK / 16];
+  load_tile<A_KMAJOR, BK>(g.A, g.lda, m0, kbase, t, ra);
+  load_tile<B_KMAJOR, BK>(g.B, g.ldb, n0, kbase, t, rb);
+  store_tile<A_KMAJOR, BK>(As[0], t, ra);
+  store_tile<B_KMAJOR, BK>(Bs[0], t, rb);
   __syncthreads();
-  const int KT = kz / kBK;
+  const int KT = kz / BK;
   for (int kt = 0; kt < KT; ++kt) {
     const int cur = kt & 1;
     if (kt + 1 < KT) {
-      load_tile<A_KMAJOR>(g.A, g.lda, m0, kbase + (kt + 1) * kBK, t, ra);
-      load_tile<B_KMAJOR>(g.B, g.ldb, n0, kbase + (kt + 1) * kBK, t, rb);
+      load_tile<A_KMAJOR, BK>(g.A, g.lda, m0, kbase + (kt + 1) * BK, t, ra);
+      load_tile<B_KMAJOR, BK>(g.B, g.ldb, n0, kbase + (kt + 1) * BK, t, rb);
     }
 #pragma unroll
-    for (int kk = 0; kk < kBK; kk += 16) {
+    for (int kk = 0; kk < BK; kk += 16) {
       bf16x8 a[2], b[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -149,8 +150,8 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     if (kt + 1 < KT) {
-      store_tile<A_KMAJOR>(As[cur ^ 1], t, ra);
-      store_tile<B_KMAJOR>(Bs[cur ^ 1], t, rb);
+      store_tile<A_KMAJOR, BK>(As[cur ^ 1], t, ra);
+      store_tile<B_KMAJOR, BK>(Bs[cur ^ 1], t, rb);
     }
     __syncthreads();
   }
@@ -193,21 +194,23 @@ __global__ void sum_slices(const float *part, uint32_t nz, size_t n, float *out)
 
 // `splitk` > 1 (fp32 store only) runs that many K slices into `part`
 // (splitk * M * ldc floats) and sums them into C.
-template <bool AK, bool BK, int EPI>
+template <bool AKM, bool BKM, int EPI>
 int launch_gemm(const GemmArgs &g, hipStream_t s, int splitk = 1, float *part = nullptr) {
   if (g.M % kTile || g.N % kTile || g.K % kBK || g.M <= 0 || g.N <= 0 || g.K <= 0)
     return set_error(QCF_ERR_DIMENSION, "gemm: M=%d N=%d K=%d must be multiples of %d/%d/%d",
                      g.M, g.N, g.K, kTile, kTile, kBK);
   if (splitk > 1 && (EPI != kEpiStoreF32 || !part || g.K % (splitk * kBK)))
     splitk = 1;
-  if (splitk == 1) {
-    hipLaunchKernelGGL((gemm_bf16<AK, BK, EPI>), dim3(g.N / kTile, g.M / kTile, 1), dim3(256), 0,
-                       s, g);
-  } else {
-    GemmArgs gp = g;
-    gp.Cf = part;
-    hipLaunchKernelGGL((gemm_bf16<AK, BK, EPI>), dim3(g.N / kTile, g.M / kTile, splitk),
-                       dim3(256), 0, s, gp);
+  // K stages of 64 where K allows (half the barriers per MFMA), else 32
+  const bool wide = kWideK && (g.K % (splitk * 64)) == 0;
+  GemmArgs gp = g;
+  if (splitk > 1) gp.Cf = part;
+  const dim3 grid(g.N / kTile, g.M / kTile, splitk);
+  if (wide)
+    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 64>), grid, dim3(256), 0, s, gp);
+  else
+    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 32>), grid, dim3(256), 0, s, gp);
+  if (splitk > 1) {
     const size_t n = (size_t)g.M * g.ldc;
     hipLaunchKernelGGL(sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part,
                        (uint32_t)splitk, n, g.Cf);

@@ -120,11 +120,11 @@ def train(decoder: MLPDecoder, make_circuit, steps: int, p_range=(5e-4, 5e-3), l
           batches_per_sample: int = 16):
     """Train on `steps` batches.  The frame sampler draws `batches_per_sample`
     batches per launch on the device from one of `n_rates` programs whose
-    noise rates are log-spaced over p_range, picked uniformly per launch
+    noise rates are evenly spaced over p_range, picked uniformly per launch
     (programs that differ only in their rate share one compiled sampling
     kernel); every batch is used once."""
     rng = np.random.default_rng(seed)
-    rates = np.exp(np.linspace(np.log(p_range[0]), np.log(p_range[1]), n_rates))
+    rates = np.linspace(p_range[0], p_range[1], n_rates)
     progs = [Program(make_circuit(float(p)).ops) for p in rates]
     B = decoder.batch
     per = max(1, int(batches_per_sample))
@@ -148,20 +148,30 @@ def train(decoder: MLPDecoder, make_circuit, steps: int, p_range=(5e-4, 5e-3), l
     return dict(steps=steps, seconds=time.perf_counter() - t0, history=hist)
 
 
-def evaluate(decoder: MLPDecoder, matcher, circuit, shots: int, tau: float, seed: int = 1,
-             chunk: int = 1 << 20):
-    """Logical error rate of the network alone, of matching alone, and of the
-    network with matching on the shots where |logit| < tau; plus the time
-    each decoder spent (network on the GPU, matching on the host)."""
-    from .decoders import unpack_planes
+def _bits_of(planes: np.ndarray, sel: np.ndarray) -> np.ndarray:
+    """Selected shots of packed planes (planes x words uint64) -> len(sel) x planes uint8."""
+    w = planes[:, sel >> 6]
+    return np.ascontiguousarray(((w >> (sel & 63).astype(np.uint64)) & np.uint64(1)).T.astype(np.uint8))
+
+
+def evaluate(decoder: MLPDecoder, matcher, circuit, shots: int, taus=(0.0,), seed: int = 1,
+             chunk: int = 1 << 20, tau=None):
+    """Logical error rates on `shots` fresh device-sampled shots: the network
+    alone, matching alone, and for each tau the network with matching on the
+    shots where |logit| < tau; with each decoder's throughput (network on the
+    GPU, matching on the host threads) and the combination's.  Returns one
+    dict per tau (a single dict when `tau` is given)."""
+    if tau is not None:
+        taus = (tau,)
     prog = circuit if isinstance(circuit, Program) else Program(circuit)
-    det_idx = np.asarray(decoder.det_bits, dtype=np.int64) - 1
+    det_idx = torch.from_numpy(np.asarray(decoder.det_bits, dtype=np.int64) - 1)
     obs_idx = decoder.obs_bit - 1
-    err_nn = err_mw = err_comb = 0
-    n_fallback = 0
-    t_nn = t_mw_all = t_mw_fb = 0.0
-    done = 0
-    k = 0
+    err_nn = err_mw = 0
+    err_comb = [0] * len(taus)
+    n_fb = [0] * len(taus)
+    t_fb = [0.0] * len(taus)
+    t_nn = t_mw = 0.0
+    done = k = 0
     while done < shots:
         s = min(chunk, shots - done)
         ld = words(s)
@@ -174,32 +184,34 @@ def evaluate(decoder: MLPDecoder, matcher, circuit, shots: int, tau: float, seed
         torch.cuda.synchronize()
         t_nn += time.perf_counter() - t
         planes = rec.view(-1, ld)
-        dets = planes[torch.from_numpy(det_idx).to(rec.device)].cpu().numpy().view(np.uint64)
-        obs = unpack_planes(planes[obs_idx:obs_idx + 1].cpu().numpy().view(np.uint64), s)[:, 0]
+        dets = planes[det_idx.to(rec.device)].cpu().numpy().view(np.uint64)
+        op = planes[obs_idx].cpu().numpy().view(np.uint64)
+        obs = ((op[np.arange(s) >> 6] >> (np.arange(s) & 63).astype(np.uint64)) &
+               np.uint64(1)).astype(np.uint8)
         zc = z.cpu().numpy()
         pred_nn = (zc > 0).astype(np.uint8)
         t = time.perf_counter()
         pred_mw = matcher.decode_packed(dets, s)[:, 0]
-        t_mw_all += time.perf_counter() - t
-        unsure = np.abs(zc) < tau
-        n_fallback += int(unsure.sum())
-        # time of matching only the unsure shots
-        if unsure.any():
-            sel = np.flatnonzero(unsure)
-            syn = unpack_planes(dets, s)[sel]
-            t = time.perf_counter()
-            fb = matcher.decode_observables(syn)[:, 0]
-            t_mw_fb += time.perf_counter() - t
-        comb = pred_nn.copy()
-        if unsure.any():
-            comb[sel] = fb
+        t_mw += time.perf_counter() - t
         err_nn += int(np.sum(pred_nn != obs))
         err_mw += int(np.sum(pred_mw != obs))
-        err_comb += int(np.sum(comb != obs))
+        for i, ta in enumerate(taus):
+            sel = np.flatnonzero(np.abs(zc) < ta)
+            comb = pred_nn.copy()
+            if len(sel):
+                syn = _bits_of(dets, sel)
+                t = time.perf_counter()
+                comb[sel] = matcher.decode_observables(syn)[:, 0]
+                t_fb[i] += time.perf_counter() - t
+            n_fb[i] += len(sel)
+            err_comb[i] += int(np.sum(comb != obs))
         done += s
         k += 1
-    return dict(shots=shots, ler_nn=err_nn / shots, ler_matching=err_mw / shots,
-                ler_combined=err_comb / shots, fallback_fraction=n_fallback / shots,
-                nn_shots_per_s=shots / max(t_nn, 1e-12),
-                matching_shots_per_s=shots / max(t_mw_all, 1e-12),
-                combined_shots_per_s=shots / max(t_nn + t_mw_fb, 1e-12), tau=tau)
+    out = []
+    for i, ta in enumerate(taus):
+        out.append(dict(shots=shots, tau=ta, ler_nn=err_nn / shots, ler_matching=err_mw / shots,
+                        ler_combined=err_comb[i] / shots, fallback_fraction=n_fb[i] / shots,
+                        nn_shots_per_s=shots / max(t_nn, 1e-12),
+                        matching_shots_per_s=shots / max(t_mw, 1e-12),
+                        combined_shots_per_s=shots / max(t_nn + t_fb[i], 1e-12)))
+    return out[0] if tau is not None else out
