@@ -859,15 +859,36 @@ extern "C" int qcf_matcher_decode_packed(const qcf_matcher *m, const uint64_t *d
   if (!m || (shots && (!dets || !obs_out))) return set_error(QCF_ERR_ARGUMENT, "null argument");
   if (ld * 64 < shots) return set_error(QCF_ERR_DIMENSION, "ld %zu words < %llu shots", ld,
                                          (unsigned long long)shots);
+  // one 64-shot word at a time: each detector word is read once and its set
+  // bits distributed to the shots' defect lists
   const uint32_t nd = m->ndet;
-  return decode_many(
-      m, shots, nthreads,
-      [&](uint64_t s, std::vector<uint32_t> *d) {
-        const uint64_t w = s >> 6, b = 1ull << (s & 63);
-        for (uint32_t j = 0; j < nd; ++j)
-          if (dets[(size_t)j * ld + w] & b) d->push_back(j);
-      },
-      obs_out, nullptr);
+  const uint64_t nwords = (shots + 63) / 64;
+  const unsigned nth = nthreads > 0 ? (unsigned)nthreads
+                                    : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::atomic<uint64_t> next{0};
+  auto worker = [&]() {
+    Blossom bl(1);
+    std::vector<std::vector<uint32_t>> lists(64);
+    for (uint64_t wi; (wi = next.fetch_add(1)) < nwords;) {
+      for (auto &l : lists) l.clear();
+      for (uint32_t j = 0; j < nd; ++j) {
+        uint64_t w = dets[(size_t)j * ld + wi];
+        while (w) {
+          lists[__builtin_ctzll(w)].push_back(j);
+          w &= w - 1;
+        }
+      }
+      const uint64_t s0 = wi * 64, s1 = std::min(shots, s0 + 64);
+      for (uint64_t s = s0; s < s1; ++s) {
+        const uint64_t o = decode_defects(m, lists[s - s0], bl, nullptr);
+        for (uint32_t j = 0; j < m->nobs; ++j) obs_out[s * m->nobs + j] = (uint8_t)((o >> j) & 1u);
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < nth; ++t) th.emplace_back(worker);
+  for (auto &t : th) t.join();
+  return QCF_OK;
 }
 
 extern "C" int qcf_debug_max_weight_matching(int n, const int64_t *w, int32_t *mate) {
