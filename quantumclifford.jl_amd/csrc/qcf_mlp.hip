@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -67,36 +68,48 @@ __device__ __forceinline__ float dgelu(float x) {
 // global memory to LDS as [row][k].  K-contiguous operands move 16 bytes per
 // thread-chunk; transposed ones are read along the output dimension and
 // scattered into the [row][k] layout.  BK/16 chunks per thread.
+// BK/16 chunks of 16 bytes per thread, held as one vector of 32-bit words
+// (a plain array of uint4 ends up in scratch memory)
+template <int BK>
+using TileRegs = uint32_t __attribute__((ext_vector_type(BK / 4)));
+
 template <bool KMAJOR, int BK>
 __device__ __forceinline__ void load_tile(const bf16 *G, int ld, int r0, int k0, int t,
-                                          uint4 (&regs)[BK / 16]) {
+                                          TileRegs<BK> &regs) {
 #pragma unroll
   for (int h = 0; h < BK / 16; ++h) {
     const int c = t + h * 256;
-    if (KMAJOR) {
+    uint4 v;
+    if constexpr (KMAJOR) {
       const int row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
-      regs[h] = *(const uint4 *)(G + (size_t)(r0 + row) * ld + k0 + kc);
+      v = *(const uint4 *)(G + (size_t)(r0 + row) * ld + k0 + kc);
     } else {
       const int k = c >> 4, rc = (c & 15) * 8;
-      regs[h] = *(const uint4 *)(G + (size_t)(k0 + k) * ld + r0 + rc);
+      v = *(const uint4 *)(G + (size_t)(k0 + k) * ld + r0 + rc);
     }
+    regs[4 * h + 0] = v.x;
+    regs[4 * h + 1] = v.y;
+    regs[4 * h + 2] = v.z;
+    regs[4 * h + 3] = v.w;
   }
 }
 
 template <bool KMAJOR, int BK>
-__device__ __forceinline__ void store_tile(bf16 (*S)[BK + 8], int t,
-                                           const uint4 (&regs)[BK / 16]) {
+__device__ __forceinline__ void store_tile(bf16 (*S)[BK + 8], int t, const TileRegs<BK> &regs) {
 #pragma unroll
   for (int h = 0; h < BK / 16; ++h) {
     const int c = t + h * 256;
-    if (KMAJOR) {
+    if constexpr (KMAJOR) {
       const int row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
-      *(uint4 *)&S[row][kc] = regs[h];
+      *(uint4 *)&S[row][kc] =
+          uint4{regs[4 * h], regs[4 * h + 1], regs[4 * h + 2], regs[4 * h + 3]};
     } else {
+      // the 8 values go to 8 rows
       const int k = c >> 4, rc = (c & 15) * 8;
-      const bf16 *v = (const bf16 *)&regs[h];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) S[rc + j][k] = v[j];
+      for (int j = 0; j < 8; ++j)
+        S[rc + j][k] =
+            __builtin_bit_cast(bf16, (uint16_t)(regs[4 * h + (j >> 1)] >> (16 * (j & 1))));
     }
   }
 }
@@ -104,10 +117,14 @@ __device__ __forceinline__ void store_tile(bf16 (*S)[BK + 8], int t,
 // C[m][n] = sum_k A(m,k) B(k,n) with A(m,k) = A[m*lda+k] (A_KMAJOR) or
 // A[k*lda+m], and B(k,n) = B[n*ldb+k] (B_KMAJOR) or B[k*ldb+n].
 // Requires M, N multiples of 128 and K a multiple of 32 (checked on the host).
-template <bool A_KMAJOR, bool B_KMAJOR, int EPI, int BK>
+template <bool A_KMAJOR, bool B_KMAJOR, int EPI, int BK, int STAGES>
 __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) bf16 As[2][kTile][BK + 8];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[2][kTile][BK + 8];
+  // STAGES == 2: double-buffered LDS, one barrier per K stage;
+  // STAGES == 1: one LDS buffer (half the LDS, twice the resident
+  // workgroups), two barriers per stage; both prefetch the next stage into
+  // registers while the matrix cores work on the current one
+  __shared__ __attribute__((aligned(16))) bf16 As[STAGES][kTile][BK + 8];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[STAGES][kTile][BK + 8];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
@@ -121,15 +138,22 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-  uint4 ra[BK / 16], rb[BK / 16];
+  TileRegs<BK> ra, rb;
   load_tile<A_KMAJOR, BK>(g.A, g.lda, m0, kbase, t, ra);
   load_tile<B_KMAJOR, BK>(g.B, g.ldb, n0, kbase, t, rb);
-  store_tile<A_KMAJOR, BK>(As[0], t, ra);
-  store_tile<B_KMAJOR, BK>(Bs[0], t, rb);
-  __syncthreads();
+  if (STAGES == 2) {
+    store_tile<A_KMAJOR, BK>(As[0], t, ra);
+    store_tile<B_KMAJOR, BK>(Bs[0], t, rb);
+    __syncthreads();
+  }
   const int KT = kz / BK;
   for (int kt = 0; kt < KT; ++kt) {
-    const int cur = kt & 1;
+    const int cur = STAGES == 2 ? (kt & 1) : 0;
+    if (STAGES == 1) {
+      store_tile<A_KMAJOR, BK>(As[0], t, ra);
+      store_tile<B_KMAJOR, BK>(Bs[0], t, rb);
+      __syncthreads();
+    }
     if (kt + 1 < KT) {
       load_tile<A_KMAJOR, BK>(g.A, g.lda, m0, kbase + (kt + 1) * BK, t, ra);
       load_tile<B_KMAJOR, BK>(g.B, g.ldb, n0, kbase + (kt + 1) * BK, t, rb);
@@ -149,7 +173,7 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
         for (int j = 0; j < 2; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
-    if (kt + 1 < KT) {
+    if (STAGES == 2 && kt + 1 < KT) {
       store_tile<A_KMAJOR, BK>(As[cur ^ 1], t, ra);
       store_tile<B_KMAJOR, BK>(Bs[cur ^ 1], t, rb);
     }
@@ -192,6 +216,16 @@ __global__ void sum_slices(const float *part, uint32_t nz, size_t n, float *out)
   out[i] = s;
 }
 
+// LDS buffers per K stage (QCF_GEMM_STAGES, 1 or 2; one measured faster:
+// 588 vs 536 TFLOP/s on the inference forward).
+int gemm_stages() {
+  static const int v = [] {
+    const char *e = getenv("QCF_GEMM_STAGES");
+    return e && e[0] == '2' ? 2 : 1;
+  }();
+  return v;
+}
+
 // `splitk` > 1 (fp32 store only) runs that many K slices into `part`
 // (splitk * M * ldc floats) and sums them into C.
 template <bool AKM, bool BKM, int EPI>
@@ -206,10 +240,15 @@ int launch_gemm(const GemmArgs &g, hipStream_t s, int splitk = 1, float *part = 
   GemmArgs gp = g;
   if (splitk > 1) gp.Cf = part;
   const dim3 grid(g.N / kTile, g.M / kTile, splitk);
-  if (wide)
-    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 64>), grid, dim3(256), 0, s, gp);
+  const bool one = gemm_stages() == 1;
+  if (wide && one)
+    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 64, 1>), grid, dim3(256), 0, s, gp);
+  else if (wide)
+    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 64, 2>), grid, dim3(256), 0, s, gp);
+  else if (one)
+    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 32, 1>), grid, dim3(256), 0, s, gp);
   else
-    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 32>), grid, dim3(256), 0, s, gp);
+    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 32, 2>), grid, dim3(256), 0, s, gp);
   if (splitk > 1) {
     const size_t n = (size_t)g.M * g.ldc;
     hipLaunchKernelGGL(sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part,
@@ -381,6 +420,7 @@ struct qcf_mlp {
   bf16 *x = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr;
   bf16 *d1 = nullptr, *d2 = nullptr, *d3 = nullptr;
   bf16 *xT = nullptr, *a1T = nullptr, *a2T = nullptr, *dT = nullptr;  // batch-minor copies
+  bf16 *WT = nullptr;  // transposed weights for the input gradients
   float *pre1 = nullptr, *pre2 = nullptr, *pre3 = nullptr;
   float *y = nullptr, *z = nullptr, *dz = nullptr, *part = nullptr, *lossbuf = nullptr;
   float *kpart = nullptr;  // split-K partial sums of the weight gradients
@@ -475,9 +515,11 @@ int fwd_bwd(qcf_mlp *M, float *loss, hipStream_t s) {
     QCF_TRY((launch_gemm<true, true, kEpiStoreF32>(g, s, sk, M->kpart)));
     QCF_TRY(colsum(M, d, nullptr, B, hout, gb, s));
     if (d_prev) {
-      g = GemmArgs{d, W, (int)B, (int)hin, (int)hout, (int)hout, (int)hin, (int)hin,
+      // W^T (hin x hout) so both operands are read K-contiguous
+      QCF_TRY(transpose(W, M->WT, hout, hin, s));
+      g = GemmArgs{d, M->WT, (int)B, (int)hin, (int)hout, (int)hout, (int)hout, (int)hin,
                    nullptr, d_prev, nullptr, pre_prev};
-      QCF_TRY((launch_gemm<true, false, kEpiDGelu>(g, s)));
+      QCF_TRY((launch_gemm<true, true, kEpiDGelu>(g, s)));
     }
     return QCF_OK;
   };
@@ -553,6 +595,7 @@ extern "C" int qcf_mlp_create(const uint32_t *det_bits, uint32_t ndet, uint32_t 
   A(&M->d3, (size_t)batch * h3);
   A(&M->xT, (size_t)batch * M->kin);
   A(&M->a1T, (size_t)batch * h1);
+  A(&M->WT, std::max((size_t)h2 * h1, (size_t)h3 * h2));
   A(&M->a2T, (size_t)batch * h2);
   A(&M->dT, (size_t)batch * std::max(std::max(h1, h2), h3));
   A(&M->pre1, (size_t)batch * h1);
