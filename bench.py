@@ -59,7 +59,7 @@ def bench_frames(steps, warmup, shots=10**7):
     from qcf import circuits
     c = circuits.rotated_surface_code(5, 25, 1e-3)
     prog = qcf.Program(c.ops)
-    ld = qcf.frames.words(shots)
+    ld = qcf.frames.plane_words(shots)
     rec = torch.empty(prog.nbits * ld, dtype=torch.int64, device="cuda")
     import ctypes as C
     seed = [0]
@@ -127,7 +127,7 @@ def bench_batched(ncirc=512, seed=0, dmax=15):
         st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
         for p, s, sd, r in zip(progs, shots, seeds, recs):
             qcf._lib.call("qcf_pf_sample", p.handle, s, sd, None, C.c_void_p(r.data_ptr()),
-                          None, qcf.frames.words(s), 1, st)
+                          None, qcf.frames.plane_words(s), 1, st)
 
     ms_s = time_launches(seq, 2, 1)
     return dict(ncirc=ncirc, total_shots=total, ms_batched=ms_b, ms_sequential=ms_s,

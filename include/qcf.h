@@ -215,13 +215,25 @@ int qcf_program_get_info(const qcf_program *prog, qcf_program_info *info);
  *            single-qubit noise site and four (x1,z1,x2,z2) per DEPOLARIZE2
  *            site, in op order.  With QCF_PF_EXPORT_MASKS the native draws
  *            are written there instead.
- *   ld     : words per plane, >= ceil(shots/64)
+ *   ld     : words per plane, >= ceil(shots/64); qcf_plane_words(shots)
+ *            is the recommended stride
  * Results do not depend on launch shape: the RNG stream of a shot depends only
  * on (seed, shot index).
  */
 int qcf_pf_sample(const qcf_program *prog, uint64_t shots, uint64_t seed,
                   uint64_t *frame, uint64_t *meas, uint64_t *masks, size_t ld,
                   uint32_t flags, void *stream);
+
+/*
+ * Recommended plane stride in words for `shots` shots: ceil(shots/2048)*32
+ * (at least 32).  Every 2048-shot tile of a plane is then one aligned
+ * 256-byte segment, so each record store writes whole memory sectors; a
+ * stride of exactly ceil(shots/64) words leaves planes misaligned whenever
+ * shots is not a multiple of 2048, and the partial sectors at every tile edge
+ * cost up to half the record bandwidth.  No reference counterpart (the
+ * reference's PauliFrame keeps one Bool per shot, src/pauli_frames.jl:30-37).
+ */
+size_t qcf_plane_words(uint64_t shots);
 
 /* Launch-shape knobs chosen by the autotuner (0 = library default). */
 typedef struct qcf_tune {

@@ -40,7 +40,7 @@ struct KParams {
   uint32_t nblocks;    // 2048-shot tiles (last tile index + 1)
   uint32_t blk0;       // first tile of this launch
   uint32_t nw32;       // valid 32-bit shot words
-  uint32_t nwbuf;      // 32-bit words up to the end of the last 64-bit word
+  uint32_t nwbuf;      // 32-bit words of a plane the last tile may store (<= stride)
   uint32_t lastmask;   // valid bits of word nw32-1
   uint64_t ld32;       // 32-bit words per plane of frame / record / masks
   uint64_t seed;
@@ -98,7 +98,7 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
                                          const Planes<GL> &F, uint32_t lane) {
   const uint4 *__restrict__ prog4 = P.prog;
   const uint32_t gw = blk * 64u + lane;
-  // lanes up to the end of the last 64-bit word store (zeros past the shots)
+  // lanes inside the plane stride store (zeros past the shots)
   const bool live = gw < P.nwbuf;
   const uint32_t lmask = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1 ? P.lastmask : 0xFFFFFFFFu);
   const uint64_t goff = gw;
@@ -521,8 +521,10 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
   QCF_TRY(ensure_device_copy(P, variant, &kp.prog, &kp.ninstr));
   const uint64_t nw32 = (shots + 31) / 32;
   kp.nw32 = (uint32_t)nw32;
-  kp.nwbuf = (uint32_t)(2 * ((shots + 63) / 64));
   kp.nblocks = (uint32_t)((nw32 + 63) / 64);
+  // lanes of the last tile that lie inside the plane stride store zeros past
+  // the shots, so a padded stride leaves defined (zero) padding
+  kp.nwbuf = (uint32_t)std::min<uint64_t>(2 * (uint64_t)ld, 64ull * kp.nblocks);
   const uint32_t tail = (uint32_t)(shots % 32);
   kp.lastmask = tail ? ((1u << tail) - 1u) : 0xFFFFFFFFu;
   kp.ld32 = 2 * (uint64_t)ld;
@@ -572,7 +574,15 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
       if (global) jwpc = std::min<uint32_t>(jwpc, kGlobalJitWavesPerCU);
       uint64_t g1 = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * jwpc);
       const uint32_t nx = jit_xcds();
-      if (nx > 1 && g1 >= nx) {
+      if (!global && jit_full_grid()) {
+        // one workgroup per tile, rounded up to whole XCD rounds (the extra
+        // workgroups find no tile and return)
+        g1 = kp.nblocks;
+        if (nx > 1 && g1 >= nx) {
+          g1 += (nx - g1 % nx) % nx;
+          jp.xcds = nx;
+        }
+      } else if (nx > 1 && g1 >= nx) {
         g1 -= g1 % nx;
         jp.xcds = nx;
       }
@@ -595,6 +605,10 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
 }  // namespace qcf
 
 using namespace qcf;
+
+extern "C" size_t qcf_plane_words(uint64_t shots) {
+  return (size_t)std::max<uint64_t>(1, (shots + 2047) / 2048) * 32;
+}
 
 extern "C" int qcf_pf_sample(const qcf_program *prog, uint64_t shots,
                              uint64_t seed, uint64_t *frame, uint64_t *meas,
@@ -717,8 +731,8 @@ int build_kparams(const qcf_program *prog, uint64_t shots, uint64_t seed,
   const uint64_t nw32 = (shots + 31) / 32;
   kp->nplanes = P->nplanes;
   kp->nw32 = (uint32_t)nw32;
-  kp->nwbuf = (uint32_t)(2 * ((shots + 63) / 64));
   kp->nblocks = (uint32_t)((nw32 + 63) / 64);
+  kp->nwbuf = (uint32_t)std::min<uint64_t>(2 * (uint64_t)ld, 64ull * kp->nblocks);
   const uint32_t tail = (uint32_t)(shots % 32);
   kp->lastmask = tail ? ((1u << tail) - 1u) : 0xFFFFFFFFu;
   kp->ld32 = 2 * (uint64_t)ld;
@@ -935,7 +949,7 @@ extern "C" int qcf_autotune(const qcf_program *const *progs, size_t n,
   std::vector<uint64_t> seeds(n, 0);
   int st = QCF_OK;
   for (size_t k = 0; k < n && st == QCF_OK; ++k) {
-    lds[k] = std::max<uint64_t>(1, (shots[k] + 63) / 64);
+    lds[k] = qcf_plane_words(shots[k]);
     const size_t bytes = std::max<size_t>(1, progs[k]->nbits) * lds[k] * 8;
     if (hipMalloc(&bufs[k], bytes) != hipSuccess) st = set_error(QCF_ERR_ALLOC, "autotune scratch");
   }

@@ -261,6 +261,23 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   const bool regs = !glob && P->nplanes <= jit_reg_planes() &&
                     (ev_per_tile >= jit_reg_events() || P->nplanes * 256u > 40u * 1024u);
   const bool nt = jit_nt_stores();
+  // LDS mode keeps the classical slots (planes 2*nq and up, which no noise
+  // site names) in registers: only the qubit planes take LDS, so more waves
+  // fit on a CU
+  const uint32_t qpl = 2u * P->nq;
+  bool hyb = !regs && !glob && P->nplanes > qpl;
+  for (size_t i = 0; hyb && i < code.size(); ++i) {
+    if ((code[i].op & 0xFFu) != I_NOISEG) continue;
+    const uint32_t G = code[i].op >> 8;
+    for (uint32_t k = 1; k <= G && i + k < code.size(); ++k) {
+      const Instr &s = code[i + k];
+      const uint32_t pl[4] = {s.a & 0xFFFFu, s.a >> 16, s.b & 0xFFFFu, s.b >> 16};
+      for (uint32_t j = 0; j < (s.b ? 4u : 2u); ++j)
+        if (pl[j] >= qpl) hyb = false;
+    }
+    i += G;
+  }
+  auto in_reg = [&](uint32_t p) { return regs || (hyb && p >= qpl); };
   constexpr uint32_t kSlots = 64;  // noise slots per event pass in register mode
   constexpr size_t kBarrierOps = 30;  // operands per compiler barrier
   struct Group {
@@ -309,7 +326,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   auto gaddr = [](uint32_t p) { return "F + (" + std::to_string(p * 64u) + "u + l)"; };
   // plane access: read expression, write and XOR statements
   auto PL = [&](uint32_t p) -> std::string {
-    if (regs) return "r" + std::to_string(p);
+    if (in_reg(p)) return "r" + std::to_string(p);
     if (glob) return "gld(" + gaddr(p) + ")";
     return plane(p);
   };
@@ -318,7 +335,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     return PL(p) + " = " + v + ";";
   };
   auto XOR = [&](uint32_t p, const std::string &v) -> std::string {
-    if (regs) return PL(p) + " ^= " + v + ";";
+    if (in_reg(p)) return PL(p) + " ^= " + v + ";";
     if (glob) return "gxr(" + gaddr(p) + ", " + v + ");";
     return "atomicXor(&" + plane(p) + ", " + v + ");";
   };
@@ -332,9 +349,10 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     return o.str();
   };
   auto emit = [&](bool tail, std::ostringstream &body) -> bool {
-  if (regs) {
-    body << "    uint32_t r0 = 0u";
-    for (uint32_t p = 1; p < P->nplanes; ++p) body << ", r" << p << " = 0u";
+  if (regs || hyb) {
+    const uint32_t p0 = regs ? 0u : qpl;
+    body << "    uint32_t r" << p0 << " = 0u";
+    for (uint32_t p = p0 + 1; p < P->nplanes; ++p) body << ", r" << p << " = 0u";
     body << ";\n";
   }
   if (!groups.empty()) body << "    uint32_t t0 = 0u, t1 = 0u, t2 = 0u, t3 = 0u;\n";
@@ -547,7 +565,10 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   src << "  }\n}\n";
   *out = src.str();
   if (lds_bytes)
-    *lds_bytes = glob ? 0u : regs ? std::max<uint32_t>(max_slots, 1u) * 256u : P->nplanes * 256u;
+    *lds_bytes = glob   ? 0u
+                 : regs ? std::max<uint32_t>(max_slots, 1u) * 256u
+                 : hyb  ? qpl * 256u
+                        : P->nplanes * 256u;
   return true;
 }
 
@@ -611,6 +632,13 @@ bool jit_global_enabled() {
 // QCF_JIT_NT=1 writes the record with nontemporal stores.
 bool jit_nt_stores() {
   const char *e = getenv("QCF_JIT_NT");
+  return e && e[0] == '1';
+}
+
+// QCF_JIT_FULL_GRID=1 launches one workgroup per tile (the dispatcher then
+// balances tiles over CUs) instead of a grid of resident waves.
+bool jit_full_grid() {
+  const char *e = getenv("QCF_JIT_FULL_GRID");
   return e && e[0] == '1';
 }
 

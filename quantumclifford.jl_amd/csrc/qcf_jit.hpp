@@ -33,6 +33,7 @@ bool jit_no_loops();
 bool jit_global_enabled();
 bool jit_nt_stores();
 uint32_t jit_xcds();
+bool jit_full_grid();
 size_t jit_max_instr();
 
 struct JitKernelInfo {

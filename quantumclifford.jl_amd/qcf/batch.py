@@ -9,7 +9,7 @@ import torch
 
 from . import _lib as _LIB
 from ._lib import Tune, call
-from .frames import Program, words
+from .frames import Program, plane_words
 
 
 def _stream():
@@ -19,7 +19,7 @@ def _stream():
 class Batch:
     """Fresh-frame sampling of several compiled circuits in ONE persistent kernel.
 
-    Records live in `self.records[k]` (nbits_k planes of words(shots_k) words).
+    Records live in `self.records[k]` (nbits_k planes of plane_words(shots_k) words).
     `launch()` enqueues the kernel (no allocation, capturable); `launch_graph()`
     replays a captured hipGraph of it."""
 
@@ -30,13 +30,13 @@ class Batch:
         self.progs = list(progs)
         self.shots = [int(s) for s in shots]
         n = len(self.progs)
-        self.records = [torch.empty(max(1, p.nbits) * max(1, words(s)), dtype=torch.int64,
+        self.records = [torch.empty(max(1, p.nbits) * plane_words(s), dtype=torch.int64,
                                     device=dev) for p, s in zip(self.progs, self.shots)]
         hs = (C.c_void_p * n)(*[p.handle.value for p in self.progs])
         sh = (C.c_uint64 * n)(*self.shots)
         sd = (C.c_uint64 * n)(*[int(s) & (2**64 - 1) for s in seeds])
         ms = (C.c_void_p * n)(*[r.data_ptr() for r in self.records])
-        ld = (C.c_size_t * n)(*[max(1, words(s)) for s in self.shots])
+        ld = (C.c_size_t * n)(*[plane_words(s) for s in self.shots])
         h = C.c_void_p()
         call("qcf_batch_create", hs, n, sh, sd, ms, ld,
              C.byref(tune) if tune is not None else None, C.byref(h))
@@ -50,7 +50,7 @@ class Batch:
 
     def record_host(self, k: int) -> np.ndarray:
         p, s = self.progs[k], self.shots[k]
-        return self.records[k].cpu().numpy().view(np.uint64).reshape(-1, max(1, words(s)))[: p.nbits]
+        return self.records[k].cpu().numpy().view(np.uint64).reshape(-1, plane_words(s))[: p.nbits]
 
     def __del__(self):
         h = getattr(self, "handle", None)

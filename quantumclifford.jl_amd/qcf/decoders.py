@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from .frames import Program, pftrajectories, words
+from .frames import Program, pftrajectories
 
 
 def _np_ptr(a: np.ndarray):
@@ -189,7 +189,7 @@ def sample_detectors(circuit, det_bits, obs_bits, shots: int, seed: int):
     """Sample on the GPU; return (detector planes ndet x ld, observable planes
     nobs x ld) on the host, packed 64 shots per uint64 word."""
     f = pftrajectories(circuit, shots, seed=seed, keep_frame=False)
-    rec = f.record_host().reshape(-1, words(shots))
+    rec = f.record_host()
     return (rec[np.asarray(det_bits) - 1], rec[np.asarray(obs_bits) - 1])
 
 

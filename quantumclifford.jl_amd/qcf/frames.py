@@ -111,7 +111,15 @@ class Program:
 
 
 def words(shots: int) -> int:
+    """Words that hold `shots` shot bits: ceil(shots/64)."""
     return (shots + 63) // 64
+
+
+def plane_words(shots: int) -> int:
+    """Plane stride in words used for frames, records and masks: ceil(shots/2048)*32,
+    at least 32, so that every 2048-shot tile of a plane is one aligned 256-byte
+    segment (qcf_plane_words)."""
+    return max(1, (int(shots) + 2047) // 2048) * 32
 
 
 class PauliFrame:
@@ -122,7 +130,7 @@ class PauliFrame:
         self.shots = int(frames)
         self.nq = int(qubits) + 1  # hidden ancilla for PauliMeasurement
         self.nbits = int(measurements)
-        self.ld = max(1, words(self.shots))
+        self.ld = plane_words(self.shots)
         dev = torch.device(device) if device is not None else torch.device(
             "cuda", torch.cuda.current_device())
         self.frame = torch.zeros(2 * self.nq * self.ld, dtype=torch.int64, device=dev)
@@ -202,7 +210,7 @@ def pftrajectories(circuit, trajectories: int = 5000, threads: bool = True, *,
 def alloc_masks(prog: Program, shots: int, device=None) -> torch.Tensor:
     dev = torch.device(device) if device is not None else torch.device(
         "cuda", torch.cuda.current_device())
-    return torch.zeros(max(1, prog.nreplay) * words(shots), dtype=torch.int64, device=dev)
+    return torch.zeros(max(1, prog.nreplay) * plane_words(shots), dtype=torch.int64, device=dev)
 
 
 def measurements(frame: PauliFrame) -> torch.Tensor:

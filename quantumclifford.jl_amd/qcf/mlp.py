@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .frames import Program, words
+from .frames import Program, plane_words
 
 
 def _stream():
@@ -128,7 +128,7 @@ def train(decoder: MLPDecoder, make_circuit, steps: int, p_range=(5e-4, 5e-3), l
     progs = [Program(make_circuit(float(p)).ops) for p in rates]
     B = decoder.batch
     per = max(1, int(batches_per_sample))
-    ld = words(B * per)
+    ld = plane_words(B * per)
     rec = torch.empty(max(1, progs[0].nbits) * ld, dtype=torch.int64, device="cuda")
     st = _stream()
     hist = []
@@ -174,7 +174,7 @@ def evaluate(decoder: MLPDecoder, matcher, circuit, shots: int, taus=(0.0,), see
     done = k = 0
     while done < shots:
         s = min(chunk, shots - done)
-        ld = words(s)
+        ld = plane_words(s)
         rec = torch.empty(max(1, prog.nbits) * ld, dtype=torch.int64, device="cuda")
         _lib.call("qcf_pf_sample", prog.handle, s, seed * 1000003 + k, None, _ptr(rec), None, ld,
                   1, _stream())

@@ -29,6 +29,17 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) == set(_lib.exported_symbols())
 
 
+def test_plane_words_aligns_tiles():
+    """The recommended plane stride holds every shot and is a whole number of
+    256-byte tiles (32 words per 2048 shots); Python and C agree."""
+    f = qcf.lib().qcf_plane_words
+    for shots in (0, 1, 63, 64, 2047, 2048, 2049, 10**7, 2**33 + 5):
+        w = f(shots)
+        assert w == qcf.frames.plane_words(shots)
+        assert w % 32 == 0 and w * 64 >= shots and w >= 32
+        assert w - 32 < max(32, qcf.frames.words(shots))
+
+
 def test_version():
     assert b"gfx950" in qcf.lib().qcf_version()
 

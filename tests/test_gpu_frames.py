@@ -17,14 +17,21 @@ pytestmark = pytest.mark.gpu
 SHOT_COUNTS = [1, 31, 32, 33, 100, 2047, 2048, 2049, 4096 + 5]
 
 
+def padded(planes, ld):
+    """(nplanes, words) uint64 -> flat int64 tensor of nplanes planes of `ld` words."""
+    out = np.zeros((planes.shape[0], ld), np.uint64)
+    out[:, : planes.shape[1]] = planes
+    return torch.from_numpy(out.view(np.int64).reshape(-1))
+
+
 def gpu_inplace(circ, nqubits, nbits, shots, frame_bits, meas_bits, masks_bits=None):
     """Run pftrajectories(state, circuit) on the GPU from given frame bits."""
     f = qcf.PauliFrame(shots, nqubits, nbits, init=False)
-    f.frame.copy_(torch.from_numpy(oracle.pack_shots(frame_bits).view(np.int64).reshape(-1)))
-    f.record.copy_(torch.from_numpy(oracle.pack_shots(meas_bits).view(np.int64).reshape(-1)))
+    f.frame.copy_(padded(oracle.pack_shots(frame_bits), f.ld))
+    f.record.copy_(padded(oracle.pack_shots(meas_bits), f.ld))
     masks = None
     if masks_bits is not None:
-        masks = torch.from_numpy(oracle.pack_shots(masks_bits).view(np.int64).reshape(-1)).cuda()
+        masks = padded(oracle.pack_shots(masks_bits), f.ld).cuda()
     qcf.pftrajectories_state(f, circ, masks=masks)
     return (oracle.unpack_shots(f.frame_host(), shots),
             oracle.unpack_shots(f.record_host(), shots))
@@ -141,8 +148,8 @@ def fresh_export_and_replay(circ, shots, seed):
     prog = qcf.Program(circ)
     masks = qcf.alloc_masks(prog, shots)
     f = qcf.pftrajectories(prog, shots, seed=seed, export_masks=masks)
-    ld = qcf.frames.words(shots)
-    mk = masks.cpu().numpy().view(np.uint64).reshape(-1, ld)[: prog.nreplay]
+    assert f.ld == qcf.frames.plane_words(shots) and f.ld % 32 == 0
+    mk = masks.cpu().numpy().view(np.uint64).reshape(-1, f.ld)[: prog.nreplay]
     ops, args = prog.ops, prog.args
     xzs, om = oracle.pf_run(ops, args, prog.nq, prog.nbits, shots, masks=mk)
     gm = oracle.unpack_shots(f.record_host(), shots)
@@ -213,6 +220,26 @@ def test_native_rates_match_oracle_statistics(gpu):
     dets = np.array(c.det_bits) - 1
     assert p1[dets].mean() > 1e-3  # detectors actually fire
     assert z.max() < 5.0, (int(z.argmax()), float(z.max()), p1[z.argmax()], p2[z.argmax()])
+
+
+@pytest.mark.parametrize("shots", [1, 100, 2047, 2049, 5000])
+def test_padding_inside_stride_is_zero(gpu, shots):
+    """A record buffer prefilled with ones: every word past the shots, up to
+    the padded stride, comes back zero; the shots themselves match a
+    zero-initialised run."""
+    import ctypes as C
+    c = circuits.rotated_surface_code(3, 3, 5e-3)
+    prog = qcf.Program(c.ops)
+    ld = qcf.frames.plane_words(shots)
+    rec = torch.full((prog.nbits * ld,), -1, dtype=torch.int64, device="cuda")
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    qcf._lib.call("qcf_pf_sample", prog.handle, shots, 77, None, C.c_void_p(rec.data_ptr()),
+                  None, ld, 1, st)
+    host = rec.cpu().numpy().view(np.uint64).reshape(prog.nbits, ld)
+    bits = oracle.unpack_shots(host, ld * 64)
+    assert not bits[:, shots:].any()
+    want = qcf.pftrajectories(prog, shots, seed=77, keep_frame=False).record_host()
+    assert np.array_equal(host, want)
 
 
 def test_noiseless_surface_code_native(gpu):

@@ -84,8 +84,8 @@ def test_in_place_specialised_equals_interpreter(gpu, monkeypatch):
 
     def run():
         f = qcf.PauliFrame(shots, n, nbits, init=False)
-        f.frame.copy_(torch.from_numpy(fb.view(np.int64).reshape(-1)))
-        f.record.copy_(torch.from_numpy(mb.view(np.int64).reshape(-1)))
+        f.frame.view(2 * (n + 1), f.ld)[:, : fb.shape[1]].copy_(torch.from_numpy(fb.view(np.int64)))
+        f.record.view(nbits, f.ld)[:, : mb.shape[1]].copy_(torch.from_numpy(mb.view(np.int64)))
         qcf.pftrajectories_state(f, prog, seed=77)
         return f.record_host(), f.frame_host()
 
