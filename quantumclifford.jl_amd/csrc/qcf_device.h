@@ -183,6 +183,57 @@ __device__ __forceinline__ void group_events(EvState &ev, uint32_t ord0, uint32_
   }
 }
 
+// Event pass of one noise group whose site entries a specialised kernel holds
+// in registers for its whole run: lane i has site i's planes, packed four
+// 8-bit fields x1 | z1 << 8 | x2 << 16 | z2 << 24 in `pa` (P8), else the
+// 16-bit pairs of w0/w1 in `pa`/`pb`; `w2`/`w3` are per-lane like `pa` when
+// W2L/W3L, else the group's common value.  TWO / ONE: every site of the
+// group acts on two / on one qubit.  Same events and flips as group_events.
+template <bool TWO, bool ONE, bool W2L, bool W3L, bool P8, class Flip>
+__device__ __forceinline__ void group_events_r(EvState &ev, uint32_t ord0, uint32_t G,
+                                               uint32_t pa, uint32_t pb, uint32_t w2,
+                                               uint32_t w3, float c, Flip flip) {
+  const uint32_t lo = ord0 * kTileShots;
+  const uint32_t span = G * kTileShots;
+  while (true) {
+    const uint32_t rel = ev.pos - lo;
+    const bool inr = rel < span;
+    if (__builtin_amdgcn_ballot_w64(inr && ev.uacc != 0xFFFFFFFFu) != 0) {
+      const uint32_t src = inr ? (rel >> 11) : 0u;
+      const uint32_t a = bperm(src, pa);
+      const uint32_t b = P8 ? 0u : bperm(src, pb);
+      const uint32_t t = W2L ? bperm(src, w2) : w2;
+      const uint32_t acc = W3L ? bperm(src, w3) : w3;
+      if (inr && ev.uacc < acc) {
+        const uint32_t within = rel & (kTileShots - 1u);
+        const uint32_t tl = within >> 5;
+        const uint32_t bit = 1u << (within & 31u);
+        const uint32_t x1 = P8 ? (a & 0xFFu) : (a & 0xFFFFu);
+        const uint32_t z1 = P8 ? ((a >> 8) & 0xFFu) : (a >> 16);
+        const uint32_t q2 = P8 ? (a >> 16) : b;
+        const bool two = TWO ? true : ONE ? false : q2 != 0u;
+        uint32_t fl;
+        if (two) {
+          fl = __umulhi(ev.upau, 15u) + 1u;
+        } else {
+          const uint32_t u15 = ev.upau >> 17, t1 = t & 0xFFFFu, t2 = t >> 16;
+          fl = (u15 < t1 || u15 >= t2 ? 1u : 0u) | (u15 >= t1 ? 2u : 0u);
+        }
+        if (fl & 1u) flip(x1, tl, bit, 0u, 0u);
+        if (fl & 2u) flip(z1, tl, bit, 1u, 0u);
+        if (!ONE) {
+          const uint32_t x2 = P8 ? (q2 & 0xFFu) : (q2 & 0xFFFFu);
+          const uint32_t z2 = P8 ? (q2 >> 8) : (q2 >> 16);
+          if (fl & 4u) flip(x2, tl, bit, 2u, 0u);
+          if (fl & 8u) flip(z2, tl, bit, 3u, 0u);
+        }
+      }
+    }
+    if (ev.last >= lo + span) break;
+    ev = gen_batch(ev, c);
+  }
+}
+
 }  // namespace qcfd
 
 #endif  // QCF_DEVICE_H

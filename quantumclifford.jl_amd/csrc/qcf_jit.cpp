@@ -13,6 +13,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -280,10 +281,11 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   auto in_reg = [&](uint32_t p) { return regs || (hyb && p >= qpl); };
   constexpr uint32_t kSlots = 64;  // noise slots per event pass in register mode
   constexpr size_t kBarrierOps = 30;  // operands per compiler barrier
+  constexpr size_t kEntryRegs = 48;  // site-entry registers a kernel may hold
   struct Group {
     uint32_t site0, count, ord0, slots;
   };
-  std::ostringstream tab;
+  std::vector<std::array<uint32_t, 4>> sw;  // site table words (w0..w3) per site
   std::vector<Group> groups;
   std::vector<size_t> range_groups;      // first group of each range
   std::vector<const Instr *> sites;
@@ -303,11 +305,9 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
           if (regs && g.slots + need > kSlots) break;
           if (regs) {
             const uint32_t b0 = g.slots;
-            tab << (b0 | (b0 + 1u) << 16) << "u,"
-                << (s.b ? ((b0 + 2u) | (b0 + 3u) << 16) : 0u) << "u," << s.d << "u," << s.e
-                << "u,\n";
+            sw.push_back({b0 | (b0 + 1u) << 16, s.b ? ((b0 + 2u) | (b0 + 3u) << 16) : 0u, s.d, s.e});
           } else {
-            tab << s.a << "u," << s.b << "u," << s.d << "u," << s.e << "u,\n";
+            sw.push_back({s.a, s.b, s.d, s.e});
           }
           sites.push_back(&s);
           site_slot.push_back(g.slots);
@@ -316,6 +316,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
           ++k;
         }
         max_slots = std::max(max_slots, g.slots);
+        if (g.count > 64u) return false;  // one site per lane
         groups.push_back(g);
       }
       i += G;
@@ -323,6 +324,64 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   }
   range_groups.push_back(groups.size());
   const uint32_t nsite = (uint32_t)sites.size();
+  // Site entries held in registers for the whole kernel: lane i of a group's
+  // registers holds its site i.  Planes pack into one register when every
+  // plane index fits 8 bits; thresholds shared by a whole group become
+  // constants.  Loaded once before the first tile, so no load in the tile
+  // loop waits behind the record stores (loads and stores share one counter).
+  bool p8 = true;
+  for (const auto &w : sw)
+    for (uint32_t v : {w[0] & 0xFFFFu, w[0] >> 16, w[1] & 0xFFFFu, w[1] >> 16})
+      if (v > 0xFFu) p8 = false;
+  struct GDesc {
+    bool two, one, w2l, w3l;
+    uint32_t w2c, w3c, ra, rb, r2, r3;
+  };
+  std::vector<GDesc> gd(groups.size());
+  std::vector<std::array<uint32_t, 64>> ereg;  // entry registers, 64 lanes each
+  auto new_reg = [&](auto word_of, const Group &g) {
+    std::array<uint32_t, 64> r{};
+    for (uint32_t l = 0; l < 64; ++l)
+      r[l] = word_of(sw[g.site0 + std::min(l, g.count - 1u)]);
+    ereg.push_back(r);
+    return (uint32_t)ereg.size() - 1u;
+  };
+  for (size_t k = 0; k < groups.size(); ++k) {
+    const Group &g = groups[k];
+    GDesc d{true, true, false, false, 0u, 0u, kNone, kNone, kNone, kNone};
+    bool w2set = false;
+    for (uint32_t j = 0; j < g.count; ++j) {
+      const auto &w = sw[g.site0 + j];
+      (w[1] ? d.one : d.two) = false;
+      if (!w[1]) {  // thresholds matter for one-qubit sites only
+        if (!w2set) d.w2c = w[2], w2set = true;
+        else if (w[2] != d.w2c) d.w2l = true;
+      }
+      if (j == 0) d.w3c = w[3];
+      else if (w[3] != d.w3c) d.w3l = true;
+    }
+    if (p8) {
+      d.ra = new_reg([](const std::array<uint32_t, 4> &w) {
+        return (w[0] & 0xFFu) | ((w[0] >> 16) & 0xFFu) << 8 | (w[1] & 0xFFu) << 16 |
+               ((w[1] >> 16) & 0xFFu) << 24;
+      }, g);
+    } else {
+      d.ra = new_reg([](const std::array<uint32_t, 4> &w) { return w[0]; }, g);
+      d.rb = new_reg([](const std::array<uint32_t, 4> &w) { return w[1]; }, g);
+    }
+    if (d.w2l) d.r2 = new_reg([](const std::array<uint32_t, 4> &w) { return w[2]; }, g);
+    if (d.w3l) d.r3 = new_reg([](const std::array<uint32_t, 4> &w) { return w[3]; }, g);
+    gd[k] = d;
+  }
+  const bool preload = !groups.empty() && ereg.size() <= kEntryRegs;
+  std::ostringstream tab;
+  if (preload) {
+    for (const auto &r : ereg)
+      for (uint32_t l = 0; l < 64; ++l) tab << r[l] << "u,";
+  } else {
+    for (const auto &w : sw) tab << w[0] << "u," << w[1] << "u," << w[2] << "u," << w[3] << "u,\n";
+  }
+  const uint32_t ntab = preload ? (uint32_t)ereg.size() * 64u : 4u * nsite;
   auto gaddr = [](uint32_t p) { return "F + (" + std::to_string(p * 64u) + "u + l)"; };
   // plane access: read expression, write and XOR statements
   auto PL = [&](uint32_t p) -> std::string {
@@ -355,7 +414,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     for (uint32_t p = p0 + 1; p < P->nplanes; ++p) body << ", r" << p << " = 0u";
     body << ";\n";
   }
-  if (!groups.empty()) body << "    uint32_t t0 = 0u, t1 = 0u, t2 = 0u, t3 = 0u;\n";
+  if (!groups.empty() && !preload) body << "    uint32_t t0 = 0u, t1 = 0u, t2 = 0u, t3 = 0u;\n";
   for (size_t ri = 0; ri < ranges.size(); ++ri) {
     const Range &rg = ranges[ri];
     if (rg.b == rg.e) continue;
@@ -383,7 +442,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
            << "u, ob = it * " << (uint32_t)lp.dord << "u;\n    (void)cb; (void)ob;\n";
     size_t gi = range_groups[ri];
     const size_t gend = range_groups[ri + 1];
-    if (gi < gend) body << load_group(gi, "t");
+    if (gi < gend && !preload) body << load_group(gi, "t");
     for (size_t i = rg.b; i < rg.e; ++i) {
       const Instr &in = code[i];
       const uint32_t op = in.op & 0xFFu;
@@ -405,11 +464,22 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
           for (uint32_t done = 0; done < G; ++gi) {
             const Group &g = groups[gi];
             body << "    {\n";
-            if (gi + 1 < gend) body << "    uint32_t n0, n1, n2, n3;\n" << load_group(gi + 1, "n");
-            body << "    if (P.has_class) qcfd::group_events(ev, " << g.ord0
-                 << (in_loop ? "u + ob, " : "u, ") << g.count
-                 << "u, 0u, t0, t1, t2, t3, 0u, P.c, flip);\n";
-            if (gi + 1 < gend) body << "    t0 = n0; t1 = n1; t2 = n2; t3 = n3;\n";
+            if (preload) {
+              const GDesc &d = gd[gi];
+              auto q = [](uint32_t r) { return "q" + std::to_string(r); };
+              body << "    if (P.has_class) qcfd::group_events_r<" << d.two << ", " << d.one
+                   << ", " << d.w2l << ", " << d.w3l << ", " << p8 << ">(ev, " << g.ord0
+                   << (in_loop ? "u + ob, " : "u, ") << g.count << "u, " << q(d.ra) << ", "
+                   << (p8 ? "0u" : q(d.rb)) << ", " << (d.w2l ? q(d.r2) : std::to_string(d.w2c) + "u")
+                   << ", " << (d.w3l ? q(d.r3) : std::to_string(d.w3c) + "u") << ", P.c, flip);\n";
+            } else {
+              if (gi + 1 < gend)
+                body << "    uint32_t n0, n1, n2, n3;\n" << load_group(gi + 1, "n");
+              body << "    if (P.has_class) qcfd::group_events(ev, " << g.ord0
+                   << (in_loop ? "u + ob, " : "u, ") << g.count
+                   << "u, 0u, t0, t1, t2, t3, 0u, P.c, flip);\n";
+              if (gi + 1 < gend) body << "    t0 = n0; t1 = n1; t2 = n2; t3 = n3;\n";
+            }
             if (regs) {
               std::vector<uint32_t> touched;
               for (uint32_t k = 0; k < g.count; ++k) {
@@ -523,7 +593,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   src << "// folded: prologue " << lp.s << ", loop " << lp.R << " x " << lp.L << ", of "
       << code.size() << " instructions\n";
   src << kDeviceSrc << kKernelPrelude;
-  src << "__constant__ uint32_t QS[" << std::max<uint32_t>(4, 4 * nsite) << "] = {\n"
+  src << "__constant__ uint32_t QS[" << std::max<uint32_t>(4, ntab) << "] = {\n"
       << (nsite ? tab.str() : std::string("0u,0u,0u,0u")) << "};\n";
   src << "extern \"C\" __global__ void __launch_bounds__(64) qcf_jit_kernel(KJ P) {\n"
       << "  extern __shared__ uint32_t L[];\n"
@@ -531,6 +601,17 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
       << (glob ? "  uint32_t *const F = P.scratch + (uint64_t)blockIdx.x * P.sstride;\n"
                  "  (void)L;\n  const FlipG flip{F};\n"
                : "  const FlipLDS flip{L};\n");
+  if (preload) {
+    // entry registers, made opaque so that they stay in registers
+    for (size_t r = 0; r < ereg.size(); ++r)
+      src << "  uint32_t q" << r << " = QS[" << r * 64 << "u + l];\n";
+    for (size_t r0 = 0; r0 < ereg.size(); r0 += kBarrierOps) {
+      src << "  asm volatile(\"\" :";
+      for (size_t r = r0; r < std::min(ereg.size(), r0 + kBarrierOps); ++r)
+        src << (r > r0 ? ", " : " ") << "\"+v\"(q" << r << ")";
+      src << ");\n";
+    }
+  }
   if (regs)  // the slot buffer starts at zero and every pass leaves it at zero
     src << "  for (uint32_t k = 0; k < " << std::max<uint32_t>(max_slots, 1u)
         << "u; ++k) L[k * 64u + l] = 0u;\n";
