@@ -26,10 +26,10 @@ for _p in (ROOT, PKG):
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per shot of the d=5 x 25 launch from rocprofv3 --pmc (profiles/
-# pf_jit_kernel_d5x25_traffic.md): WRITE_SIZE 1.544e6 KiB and FETCH_SIZE
-# 696 KiB per 10^7-shot dispatch, scaled by what the counters read for a
+# pf_jit_kernel_d5x25_traffic.md): WRITE_SIZE 1,496,639.5 KiB and FETCH_SIZE
+# 325.7 KiB per 10^7-shot dispatch, scaled by what the counters read for a
 # 1 GiB copy (WRITE_SIZE 1.0x, FETCH_SIZE 0.5x of the bytes moved).
-TRAFFIC_BYTES_PER_SHOT = (1.544e6 * 1.0 + 696.4 * 2.0) * 1024 / 1e7
+TRAFFIC_BYTES_PER_SHOT = (1496639.5 * 1.0 + 325.7 * 2.0) * 1024 / 1e7
 INT_VALU_PEAK = 256 * 64 * 2.4e9  # int32 lane-ops/s (256 CUs x 64 lanes x 2.4 GHz)
 METRIC = "batched vs per-circuit shots/s, d=5 surface 25 rds; NN vs MWPM LER"
 
@@ -165,10 +165,10 @@ def cpu_baseline(target_s=12.0):
 def bench_mlp(steps: int, eval_shots: int):
     """Perceptron decoder (600 -> 1024 -> 1024 -> 512 -> 1, BF16 on the matrix
     cores) trained for `steps` batches of 8192 device-sampled d=5 x 25 shots
-    at p evenly spread over [5e-4, 5e-3]; then, on fresh shots (`eval_shots`
-    at p = 1e-3, a tenth of that at 2e-3 and 3e-3, where matching is slower),
-    logical error rates against the matching decoder, alone and with the
-    fallback |logit| < tau -> matching."""
+    at p evenly spread over [5e-4, 5e-3]; then, on `eval_shots` fresh shots at
+    each p in {1e-3, 2e-3, 3e-3}, logical error rates against the matching
+    decoder, alone and with the fallback |logit| < tau -> matching (five
+    thresholds at 1e-3, three at the higher rates, where matching is slower)."""
     from qcf import circuits, decoders, mlp
 
     def make(p):
@@ -185,8 +185,9 @@ def bench_mlp(steps: int, eval_shots: int):
                              sum(a * b for a, b in layers[1:]))
     log(f"  trained {steps} batches in {tr['seconds']:.1f} s")
     evals = {}
-    for p, n in ((1e-3, eval_shots), (2e-3, max(1, eval_shots // 10)),
-                 (3e-3, max(1, eval_shots // 10))):
+    for p, taus in ((1e-3, (0.0, 1.0, 1.5, 2.0, 4.0)), (2e-3, (0.0, 1.5, 2.0)),
+                    (3e-3, (0.0, 1.5, 2.0))):
+        n = eval_shots
         cp = make(p)
         matcher = decoders.MatchingDecoder(cp.ops, cp.det_bits, cp.obs_bits)
         rows = []
@@ -194,8 +195,8 @@ def bench_mlp(steps: int, eval_shots: int):
         done = 0
         while done < n:
             s = min(1 << 20, n - done)
-            rows.append(mlp.evaluate(dec, matcher, cp.ops, shots=s, taus=(0.0, 1.0, 1.5, 2.0, 4.0),
-                                     seed=7 + done, chunk=s))
+            rows.append(mlp.evaluate(dec, matcher, cp.ops, shots=s, taus=taus, seed=7 + done,
+                                     chunk=s))
             done += s
             log(f"  p={p}: {done} shots evaluated")
         merged = []

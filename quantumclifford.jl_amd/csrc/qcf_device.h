@@ -188,7 +188,8 @@ __device__ __forceinline__ void group_events(EvState &ev, uint32_t ord0, uint32_
 // 8-bit fields x1 | z1 << 8 | x2 << 16 | z2 << 24 in `pa` (P8), else the
 // 16-bit pairs of w0/w1 in `pa`/`pb`; `w2`/`w3` are per-lane like `pa` when
 // W2L/W3L, else the group's common value.  TWO / ONE: every site of the
-// group acts on two / on one qubit.  Same events and flips as group_events.
+// group acts on two / on one qubit.  Same events and flips as group_events;
+// the flip callback must accept a zero bit (a no-op XOR).
 template <bool TWO, bool ONE, bool W2L, bool W3L, bool P8, class Flip>
 __device__ __forceinline__ void group_events_r(EvState &ev, uint32_t ord0, uint32_t G,
                                                uint32_t pa, uint32_t pb, uint32_t w2,
@@ -219,13 +220,15 @@ __device__ __forceinline__ void group_events_r(EvState &ev, uint32_t ord0, uint3
           const uint32_t u15 = ev.upau >> 17, t1 = t & 0xFFFFu, t2 = t >> 16;
           fl = (u15 < t1 || u15 >= t2 ? 1u : 0u) | (u15 >= t1 ? 2u : 0u);
         }
-        if (fl & 1u) flip(x1, tl, bit, 0u, 0u);
-        if (fl & 2u) flip(z1, tl, bit, 1u, 0u);
+        // every plane of the site takes an XOR, with zero where the Pauli
+        // leaves it alone: no divergent branch per plane
+        flip(x1, tl, (fl & 1u) ? bit : 0u, 0u, 0u);
+        flip(z1, tl, (fl & 2u) ? bit : 0u, 1u, 0u);
         if (!ONE) {
           const uint32_t x2 = P8 ? (q2 & 0xFFu) : (q2 & 0xFFFFu);
           const uint32_t z2 = P8 ? (q2 >> 8) : (q2 >> 16);
-          if (fl & 4u) flip(x2, tl, bit, 2u, 0u);
-          if (fl & 8u) flip(z2, tl, bit, 3u, 0u);
+          flip(x2, tl, (fl & 4u) ? bit : 0u, 2u, 0u);
+          flip(z2, tl, (fl & 8u) ? bit : 0u, 3u, 0u);
         }
       }
     }

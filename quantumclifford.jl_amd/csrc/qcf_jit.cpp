@@ -254,13 +254,16 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   const bool glob = (variant & 4) != 0;
   // registers pay an LDS exchange per noisy plane per noise pass; LDS planes
   // pay LDS traffic per gate but take events for free, and measured faster
-  // until events get dense (d=5 x 25: 0.84 vs 0.94 ms at p=1e-3, 2.79 vs
-  // 2.07 ms at p=1e-2), so registers are used for small programs with many
+  // until events get dense (d=5 x 25: 0.62 vs 0.76 ms at p=1e-3, 2.1 vs
+  // 1.64 ms at p=1e-2), so registers are used for small programs with many
   // events per tile
   const double ev_per_tile =
       P->classes.empty() ? 0.0 : (double)P->sites.size() * kBlockShots * P->classes[0].p;
+  // noiseless programs take no exchanges at all and gain the occupancy of
+  // registers (d=5 x 25: 0.295 vs 0.395 ms)
   const bool regs = !glob && P->nplanes <= jit_reg_planes() &&
-                    (ev_per_tile >= jit_reg_events() || P->nplanes * 256u > 40u * 1024u);
+                    (P->classes.empty() || ev_per_tile >= jit_reg_events() ||
+                     2u * P->nq * 256u > 40u * 1024u);
   const bool nt = jit_nt_stores();
   // LDS mode keeps the classical slots (planes 2*nq and up, which no noise
   // site names) in registers: only the qubit planes take LDS, so more waves
@@ -279,7 +282,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     i += G;
   }
   auto in_reg = [&](uint32_t p) { return regs || (hyb && p >= qpl); };
-  constexpr uint32_t kSlots = 64;  // noise slots per event pass in register mode
+  const uint32_t kSlots = jit_slots();  // noise slots per event pass in register mode
   constexpr size_t kBarrierOps = 30;  // operands per compiler barrier
   constexpr size_t kEntryRegs = 48;  // site-entry registers a kernel may hold
   struct Group {
@@ -689,6 +692,15 @@ std::unordered_map<std::string, std::vector<char>> g_code_cache;  // source -> c
 double jit_reg_events() {
   const char *e = getenv("QCF_JIT_REG_EVENTS");
   return e ? strtod(e, nullptr) : 20000.0;
+}
+
+// Noise slots (256 bytes of LDS each) one event pass may use in register
+// mode (QCF_JIT_SLOTS, 16..128; default 80: 20 KB, so LDS still lets eight
+// waves share a CU, and a layer of 20 two-qubit noise sites is one pass).
+uint32_t jit_slots() {
+  const char *e = getenv("QCF_JIT_SLOTS");
+  const uint32_t v = e ? (uint32_t)strtoul(e, nullptr, 10) : 80u;
+  return std::min<uint32_t>(128u, std::max<uint32_t>(16u, v));
 }
 
 // Programs with at most this many planes keep them in registers.

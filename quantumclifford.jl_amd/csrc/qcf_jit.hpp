@@ -34,6 +34,7 @@ bool jit_global_enabled();
 bool jit_nt_stores();
 uint32_t jit_xcds();
 bool jit_full_grid();
+uint32_t jit_slots();
 size_t jit_max_instr();
 
 struct JitKernelInfo {
