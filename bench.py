@@ -27,9 +27,9 @@ for _p in (ROOT, PKG):
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per shot of the d=5 x 25 launch from rocprofv3 --pmc (profiles/
 # pf_jit_kernel_d5x25_traffic.md): WRITE_SIZE 1,496,639.5 KiB and FETCH_SIZE
-# 325.7 KiB per 10^7-shot dispatch, scaled by what the counters read for a
+# 287.6 KiB per 10^7-shot dispatch, scaled by what the counters read for a
 # 1 GiB copy (WRITE_SIZE 1.0x, FETCH_SIZE 0.5x of the bytes moved).
-TRAFFIC_BYTES_PER_SHOT = (1496639.5 * 1.0 + 325.7 * 2.0) * 1024 / 1e7
+TRAFFIC_BYTES_PER_SHOT = (1496639.5 * 1.0 + 287.6 * 2.0) * 1024 / 1e7
 INT_VALU_PEAK = 256 * 64 * 2.4e9  # int32 lane-ops/s (256 CUs x 64 lanes x 2.4 GHz)
 METRIC = "batched vs per-circuit shots/s, d=5 surface 25 rds; NN vs MWPM LER"
 
@@ -97,6 +97,38 @@ def bench_mul(steps, warmup, n=10**9):
     return dict(n=n, ms=ms, gbs=nbytes / (ms * 1e-3) / 1e9, bytes=nbytes)
 
 
+def bench_mul_config(steps, warmup):
+    """The rest of the mul_left! configuration: two random Paulis of 10^6
+    qubits (launch-bound), and the pivot sweep (every row m != pivot:
+    mul_left!(s, m, pivot), src/canonicalization.jl:105-109) on stabilizer
+    tableaux of 1000 and 10000 qubits built by a random Clifford circuit
+    (4n H/S/CNOT gates, seed 0) from |0...0>, plain and canonicalize!-style
+    (rows with a random half of a mask); algorithmic bytes of a sweep: read
+    and write every other row, read the pivot row, phases."""
+    import numpy as np
+    import torch
+    import qcf
+    out = {}
+    rng = np.random.default_rng(0)
+    n = 10**6
+    p1, x1 = qcf.randoms.random_pauli_words(rng, n)
+    p2, x2 = qcf.randoms.random_pauli_words(rng, n)
+    l, r = qcf.PauliOperator(p1, n, x1), qcf.PauliOperator(p2, n, x2)
+    ms = time_launches(lambda: qcf.mul_left_(r, l), steps * 10, warmup)
+    out["mul_left_1mq"] = dict(n=n, ms=ms, gbs=6 * l.nwords * 8 / (ms * 1e-3) / 1e9)
+    for n in (1000, 10000):
+        ph, xzs = qcf.randoms.random_clifford_tableau(np.random.default_rng(0), n)
+        t = qcf.Tableau(ph, n, xzs)
+        W = t.nwords
+        nbytes = (n - 1) * 2 * (2 * W * 8) + 2 * W * 8 + 2 * n
+        ms = time_launches(lambda: qcf.mul_left_sweep_(t, 1), steps * 5, warmup)
+        mask = torch.from_numpy((np.random.default_rng(1).integers(0, 2, n)).astype(np.uint8)).cuda()
+        ms_m = time_launches(lambda: qcf.mul_left_sweep_(t, 1, row_mask=mask), steps * 5, warmup)
+        out[f"sweep_{n}"] = dict(n=n, rows=n, ms=ms, gbs=nbytes / (ms * 1e-3) / 1e9,
+                                 bytes=nbytes, ms_masked=ms_m)
+    return out
+
+
 def bench_batched(ncirc=512, seed=0, dmax=15):
     """512 rotated memory circuits, d ~ U{3..dmax odd}, rounds = d, p log-uniform
     [1e-4,1e-2], shots log-uniform [1e4,1e6]: one persistent launch vs one launch
@@ -156,10 +188,22 @@ def cpu_baseline(target_s=12.0):
         dt += time.perf_counter() - t0
         shots += chunk
         k += 1
+    # mul_left! (the reference's is single-threaded): the oracle's scalar
+    # restatement on one host core, two random 10^8-qubit operators
+    n = 10**8
+    a_ = np.random.default_rng(2).integers(0, 2**64, 2 * (n // 64), dtype=np.uint64)
+    b_ = np.random.default_rng(3).integers(0, 2**64, 2 * (n // 64), dtype=np.uint64)
+    oracle.mul_left_vec(a_, b_)
+    t0 = time.perf_counter()
+    reps = 10
+    for _ in range(reps):
+        oracle.mul_left_vec(a_, b_)
+    dtm = (time.perf_counter() - t0) / reps
+    mul = dict(n=n, ms=dtm * 1e3, gbs=6 * (n // 64) * 8 / dtm / 1e9, cores=1, kind="port")
     return dict(value=shots / dt, unit="shots/s", cores=cores, kind="port",
                 sample=f"{shots} shots of the d=5 x 25-round circuit (p=1e-3) in chunks of "
                        f"{chunk}, oracle restatement with OpenMP over {cores} threads, "
-                       f"{dt:.1f} s")
+                       f"{dt:.1f} s", mul_left=mul)
 
 
 def bench_mlp(steps: int, eval_shots: int):
@@ -246,6 +290,12 @@ def main():
             log(f"  {extras['mul_left_1gq']}")
         except Exception as e:  # reported, not hidden
             extras["mul_left_1gq"] = {"error": repr(e)}
+        log("mul_left! 10^6 qubits and tableau sweeps (1000, 10000 qubits) ...")
+        try:
+            extras["mul_config"] = bench_mul_config(10, 2)
+            log(f"  {extras['mul_config']}")
+        except Exception as e:  # reported, not hidden
+            extras["mul_config"] = {"error": repr(e)}
         log("batched 512 circuits ...")
         try:
             extras["batched"] = bench_batched(dmax=a.batched_dmax)

@@ -556,7 +556,9 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
   // faster for d >= 9 while costing seconds of compilation.
   if (!masks && !exp && !jit_disabled() && (!global || jit_global_enabled())) {
     JitKernelInfo jk;
-    int js = jit_get(P, variant | (global ? 4 : 0), &jk);
+    // records under 4 GB take 32-bit byte offsets from one base
+    const bool rec32 = (uint64_t)std::max<uint32_t>(P->nbits, 1u) * ld * 8ull < (1ull << 32);
+    int js = jit_get(P, variant | (global ? 4 : 0) | (rec32 ? 8 : 0), &jk);
     if (js == QCF_OK) {
       JitParams jp{};
       jp.rec = kp.rec;

@@ -252,6 +252,10 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   // planes in global memory when the launcher found the frame too large for
   // LDS (variant bit 2), else registers for small programs, else LDS
   const bool glob = (variant & 4) != 0;
+  // variant bit 3: the record spans less than 4 GB, so a record address is
+  // one 64-bit base plus a 32-bit byte offset (one scalar multiply per
+  // store instead of a 64-bit multiply-add)
+  const bool rec32 = (variant & 8) != 0;
   // registers pay an LDS exchange per noisy plane per noise pass; LDS planes
   // pay LDS traffic per gate but take events for free, and measured faster
   // until events get dense (d=5 x 25: 0.62 vs 0.76 ms at p=1e-3, 2.1 vs
@@ -423,6 +427,9 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     if (rg.b == rg.e) continue;
     const bool in_loop = rg.loop;
     auto at = [&](const char *arr, uint32_t col, bool shifted) {
+      if (rec32 && std::string(arr) == "P.rec")
+        return "*(uint32_t *)(recb + (uint32_t)((" + std::to_string(col) +
+               (shifted ? "u + cb" : "u") + ") * ld4 + gw4))";
       return std::string(arr) + "[(uint64_t)(" + std::to_string(col) +
              (shifted ? "u + cb" : "u") + ") * ld + gw]";
     };
@@ -634,6 +641,11 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     // hoisting every record address out of the tile loop
     uint64_t ld = P.ld32;
     asm volatile("" : "+s"(ld));
+    uint32_t ld4 = (uint32_t)ld * 4u;  // plane stride in bytes (rec32 kernels)
+    asm volatile("" : "+s"(ld4));
+    char *const recb = (char *)P.rec;
+    const uint32_t gw4 = gw * 4u;
+    (void)ld4; (void)recb; (void)gw4;
     qcfd::Xoshiro g = qcfd::gauge_stream(P.seed, gw);
     qcfd::EvState ev = qcfd::ev_init(P.seed, blk, l);
     if (P.has_class) ev = qcfd::gen_batch(ev, P.c);
@@ -837,7 +849,7 @@ extern "C" int qcf_debug_jit_source(const qcf_program *prog, int variant, char *
   qcf::clear_error();
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src;
-  if (!qcf::generate(prog, variant & 7, &src, nullptr))
+  if (!qcf::generate(prog, variant & 15, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   if (needed) *needed = src.size() + 1;
   if (buf && cap) {
@@ -853,7 +865,7 @@ extern "C" int qcf_debug_jit_compile(const qcf_program *prog, int variant, size_
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src, log;
   std::vector<char> bin;
-  if (!qcf::generate(prog, variant & 7, &src, nullptr))
+  if (!qcf::generate(prog, variant & 15, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   QCF_TRY(qcf::compile(src, &bin, &log));
   if (code_bytes) *code_bytes = bin.size();

@@ -27,30 +27,32 @@ def random_clifford_tableau(rng: np.random.Generator, n: int, ngates: int | None
     xzs uint64[n, 2W]) row-contiguous."""
     w = (n + 63) // 64
     ngates = 4 * n if ngates is None else ngates
-    x = np.zeros((n, n), np.uint8)
-    z = np.eye(n, dtype=np.uint8)
+    # qubit-major bit matrices (xt[q, row]) so that each gate touches
+    # contiguous rows of them
+    xt = np.zeros((n, n), np.uint8)
+    zt = np.eye(n, dtype=np.uint8)
     ph = np.zeros(n, np.uint8)
     kinds = rng.integers(0, 3, ngates)
     qa = rng.integers(0, n, ngates)
     qb = rng.integers(0, n - 1, ngates) if n > 1 else np.zeros(ngates, np.int64)
     for k, a, b in zip(kinds, qa, qb):
         if k == 0:  # H: (z, x), phase if x&z
-            ph = (ph + 2 * (x[:, a] & z[:, a])) & 3
-            x[:, a], z[:, a] = z[:, a].copy(), x[:, a].copy()
+            ph = (ph + 2 * (xt[a] & zt[a])) & 3
+            xt[a], zt[a] = zt[a].copy(), xt[a].copy()
         elif k == 1:  # S: (x, x^z), phase if x&z
-            ph = (ph + 2 * (x[:, a] & z[:, a])) & 3
-            z[:, a] ^= x[:, a]
+            ph = (ph + 2 * (xt[a] & zt[a])) & 3
+            zt[a] ^= xt[a]
         elif n > 1:  # CNOT(a, b)
             b = b + (b >= a)
-            x1, z1, x2, z2 = x[:, a], z[:, a], x[:, b], z[:, b]
+            x1, z1, x2, z2 = xt[a], zt[a], xt[b], zt[b]
             flip = (x1 & z1 & x2 & z2) | (x1 & z2 & (1 - (z1 | x2)))
             ph = (ph + 2 * flip) & 3
-            z[:, a] = z1 ^ z2
-            x[:, b] = x2 ^ x1
+            zt[a] = z1 ^ z2
+            xt[b] = x2 ^ x1
     xs = np.zeros((n, w * 64), np.uint8)
     zs = np.zeros((n, w * 64), np.uint8)
-    xs[:, :n] = x
-    zs[:, :n] = z
+    xs[:, :n] = xt.T
+    zs[:, :n] = zt.T
     px = np.packbits(xs, axis=1, bitorder="little").view(np.uint64)
     pz = np.packbits(zs, axis=1, bitorder="little").view(np.uint64)
     return ph.astype(np.uint8), np.ascontiguousarray(np.concatenate([px, pz], axis=1))
