@@ -167,6 +167,42 @@ def bench_batched(ncirc=512, seed=0, dmax=15):
                 ratio=ms_s / ms_b, tune=[tune.waves_per_cu, tune.tile_blocks, tune.lds_cap], dmax=dmax)
 
 
+def bench_reference_circuits(steps, warmup):
+    """The reference's own frame-benchmark circuits: the noiseless 441-qubit
+    unrotated surface code of benchmark/surface-code-circuit.jl (d=11, 12
+    rounds, 2751 record bits; benchmark-surface-code-gpu.jl times 2^3..2^19
+    trajectories) at 2^19 and 10^7 shots, and x_diag_circuit(1000) at 1000
+    trajectories (benchmarks.jl q1001_r10000)."""
+    import ctypes as C
+    import torch
+    import qcf
+    from qcf import circuits
+    out = {}
+    cases = (("surface_code_d11_2pow19", circuits.unrotated_surface_code_benchmark(11, 12), 2**19),
+             ("surface_code_d11_1e7", circuits.unrotated_surface_code_benchmark(11, 12), 10**7),
+             ("x_diag_1000_1000shots", circuits.x_diag_circuit(1000), 1000))
+    for name, ops, shots in cases:
+        prog = qcf.Program(ops)
+        ld = qcf.frames.plane_words(shots)
+        rec = torch.empty(prog.nbits * ld, dtype=torch.int64, device="cuda")
+        seed = [100]
+
+        def step():
+            seed[0] += 1
+            qcf._lib.call("qcf_pf_sample", prog.handle, shots, seed[0], None,
+                          C.c_void_p(rec.data_ptr()), None, ld, 1,
+                          C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+        ms = time_launches(step, steps, warmup)
+        sps = shots / (ms * 1e-3)
+        out[name] = dict(shots=shots, ms=ms, shots_s=sps, nqubits=prog.nq - 1,
+                         record_bits=prog.nbits, bitops_s=prog.bitops_per_shot * sps,
+                         rec_gbs=prog.nbits / 8 * sps / 1e9)
+        log(f"  {name}: {out[name]}")
+        del rec
+    return out
+
+
 def cpu_baseline(target_s=12.0):
     """The oracle (C restatement, OpenMP over shots like the reference's
     Threads.@threads split) on the d=5 x 25 circuit with per-shot RNG draws."""
@@ -296,6 +332,11 @@ def main():
             log(f"  {extras['mul_config']}")
         except Exception as e:  # reported, not hidden
             extras["mul_config"] = {"error": repr(e)}
+        log("the reference's benchmark circuits ...")
+        try:
+            extras["reference_circuits"] = bench_reference_circuits(5, 1)
+        except Exception as e:  # reported, not hidden
+            extras["reference_circuits"] = {"error": repr(e)}
         log("batched 512 circuits ...")
         try:
             extras["batched"] = bench_batched(dmax=a.batched_dmax)

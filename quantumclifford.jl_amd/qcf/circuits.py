@@ -212,3 +212,79 @@ def rotated_surface_code(d: int = 5, rounds: int | None = None, p: float = 1e-3,
                       meta=dict(kind="rotated_memory_z", d=d, rounds=rounds, p=p,
                                 ndata=len(data), nmeas_q=len(meas)), det_pos=dpos)
     return _append_xors(c)
+
+
+# ---- the reference's own benchmark circuits --------------------------------
+def unrotated_surface_code_benchmark(d: int = 11, rounds: int = 12) -> list:
+    """The noiseless unrotated surface-code circuit the reference benchmarks its
+    GPU frame sampler on (benchmark/surface-code-circuit.jl, included by
+    benchmark/benchmark-surface-code-gpu.jl:11; d=11, 12 rounds: 441 qubits,
+    17,802 operations, 2,751 measurement bits), rebuilt from its structure.
+
+    Qubits sit on an L x L grid, L = 2d-1, qubit = L*row + col + 1: data where
+    row+col is even, X-type measure qubits at (odd row, even col), Z-type at
+    (even row, odd col).  Every list below is in ascending qubit order.
+      * sMRZ(q, 0) on the data, then on the X-type measure qubits;
+      * round 1: H on the X-type measure qubits; CNOT layers in the neighbour
+        order left, up, down, right, X-type measure qubit as control, data as
+        target; H again; sMZ of the X-type measure qubits;
+      * rounds 2..rounds: sMRZ(q, 0) on all measure qubits; H; the same four
+        layers over all measure qubits, each layer in ascending measure-qubit
+        order (Z-type: data as control, measure qubit as target); H; sMZ of
+        all measure qubits, and in the last round of the data after them.
+    Measurement bits are numbered in order from 1."""
+    L = 2 * d - 1
+
+    def q(r, c):
+        return L * r + c + 1
+
+    cells = [(r, c) for r in range(L) for c in range(L)]
+    data = [q(r, c) for r, c in cells if (r + c) % 2 == 0]
+    xm = [(r, c) for r, c in cells if r % 2 == 1 and c % 2 == 0]
+    zm = [(r, c) for r, c in cells if r % 2 == 0 and c % 2 == 1]
+    meas = sorted(xm + zm, key=lambda rc: q(*rc))
+    xq = [q(*m) for m in xm]
+    order = [(0, -1), (-1, 0), (1, 0), (0, 1)]
+    ops = [sMRZ(x, 0) for x in data] + [sMRZ(x, 0) for x in xq]
+    bit = 0
+    for r in range(rounds):
+        checks = xm if r == 0 else meas
+        if r > 0:
+            ops += [sMRZ(q(*m), 0) for m in meas]
+        ops += [sHadamard(x) for x in xq]
+        for dr, dc in order:
+            for m in checks:
+                nr, nc = m[0] + dr, m[1] + dc
+                if not (0 <= nr < L and 0 <= nc < L):
+                    continue
+                if m[0] % 2 == 1:
+                    ops.append(sCNOT(q(*m), q(nr, nc)))
+                else:
+                    ops.append(sCNOT(q(nr, nc), q(*m)))
+        ops += [sHadamard(x) for x in xq]
+        targets = [q(*m) for m in checks] + (data if r == rounds - 1 else [])
+        for x in targets:
+            bit += 1
+            ops.append(sMZ(x, bit))
+    return ops
+
+
+def x_diag_circuit(csize: int) -> list:
+    """benchmark/benchmarks.jl:103-110: H on qubits 1..csize, each followed by a
+    CNOT onto qubit csize+1, which is then measured into bit 1."""
+    ops = []
+    for i in range(1, csize + 1):
+        ops += [sHadamard(i), sCNOT(i, csize + 1)]
+    ops.append(sMZ(csize + 1, 1))
+    return ops
+
+
+def x_diag_circuit_noisy_measurement(csize: int) -> list:
+    """benchmark/benchmarks.jl:161-171: per i in 1..csize, PauliError(i, 0.1),
+    H(i), CNOT(i -> csize+1), sMZ(csize+1, i), then ClassicalXOR of bits
+    1..(i mod 6 + 2) into bit i."""
+    ops = []
+    for i in range(1, csize + 1):
+        ops += [PauliError(i, 0.1), sHadamard(i), sCNOT(i, csize + 1), sMZ(csize + 1, i),
+                ClassicalXOR(tuple(range(1, i % 6 + 3)), i)]
+    return ops
