@@ -203,6 +203,33 @@ def bench_reference_circuits(steps, warmup):
     return out
 
 
+def bench_canonicalize(steps, warmup):
+    """canonicalize! on the device: a random 1000-qubit stabilizer (the
+    reference's README.md:107-115 benchmark, 8.5 ms on one Zen4 core) and
+    Stabilizer(random_tableau(6000, 6000)) (test/test_gpu_canonicalization.jl:
+    19-29, under 1900 ms asserted; 1900-2400 ms on the CPU per that test)."""
+    import numpy as np
+    import qcf
+    out = {}
+    cases = (("stabilizer_1000", qcf.randoms.random_clifford_tableau(np.random.default_rng(0), 1000),
+              1000),
+             ("random_tableau_6000", qcf.randoms.random_tableau(np.random.default_rng(0), 6000, 6000),
+              6000))
+    for name, (ph, xzs), n in cases:
+        src = qcf.Tableau(ph, n, xzs)
+        work = src.copy()
+
+        def step():
+            work.xzs.copy_(src.xzs)
+            work.phases.copy_(src.phases)
+            qcf.canonicalize_(work)
+
+        ms = time_launches(step, steps, warmup)
+        out[name] = dict(n=n, rows=len(ph), ms=ms)
+        log(f"  {name}: {out[name]}")
+    return out
+
+
 def cpu_baseline(target_s=12.0):
     """The oracle (C restatement, OpenMP over shots like the reference's
     Threads.@threads split) on the d=5 x 25 circuit with per-shot RNG draws."""
@@ -236,10 +263,20 @@ def cpu_baseline(target_s=12.0):
         oracle.mul_left_vec(a_, b_)
     dtm = (time.perf_counter() - t0) / reps
     mul = dict(n=n, ms=dtm * 1e3, gbs=6 * (n // 64) * 8 / dtm / 1e9, cores=1, kind="port")
+    # canonicalize! (single-threaded in the reference): the oracle on one core
+    canon = {}
+    for name, (ph, xzs), nq in (
+            ("stabilizer_1000", qcf.randoms.random_clifford_tableau(np.random.default_rng(0), 1000),
+             1000),
+            ("random_tableau_6000", qcf.randoms.random_tableau(np.random.default_rng(0), 6000, 6000),
+             6000)):
+        t0 = time.perf_counter()
+        oracle.canonicalize(xzs, ph, nq)
+        canon[name] = dict(ms=1e3 * (time.perf_counter() - t0), cores=1, kind="port")
     return dict(value=shots / dt, unit="shots/s", cores=cores, kind="port",
                 sample=f"{shots} shots of the d=5 x 25-round circuit (p=1e-3) in chunks of "
                        f"{chunk}, oracle restatement with OpenMP over {cores} threads, "
-                       f"{dt:.1f} s", mul_left=mul)
+                       f"{dt:.1f} s", mul_left=mul, canonicalize=canon)
 
 
 def bench_mlp(steps: int, eval_shots: int):
@@ -337,6 +374,11 @@ def main():
             extras["reference_circuits"] = bench_reference_circuits(5, 1)
         except Exception as e:  # reported, not hidden
             extras["reference_circuits"] = {"error": repr(e)}
+        log("canonicalize! ...")
+        try:
+            extras["canonicalize"] = bench_canonicalize(5, 1)
+        except Exception as e:  # reported, not hidden
+            extras["canonicalize"] = {"error": repr(e)}
         log("batched 512 circuits ...")
         try:
             extras["batched"] = bench_batched(dmax=a.batched_dmax)

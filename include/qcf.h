@@ -104,6 +104,22 @@ int qcf_mul_rows(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nwords,
                  size_t skip, const uint8_t *row_mask, int phases_on,
                  int right, void *stream);
 
+/*
+ * canonicalize!(state::Stabilizer; phases, ranks) (src/canonicalization.jl:
+ * 97-135) in place on a row-contiguous tableau (row m at xzs + m*ld, words
+ * [x 0..nwords) [z nwords..2*nwords), `nqubits` columns, `nrows` rows, phases
+ * one byte per row).  Replaces the CUDA extension's gpu_canonicalize!
+ * (ext/QuantumCliffordGPUExt/canonicalization.jl:116-166).  phases_on == 0
+ * leaves `phases` untouched (the reference then neither updates nor swaps
+ * them) and allows phases == NULL.  When `ranks` (device, 2 x uint32) is not
+ * NULL it receives the pivots of the X stage and of both stages (the
+ * reference's rx-1 and i-1 of ranks=true).  Stream-ordered; the workspace is
+ * library-owned (per device, grown on demand).  nrows <= 16384.
+ */
+int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nwords,
+                     size_t nrows, size_t nqubits, int phases_on, uint32_t *ranks,
+                     void *stream);
+
 /* ======================================================================= */
 /*  Pauli frames (src/pauli_frames.jl)                                      */
 /* ======================================================================= */

@@ -37,6 +37,7 @@ def lib():
         h.oracle_mul_left_pauli.argtypes = [_P, _P, _P, _P, _SZ, C.c_int]
         h.oracle_mul_right_pauli.argtypes = [_P, _P, _P, _P, _SZ, C.c_int]
         h.oracle_mul_rows.argtypes = [_P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ, _P, C.c_int, C.c_int]
+        h.oracle_canonicalize.argtypes = [_P, _P, _SZ, _SZ, _SZ, _SZ, C.c_int, _P]
         h.oracle_prodphase.argtypes = [_P, _P, _SZ]
         h.oracle_prodphase.restype = C.c_uint8
         h.oracle_comm.argtypes = [_P, _P, _SZ]
@@ -77,6 +78,16 @@ def mul_rows(xzs, phases, src, src_phase, skip=None, row_mask=None, phases_on=Tr
     lib().oracle_mul_rows(_p(xzs), _p(phases), ld, src.size // 2, nrows, _p(src), _p(src_phase),
                           2**64 - 1 if skip is None else skip, _p(row_mask), int(phases_on),
                           int(right))
+
+
+def canonicalize(xzs, phases, nqubits, phases_on=True):
+    """canonicalize! in place on xzs (nrows, ld) uint64 row-contiguous and
+    phases (nrows,) uint8; returns (X-stage pivots, all pivots)."""
+    nrows, ld = xzs.shape
+    ranks = np.zeros(2, np.uint64)
+    lib().oracle_canonicalize(_p(xzs), _p(phases), ld, ld // 2, nrows, nqubits,
+                              int(phases_on), _p(ranks))
+    return int(ranks[0]), int(ranks[1])
 
 
 def prodphase(l, r) -> int:

@@ -109,6 +109,52 @@ void oracle_mul_rows(uint64_t *xzs, uint8_t *phases, size_t ld, size_t len,
   }
 }
 
+/* canonicalize!(state::Stabilizer; phases, ranks) (src/canonicalization.jl:97-135)
+ * on a row-contiguous tableau (row m at xzs + m*ld, words [x 0..len); z
+ * len..2len)).  X stage: for every qubit column j, the first row k >= i with
+ * an X (or Y) at j is swapped into row i (rowswap!, src/QuantumClifford.jl:
+ * 899-905: the phases move only when phases are tracked), then every other row
+ * m with an X at j becomes row_i * row_m (mul_left!(s, m, i; phases),
+ * src/mul_leftright.jl:195-198); i advances.  Z stage: the same on Z bits,
+ * continuing from the X stage's i.  ranks[0] = pivots of the X stage,
+ * ranks[1] = all pivots (the reference's rx-1, i-1). */
+static inline int tab_bit(const uint64_t *row, size_t len, size_t j, int z) {
+  return (int)((row[(z ? len : 0) + j / 64] >> (j % 64)) & 1u);
+}
+static void tab_rowswap(uint64_t *xzs, uint8_t *phases, size_t ld, size_t len, size_t a,
+                        size_t b, int phases_on) {
+  if (a == b) return;
+  if (phases_on) {
+    const uint8_t t = phases[a];
+    phases[a] = phases[b];
+    phases[b] = t;
+  }
+  for (size_t w = 0; w < 2 * len; ++w) {
+    const uint64_t t = xzs[a * ld + w];
+    xzs[a * ld + w] = xzs[b * ld + w];
+    xzs[b * ld + w] = t;
+  }
+}
+void oracle_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_t len, size_t nrows,
+                         size_t nqubits, int phases_on, uint64_t ranks[2]) {
+  size_t i = 0;
+  for (int z = 0; z < 2; ++z) {
+    for (size_t j = 0; j < nqubits; ++j) {
+      size_t k = i;
+      while (k < nrows && !tab_bit(xzs + k * ld, len, j, z)) ++k;
+      if (k >= nrows) continue;
+      tab_rowswap(xzs, phases, ld, len, k, i, phases_on);
+      for (size_t m = 0; m < nrows; ++m) {
+        if (m == i || !tab_bit(xzs + m * ld, len, j, z)) continue;
+        const uint8_t s = oracle_mul_left_vec(xzs + m * ld, xzs + i * ld, len, phases_on);
+        if (phases_on) phases[m] = (uint8_t)((s + phases[m] + phases[i]) & 3u);
+      }
+      ++i;
+    }
+    if (ranks) ranks[z] = i;
+  }
+}
+
 /* prodphase(l, r) (src/QuantumClifford.jl:723-732): independent formula */
 uint8_t oracle_prodphase(const uint64_t *l, const uint64_t *r, size_t len) {
   int64_t res = 0;

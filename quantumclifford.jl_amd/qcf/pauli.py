@@ -176,6 +176,67 @@ class Tableau:
     def __len__(self):
         return self.nrows
 
+    def row_strings(self) -> list:
+        """Rows in the reference's text form, e.g. '+ XZ_Y'."""
+        xzs, ph = self.xzs_host(), self.phases_host()
+        W = self.nwords
+        out = []
+        for m in range(self.nrows):
+            xs = unpack_bits(xzs[m, :W], self.nqubits)
+            zs = unpack_bits(xzs[m, W:2 * W], self.nqubits)
+            out.append(PHASE_STR[int(ph[m]) & 3] +
+                       "".join("_XZY"[int(x) | (int(z) << 1)] for x, z in zip(xs, zs)))
+        return out
+
+    def __repr__(self):
+        return "\n".join(self.row_strings())
+
+    def copy(self) -> "Tableau":
+        t = Tableau.__new__(Tableau)
+        t.nrows, t.nqubits, t.ld = self.nrows, self.nqubits, self.ld
+        t.xzs, t.phases = self.xzs.clone(), self.phases.clone()
+        return t
+
+    def __eq__(self, other):
+        return (isinstance(other, Tableau) and self.nqubits == other.nqubits
+                and self.nrows == other.nrows
+                and np.array_equal(self.phases_host(), other.phases_host())
+                and np.array_equal(self.xzs_host()[:, :2 * self.nwords],
+                                   other.xzs_host()[:, :2 * other.nwords]))
+
+
+def S(text: str, device=None) -> Tableau:
+    """The S"..." string macro (src/stabilizer.jl): one Pauli per line, each with
+    an optional phase prefix (+, -, i, +i, -i) and letters _ I X Y Z."""
+    lines = [ln.strip() for ln in text.strip().splitlines() if ln.strip()]
+    rows = []
+    for ln in lines:
+        letters = [c for c in ln if c in "_IZXY"]
+        phase = _PHASEDICT["".join(c for c in ln if c not in "_IZXY").replace(" ", "")]
+        rows.append((phase, letters))
+    n = len(rows[0][1])
+    if any(len(r[1]) != n for r in rows):
+        raise DimensionMismatch("all rows of S\"...\" need the same number of qubits")
+    W = words_for(n)
+    xzs = np.zeros((len(rows), 2 * W), np.uint64)
+    for m, (_, letters) in enumerate(rows):
+        xzs[m, :W] = pack_bits([c in "XY" for c in letters], W)
+        xzs[m, W:] = pack_bits([c in "ZY" for c in letters], W)
+    return Tableau([r[0] for r in rows], n, xzs, device)
+
+
+def canonicalize_(s: Tableau, phases: bool = True, ranks: bool = False):
+    """canonicalize!(state; phases, ranks) (src/canonicalization.jl:97-135) in
+    place on the device.  With ranks=True returns (s, rx, rz) like the
+    reference's (state, rx-1, i-1): pivots of the X stage and of both stages."""
+    out = torch.zeros(2, dtype=torch.int32, device=s.xzs.device) if ranks else None
+    call("qcf_canonicalize", _ptr(s.xzs), _ptr(s.phases), s.ld, s.nwords, s.nrows, s.nqubits,
+         int(bool(phases)), _ptr(out) if ranks else None, _stream())
+    if ranks:
+        rx, rz = (int(v) for v in out.cpu().tolist())
+        return s, rx, rz
+    return s
+
 
 def mul_left_row_(s: Tableau, m: int, i: int, phases: bool = True) -> Tableau:
     """mul_left!(s::Tableau, m, i): row m := row i * row m (src/mul_leftright.jl:177-181).

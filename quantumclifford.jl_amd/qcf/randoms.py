@@ -56,3 +56,17 @@ def random_clifford_tableau(rng: np.random.Generator, n: int, ngates: int | None
     px = np.packbits(xs, axis=1, bitorder="little").view(np.uint64)
     pz = np.packbits(zs, axis=1, bitorder="little").view(np.uint64)
     return ph.astype(np.uint8), np.ascontiguousarray(np.concatenate([px, pz], axis=1))
+
+
+def random_tableau(rng: np.random.Generator, nrows: int, n: int):
+    """random_tableau(nrows, n) (src/randoms.jl:94-105 with the default
+    nophase=true): i.i.d. Bernoulli(1/2) x/z bits, pad bits cleared, phases 0.
+    Rows need not commute.  Returns (phases uint8[nrows], xzs uint64[nrows, 2W])."""
+    w = (n + 63) // 64
+    xzs = rng.integers(0, 2**64, size=(nrows, 2 * w), dtype=np.uint64)
+    r = n % 64
+    if r:
+        m = np.uint64((1 << r) - 1)
+        xzs[:, w - 1] &= m
+        xzs[:, 2 * w - 1] &= m
+    return np.zeros(nrows, np.uint8), xzs
