@@ -43,7 +43,8 @@ struct CanonState {
   uint32_t i;     // pivots so far (the next logical pivot row)
   uint32_t npiv;  // pivots of the current panel
   uint32_t rx;    // pivots of the X stage
-  uint32_t pad;
+  uint32_t done;  // snapshot workgroups finished (reset by the last one)
+  uint64_t ph_lo, ph_hi;  // bit k: low / high bit of snapshot k's phase
 };
 
 constexpr int kFactorThreads = 1024;
@@ -64,28 +65,37 @@ __device__ __forceinline__ void mul_word_c(uint64_t x1, uint64_t z1, uint64_t &x
 __global__ void canon_init(uint32_t *inv, uint32_t nrows, CanonState *st) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < nrows) inv[s] = s;
-  if (s == 0) *st = CanonState{0, 0, 0, 0};
+  if (s == 0) *st = CanonState{0, 0, 0, 0, 0, 0};
 }
 
-__device__ __forceinline__ unsigned long long wave_min_u64(unsigned long long v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(v, off, 64);
-    v = o < v ? o : v;
-  }
-  return v;
+// Minimum over the 64 lanes with DPP moves (an inclusive min-scan whose last
+// lane holds the total: row shifts 1, 2, 4, 8, then the row broadcasts).
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+  const int inf = (int)UINT_MAX;
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(inf, (int)v, 0x111, 0xf, 0xf, false));
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(inf, (int)v, 0x112, 0xf, 0xf, false));
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(inf, (int)v, 0x114, 0xf, 0xf, false));
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(inf, (int)v, 0x118, 0xf, 0xf, false));
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(inf, (int)v, 0x142, 0xa, 0xf, false));
+  v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(inf, (int)v, 0x143, 0xc, 0xf, false));
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // One panel: word `c` of every row (x word w, or z word W+w), `nb` valid bits.
-// Slot s (physical row) belongs to thread s % 1024, register j = s / 1024.
+// Slot s (physical row) belongs to thread s % 1024, register j = s / 1024; the
+// panel words also live in LDS (s_word[s], dynamic) so that every thread can
+// read the pivot's word after the step's one barrier.  Keys are
+// (logical row << 14) | slot, so nrows <= 16384.  The step's shared values are
+// double-buffered by step parity: a buffer is rewritten two steps later, after
+// a barrier that every reader of it has passed.
 template <int J>
 __global__ __launch_bounds__(kFactorThreads) void canon_factor(
     const uint64_t *__restrict__ xzs, size_t ld, uint32_t nrows, uint32_t c, uint32_t nb,
     uint32_t *__restrict__ inv, uint64_t *__restrict__ mask, uint32_t *__restrict__ piv,
-    CanonState *st, int mark_rx) {
-  __shared__ unsigned long long s_part[kFactorThreads / 64];
-  __shared__ uint32_t s_si;
-  __shared__ unsigned long long s_pw;
+    CanonState *st, uint32_t *__restrict__ cnt, int mark_rx) {
+  extern __shared__ uint64_t s_word[];
+  __shared__ uint32_t s_part[2][kFactorThreads / 64];
+  __shared__ uint32_t s_si[2];
   const uint32_t t = threadIdx.x;
   uint64_t word[J], m[J];
   uint32_t lid[J];
@@ -96,35 +106,33 @@ __global__ __launch_bounds__(kFactorThreads) void canon_factor(
     word[j] = ok ? xzs[(size_t)s * ld + c] : 0ull;
     lid[j] = ok ? inv[s] : UINT_MAX;
     m[j] = 0;
+    if (ok) s_word[s] = word[j];
   }
+  if (t < 64) cnt[t] = 0;  // per-pivot phase counters of this panel's snapshot pass
   const uint32_t i0 = st->i;
   uint32_t i = i0, k = 0;
+  __syncthreads();
   for (uint32_t b = 0; b < nb && i < nrows; ++b) {
-    // the first logical row >= i with bit b set: minimum of (logical, slot)
-    unsigned long long key = ULLONG_MAX;
+    const uint32_t par = b & 1u;
+    // the first logical row >= i with bit b set
+    uint32_t key = UINT_MAX;
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-      const unsigned long long kk = ((unsigned long long)lid[j] << 32) | (t + j * kFactorThreads);
+      const uint32_t s = t + j * kFactorThreads;
+      const uint32_t kk = (lid[j] << 14) | s;
       if (((word[j] >> b) & 1ull) && lid[j] >= i && kk < key) key = kk;
-      if (lid[j] == i) s_si = t + j * kFactorThreads;
+      if (lid[j] == i) s_si[par] = s;
     }
-    key = wave_min_u64(key);
-    if ((t & 63) == 0) s_part[t >> 6] = key;
+    key = wave_min_u32(key);
+    if ((t & 63) == 0) s_part[par][t >> 6] = key;
     __syncthreads();
-    unsigned long long best = s_part[0];
+    uint32_t best = UINT_MAX;
 #pragma unroll
-    for (int w = 1; w < kFactorThreads / 64; ++w) best = s_part[w] < best ? s_part[w] : best;
-    const uint32_t si = s_si;
-    if (best == ULLONG_MAX) {  // no pivot in this column
-      __syncthreads();
-      continue;
-    }
-    const uint32_t ps = (uint32_t)best, pl = (uint32_t)(best >> 32);
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-      if (t + j * kFactorThreads == ps) s_pw = word[j];
-    __syncthreads();
-    const uint64_t pw = s_pw;
+    for (int w = 0; w < kFactorThreads / 64; ++w) best = min(best, s_part[par][w]);
+    if (best == UINT_MAX) continue;  // no pivot in this column (uniform)
+    const uint32_t ps = best & 0x3FFFu, pl = best >> 14;
+    const uint32_t si = s_si[par];
+    const uint64_t pw = s_word[ps];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
       const uint32_t s = t + j * kFactorThreads;
@@ -134,6 +142,7 @@ __global__ __launch_bounds__(kFactorThreads) void canon_factor(
         if (s == si) lid[j] = pl;  // ... and row i to the pivot's old place
         if ((word[j] >> b) & 1ull) {
           word[j] ^= pw;
+          s_word[s] = word[j];
           m[j] |= 1ull << k;
         }
       }
@@ -159,52 +168,95 @@ __global__ __launch_bounds__(kFactorThreads) void canon_factor(
 }
 
 // Snapshots: pivot k's full row at its step = original row times the earlier
-// pivots (k' < k) of its set, in order.  One thread per word position.
-__global__ __launch_bounds__(1024) void canon_snap(
+// pivots (k' < k) of its set, in order.  One lane per word position, 64 words
+// per workgroup; the lane's column of the 64 snapshots stays in LDS
+// (s_x/s_z[k*64 + lane], 64 KB), loads batched eight at a time ahead of the
+// dependent products.  Per-pivot phase counts go to cnt[] (zeroed by the
+// factor kernel); the last workgroup to finish resolves the snapshot phases.
+__global__ __launch_bounds__(64) void canon_snap(
     const uint64_t *__restrict__ xzs, const uint8_t *__restrict__ phases, size_t ld, size_t W,
-    const uint64_t *__restrict__ mask, const uint32_t *__restrict__ piv, const CanonState *st,
-    uint64_t *__restrict__ snap, uint32_t *__restrict__ snap_ph, int phases_on) {
-  __shared__ uint32_t cnt[64];
+    const uint64_t *__restrict__ mask, const uint32_t *__restrict__ piv, CanonState *st,
+    uint64_t *__restrict__ snap, uint32_t *__restrict__ cnt, int phases_on) {
+  extern __shared__ uint64_t s_snap[];  // x: [0, 4096), z: [4096, 8192)
+  __shared__ uint64_t l_mask[64];
+  __shared__ uint32_t l_piv[64];
+  __shared__ uint32_t l_cnt[64];
+  __shared__ uint32_t l_last;
   const uint32_t npiv = st->npiv;
   if (npiv == 0) return;  // uniform
   const uint32_t t = threadIdx.x;
-  if (t < 64) cnt[t] = 0;
-  __syncthreads();
-  const size_t row = 2 * W;
-  for (size_t w = t; w < W; w += blockDim.x) {
-    for (uint32_t k = 0; k < npiv; ++k) {
-      const uint32_t ps = piv[k];
-      uint64_t x = xzs[(size_t)ps * ld + w], z = xzs[(size_t)ps * ld + W + w];
-      uint64_t early = mask[ps] & ((1ull << k) - 1ull);
-      uint64_t c1 = 0, c2 = 0;
-      while (early) {
-        const int k2 = __ffsll((unsigned long long)early) - 1;
-        early &= early - 1ull;
-        mul_word_c(snap[k2 * row + w], snap[k2 * row + W + w], x, z, c1, c2);
-      }
-      snap[k * row + w] = x;
-      snap[k * row + W + w] = z;
-      if (phases_on) {
-        const uint32_t f = (__popcll(c1) + 2u * __popcll(c2)) & 3u;
-        if (f) atomicAdd(&cnt[k], f);
-      }
-    }
+  uint64_t *s_x = s_snap, *s_z = s_snap + 64 * 64;
+  l_cnt[t] = 0;
+  if (t < npiv) {
+    l_piv[t] = piv[t];
+    l_mask[t] = mask[piv[t]];
   }
   __syncthreads();
-  if (t == 0 && phases_on) {
-    uint32_t ph[64];
-    for (uint32_t k = 0; k < npiv; ++k) {
-      const uint32_t ps = piv[k];
-      uint32_t v = phases[ps] + cnt[k];
-      uint64_t early = mask[ps] & ((1ull << k) - 1ull);
-      while (early) {
-        const int k2 = __ffsll((unsigned long long)early) - 1;
-        early &= early - 1ull;
-        v += ph[k2];
+  const size_t w = (size_t)blockIdx.x * 64 + t;
+  const bool ok = w < W;
+  const size_t row = 2 * W;
+  // the original pivot rows first: all loads in flight at once
+  for (uint32_t k = 0; k < npiv; ++k) {
+    const size_t ps = l_piv[k];
+    s_x[k * 64 + t] = ok ? xzs[ps * ld + w] : 0ull;
+    s_z[k * 64 + t] = ok ? xzs[ps * ld + W + w] : 0ull;
+  }
+  for (uint32_t k = 0; k < npiv; ++k) {
+    uint64_t x = s_x[k * 64 + t], z = s_z[k * 64 + t];
+    uint64_t early = l_mask[k] & ((1ull << k) - 1ull);
+    uint64_t c1 = 0, c2 = 0;
+    while (early) {
+      uint64_t ax[8], az[8];
+      int n = 0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (early) {
+          const int k2 = __ffsll((unsigned long long)early) - 1;
+          early &= early - 1ull;
+          ax[q] = s_x[k2 * 64 + t];
+          az[q] = s_z[k2 * 64 + t];
+          n = q + 1;
+        }
       }
-      ph[k] = v & 3u;
-      snap_ph[k] = ph[k];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < n) mul_word_c(ax[q], az[q], x, z, c1, c2);
     }
+    s_x[k * 64 + t] = x;
+    s_z[k * 64 + t] = z;
+    if (ok) {
+      snap[k * row + w] = x;
+      snap[k * row + W + w] = z;
+    }
+    if (phases_on) {
+      const uint32_t f = (__popcll(c1) + 2u * __popcll(c2)) & 3u;
+      if (f) atomicAdd(&l_cnt[k], f);
+    }
+  }
+  if (!phases_on) return;  // uniform
+  __syncthreads();
+  if (t < npiv && l_cnt[t]) atomicAdd(&cnt[t], l_cnt[t]);
+  __threadfence();
+  __syncthreads();
+  if (t == 0) l_last = atomicAdd(&st->done, 1u) == gridDim.x - 1 ? 1u : 0u;
+  __syncthreads();
+  if (!l_last) return;  // uniform
+  __threadfence();
+  // snapshot phases: phases[p_k] + its product counters + the phases of the
+  // earlier snapshots in its set, as two bit masks (low and high phase bits)
+  if (t < npiv) l_cnt[t] = (uint32_t)phases[l_piv[t]] + atomicAdd(&cnt[t], 0u);
+  __syncthreads();
+  if (t == 0) {
+    uint64_t lo = 0, hi = 0;
+    for (uint32_t k = 0; k < npiv; ++k) {
+      const uint64_t early = l_mask[k] & ((1ull << k) - 1ull);
+      const uint32_t v = (l_cnt[k] + __popcll(early & lo) + 2u * __popcll(early & hi)) & 3u;
+      lo |= (uint64_t)(v & 1u) << k;
+      hi |= (uint64_t)(v >> 1) << k;
+    }
+    st->ph_lo = lo;
+    st->ph_hi = hi;
+    atomicExch(&st->done, 0u);
   }
 }
 
@@ -214,8 +266,9 @@ template <int G>
 __global__ __launch_bounds__(256) void canon_apply(
     uint64_t *__restrict__ xzs, uint8_t *__restrict__ phases, size_t ld, size_t W,
     uint32_t nrows, const uint64_t *__restrict__ mask, const uint64_t *__restrict__ snap,
-    const uint32_t *__restrict__ snap_ph, const CanonState *st, int phases_on) {
+    const CanonState *st, int phases_on) {
   if (st->npiv == 0) return;  // uniform
+  const uint64_t ph_lo = st->ph_lo, ph_hi = st->ph_hi;
   constexpr uint32_t kRowsPerBlock = 256 / G;
   const uint32_t lane = threadIdx.x % G;
   const size_t row = 2 * W;
@@ -228,12 +281,12 @@ __global__ __launch_bounds__(256) void canon_apply(
       for (size_t w = lane; w < W; w += G) {
         uint64_t x = xr[w], z = xr[W + w];
         uint64_t mm = m;
-        // snapshot words of up to four pivots are loaded before they are used
+        // snapshot words of up to eight pivots are loaded before they are used
         while (mm) {
-          uint64_t sx[4], sz[4];
+          uint64_t sx[8], sz[8];
           int n = 0;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < 8; ++q) {
             if (mm) {
               const int k = __ffsll((unsigned long long)mm) - 1;
               mm &= mm - 1ull;
@@ -243,7 +296,7 @@ __global__ __launch_bounds__(256) void canon_apply(
             }
           }
 #pragma unroll
-          for (int q = 0; q < 4; ++q)
+          for (int q = 0; q < 8; ++q)
             if (q < n) mul_word_c(sx[q], sz[q], x, z, c1, c2);
         }
         xr[w] = x;
@@ -255,13 +308,7 @@ __global__ __launch_bounds__(256) void canon_apply(
 #pragma unroll
       for (int off = G / 2; off > 0; off >>= 1) f += __shfl_xor(f, off, G);
       if (lane == 0 && m) {
-        uint32_t v = phases[r] + f;
-        uint64_t mm = m;
-        while (mm) {
-          const int k = __ffsll((unsigned long long)mm) - 1;
-          mm &= mm - 1ull;
-          v += snap_ph[k];
-        }
+        const uint32_t v = phases[r] + f + __popcll(m & ph_lo) + 2u * __popcll(m & ph_hi);
         phases[r] = (uint8_t)(v & 3u);
       }
     }
@@ -338,20 +385,27 @@ int canon_scratch(size_t bytes, char **out) {
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+constexpr uint32_t kSnapLds = 2 * 64 * 64 * sizeof(uint64_t);  // 64 KB, the default limit
+
 template <int J>
-void launch_factor(const uint64_t *xzs, size_t ld, uint32_t nrows, uint32_t c, uint32_t nb,
-                   uint32_t *inv, uint64_t *mask, uint32_t *piv, CanonState *st, int mark_rx,
-                   hipStream_t stream) {
-  hipLaunchKernelGGL(canon_factor<J>, dim3(1), dim3(kFactorThreads), 0, stream, xzs, ld, nrows,
-                     c, nb, inv, mask, piv, st, mark_rx);
+int launch_factor(const uint64_t *xzs, size_t ld, uint32_t nrows, uint32_t c, uint32_t nb,
+                  uint32_t *inv, uint64_t *mask, uint32_t *piv, CanonState *st, uint32_t *cnt,
+                  int mark_rx, hipStream_t stream) {
+  const uint32_t lds = nrows * (uint32_t)sizeof(uint64_t);  // at most 128 KB
+  if (lds > 64u * 1024u)
+    QCF_HIP_CHECK(hipFuncSetAttribute((const void *)canon_factor<J>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(canon_factor<J>, dim3(1), dim3(kFactorThreads), lds, stream, xzs, ld, nrows,
+                     c, nb, inv, mask, piv, st, cnt, mark_rx);
+  return QCF_OK;
 }
 
 template <int G>
 void launch_apply(uint64_t *xzs, uint8_t *phases, size_t ld, size_t W, uint32_t nrows,
-                  const uint64_t *mask, const uint64_t *snap, const uint32_t *snap_ph,
-                  const CanonState *st, int phases_on, uint32_t grid, hipStream_t stream) {
+                  const uint64_t *mask, const uint64_t *snap, const CanonState *st,
+                  int phases_on, uint32_t grid, hipStream_t stream) {
   hipLaunchKernelGGL(canon_apply<G>, dim3(grid), dim3(256), 0, stream, xzs, phases, ld, W, nrows,
-                     mask, snap, snap_ph, st, phases_on);
+                     mask, snap, st, phases_on);
 }
 
 }  // namespace
@@ -383,7 +437,7 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
     return set_error(QCF_ERR_UNSUPPORTED, "canonicalize!: %zu rows exceed the limit of %u", nrows,
                      kMaxCanonRows);
   const size_t W = nwords;
-  // workspace: state | inv | mask | piv | snapshot phases | snapshots | gathered rows | phases
+  // workspace: state | inv | mask | piv | phase counters | snapshots | gathered rows | phases
   size_t off = 0;
   const size_t o_st = off;
   off = align256(off + sizeof(CanonState));
@@ -393,7 +447,7 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
   off = align256(off + nrows * sizeof(uint64_t));
   const size_t o_piv = off;
   off = align256(off + 64 * sizeof(uint32_t));
-  const size_t o_sph = off;
+  const size_t o_cnt = off;
   off = align256(off + 64 * sizeof(uint32_t));
   const size_t o_snap = off;
   off = align256(off + 64 * 2 * W * sizeof(uint64_t));
@@ -407,7 +461,7 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
   uint32_t *inv = (uint32_t *)(base + o_inv);
   uint64_t *mask = (uint64_t *)(base + o_mask);
   uint32_t *piv = (uint32_t *)(base + o_piv);
-  uint32_t *sph = (uint32_t *)(base + o_sph);
+  uint32_t *cnt = (uint32_t *)(base + o_cnt);
   uint64_t *snap = (uint64_t *)(base + o_snap);
   uint64_t *tmp = (uint64_t *)(base + o_tmp);
   uint8_t *tph = (uint8_t *)(base + o_tph);
@@ -421,7 +475,7 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
       std::max<uint32_t>(1, std::min<uint32_t>((n + rows_per_block - 1) / rows_per_block,
                                                (uint32_t)ds->cu_count * 8));
   const uint32_t J = (n + kFactorThreads - 1) / kFactorThreads;
-  const uint32_t snap_threads = (uint32_t)std::min<size_t>(1024, ((W + 63) / 64) * 64);
+  const uint32_t snap_grid = (uint32_t)((W + 63) / 64);
   for (int zstage = 0; zstage < 2; ++zstage) {
     for (size_t w = 0; w < W; ++w) {
       const size_t first = 64 * w;
@@ -429,30 +483,31 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
       const uint32_t nb = (uint32_t)std::min<size_t>(64, nqubits - first);
       const uint32_t c = (uint32_t)(zstage ? W + w : w);
       const int mark = (zstage == 1 && w == 0) ? 1 : 0;
-      if (J <= 1)
-        launch_factor<1>(xzs, ld, n, c, nb, inv, mask, piv, st, mark, stream);
-      else if (J <= 2)
-        launch_factor<2>(xzs, ld, n, c, nb, inv, mask, piv, st, mark, stream);
-      else if (J <= 4)
-        launch_factor<4>(xzs, ld, n, c, nb, inv, mask, piv, st, mark, stream);
-      else if (J <= 8)
-        launch_factor<8>(xzs, ld, n, c, nb, inv, mask, piv, st, mark, stream);
-      else
-        launch_factor<16>(xzs, ld, n, c, nb, inv, mask, piv, st, mark, stream);
-      hipLaunchKernelGGL(canon_snap, dim3(1), dim3(snap_threads), 0, stream, xzs, phases, ld, W,
-                         mask, piv, st, snap, sph, phases_on);
+#define QCF_FACTOR(JJ) \
+  QCF_TRY(launch_factor<JJ>(xzs, ld, n, c, nb, inv, mask, piv, st, cnt, mark, stream))
+      if (J <= 1) QCF_FACTOR(1);
+      else if (J <= 2) QCF_FACTOR(2);
+      else if (J <= 3) QCF_FACTOR(3);
+      else if (J <= 4) QCF_FACTOR(4);
+      else if (J <= 6) QCF_FACTOR(6);
+      else if (J <= 8) QCF_FACTOR(8);
+      else if (J <= 12) QCF_FACTOR(12);
+      else QCF_FACTOR(16);
+#undef QCF_FACTOR
+      hipLaunchKernelGGL(canon_snap, dim3(snap_grid), dim3(64), kSnapLds, stream, xzs, phases, ld,
+                         W, mask, piv, st, snap, cnt, phases_on);
       switch (G) {
         case 64:
-          launch_apply<64>(xzs, phases, ld, W, n, mask, snap, sph, st, phases_on, apply_grid, stream);
+          launch_apply<64>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream);
           break;
         case 32:
-          launch_apply<32>(xzs, phases, ld, W, n, mask, snap, sph, st, phases_on, apply_grid, stream);
+          launch_apply<32>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream);
           break;
         case 16:
-          launch_apply<16>(xzs, phases, ld, W, n, mask, snap, sph, st, phases_on, apply_grid, stream);
+          launch_apply<16>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream);
           break;
         default:
-          launch_apply<8>(xzs, phases, ld, W, n, mask, snap, sph, st, phases_on, apply_grid, stream);
+          launch_apply<8>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream);
       }
     }
   }
