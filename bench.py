@@ -344,6 +344,9 @@ def main():
     ap.add_argument("--batched-dmax", type=int, default=15)
     ap.add_argument("--mlp-steps", type=int, default=24414)  # 2e8 shots of 8192
     ap.add_argument("--mlp-eval-shots", type=int, default=10**7)
+    ap.add_argument("--only", choices=("reference_circuits", "canonicalize", "mul_config",
+                                       "batched"),
+                    help="run one extra alone and print its JSON (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the frame sampler shards "
@@ -352,6 +355,13 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if a.only:
+        fn = {"reference_circuits": lambda: bench_reference_circuits(5, 1),
+              "canonicalize": lambda: bench_canonicalize(5, 1),
+              "mul_config": lambda: bench_mul_config(10, 2),
+              "batched": lambda: bench_batched(dmax=a.batched_dmax)}[a.only]
+        print(json.dumps({a.only: fn()}), flush=True)
+        return
     log("frames: d=5 x 25 rounds, p=1e-3 ...")
     fr = bench_frames(a.steps, a.warmup, a.shots)
     log(f"  {fr['ms']:.3f} ms/step, {fr['shots_s']:.3e} shots/s")

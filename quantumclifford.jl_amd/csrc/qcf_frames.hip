@@ -67,24 +67,36 @@ __device__ __forceinline__ void export_flip(const KParams &P, uint32_t plane,
   atomicXor(&P.masks[(uint64_t)plane * P.ld32 + gword], bit);
 }
 
-// Plane storage of a tile: LDS (the fast path) or, for circuits whose frame
-// does not fit on chip, a per-wave region of global memory.  In global memory
-// every access is an agent-scope atomic so that bit flips written by other
-// lanes of the wave (noise events) are never hidden by a stale L1 line.
-template <bool GL>
+// Plane storage of a tile: LDS (kPlLds, the fast path) or, for circuits whose
+// frame does not fit on chip, a per-wave region of global memory (kPlAtomic).
+// In global memory every access is an agent-scope atomic: bit flips that
+// other lanes of the wave write (noise events) are never hidden by a stale L1
+// line, and an XOR is one fire-and-forget atomic at L2.  (Plain cached
+// accesses, valid for programs without device-drawn events, measured slower:
+// d=11 x 12 rounds, 2^19 shots, 12.5 vs 7.3 ms -- the XOR becomes a load on
+// the critical path.  Narrow tiles -- fewer lanes per wave, more waves for
+// launches with few tiles -- measured slower too: 7.3 ms at 64 lanes, 9.4 /
+// 9.8 / 10.6 ms at 32 / 16 / 8.)
+constexpr int kPlLds = 0, kPlAtomic = 1;
+template <int GL>
 struct Planes {
   uint32_t *base;
   __device__ __forceinline__ uint32_t rd(uint32_t p, uint32_t l) const {
-    if (GL) return __hip_atomic_load(base + p * 64u + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (GL == kPlAtomic)
+      return __hip_atomic_load(base + p * 64u + l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return base[p * 64u + l];
   }
   __device__ __forceinline__ void wr(uint32_t p, uint32_t l, uint32_t v) const {
-    if (GL) __hip_atomic_store(base + p * 64u + l, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else base[p * 64u + l] = v;
+    if (GL == kPlAtomic)
+      __hip_atomic_store(base + p * 64u + l, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      base[p * 64u + l] = v;
   }
   __device__ __forceinline__ void xr(uint32_t p, uint32_t l, uint32_t v) const {
-    if (GL) __hip_atomic_fetch_xor(base + p * 64u + l, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else atomicXor(base + p * 64u + l, v);
+    if (GL == kPlAtomic)
+      __hip_atomic_fetch_xor(base + p * 64u + l, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      atomicXor(base + p * 64u + l, v);
   }
 };
 
@@ -93,7 +105,7 @@ struct Planes {
 #define QCF_WAIT_VMEM() __builtin_amdgcn_s_waitcnt(0x0F70)
 
 // Run the whole program on one 2048-shot tile.
-template <int MODE, bool GL>
+template <int MODE, int GL>
 __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
                                          const Planes<GL> &F, uint32_t lane) {
   const uint4 *__restrict__ prog4 = P.prog;
@@ -273,12 +285,12 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
   }
 }
 
-template <int MODE, bool GL>
+template <int MODE, int GL>
 __global__ __launch_bounds__(64) void pf_kernel(KParams P) {
   extern __shared__ uint32_t lds[];
   const uint32_t lane = threadIdx.x;
   Planes<GL> F;
-  F.base = GL ? P.scratch + (uint64_t)blockIdx.x * P.sstride : lds;
+  F.base = GL != kPlLds ? P.scratch + (uint64_t)blockIdx.x * P.sstride : lds;
   for (uint32_t blk = P.blk0 + blockIdx.x; blk < P.nblocks; blk += gridDim.x)
     run_tile<MODE, GL>(P, blk, F, lane);
 }
@@ -304,11 +316,11 @@ __global__ __launch_bounds__(64) void pf_batch_kernel(const KParams *params,
     const WorkItem w = items[it];
     const KParams P = params[w.circuit];
     if (P.scratch) {
-      Planes<true> F{P.scratch + (uint64_t)blockIdx.x * P.sstride};
-      for (uint32_t b = 0; b < w.nblk; ++b) run_tile<kNative, true>(P, w.blk0 + b, F, lane);
+      Planes<kPlAtomic> F{P.scratch + (uint64_t)blockIdx.x * P.sstride};
+      for (uint32_t b = 0; b < w.nblk; ++b) run_tile<kNative, kPlAtomic>(P, w.blk0 + b, F, lane);
     } else {
-      Planes<false> F{lds};
-      for (uint32_t b = 0; b < w.nblk; ++b) run_tile<kNative, false>(P, w.blk0 + b, F, lane);
+      Planes<kPlLds> F{lds};
+      for (uint32_t b = 0; b < w.nblk; ++b) run_tile<kNative, kPlLds>(P, w.blk0 + b, F, lane);
     }
   }
   if (lane == 0) {
@@ -447,10 +459,10 @@ int allow_lds(const void *fn, uint32_t lds_bytes) {
 template <int MODE>
 int launch_pf(const KParams &kp, uint32_t lds_bytes, uint32_t grid, hipStream_t s) {
   if (kp.scratch) {
-    hipLaunchKernelGGL((pf_kernel<MODE, true>), dim3(grid), dim3(64), 0, s, kp);
+    hipLaunchKernelGGL((pf_kernel<MODE, kPlAtomic>), dim3(grid), dim3(64), 0, s, kp);
   } else {
-    QCF_TRY(allow_lds((const void *)pf_kernel<MODE, false>, lds_bytes));
-    hipLaunchKernelGGL((pf_kernel<MODE, false>), dim3(grid), dim3(64), lds_bytes, s, kp);
+    QCF_TRY(allow_lds((const void *)pf_kernel<MODE, kPlLds>, lds_bytes));
+    hipLaunchKernelGGL((pf_kernel<MODE, kPlLds>), dim3(grid), dim3(64), lds_bytes, s, kp);
   }
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
@@ -458,9 +470,11 @@ int launch_pf(const KParams &kp, uint32_t lds_bytes, uint32_t grid, hipStream_t 
 
 // Frames too large for LDS live in a per-wave region of this device buffer
 // (grown on demand; only single-circuit launches use it, batches own theirs).
-// (d=11 unrotated surface code, 884 planes, 10^7 shots: 8 -> 16 waves per CU
-// took 36.0 -> 29.5 ms; the waves wait on L2/MALL latency, so more of them help)
-constexpr uint32_t kGlobalWavesPerCU = 16;
+// (d=11 unrotated surface code, 884 planes, 10^7 shots: 8, 16, 24, 32 waves
+// per CU took 36.0, 29.4, 24.0, 23.9 ms; the waves wait on L2/MALL latency,
+// so more of them help until occupancy stops them)
+constexpr uint32_t kGlobalWavesPerCU = 24;
+constexpr uint32_t kMaxBatchWavesPerCU = 32;
 constexpr uint32_t kGlobalJitWavesPerCU = 16;  // specialised kernels, planes in global memory
 struct FrameScratch {
   void *ptr = nullptr;
@@ -794,7 +808,7 @@ int set_tune(qcf_batch *B, const qcf_tune *tune) {
   uint32_t tb = tune ? tune->tile_blocks : 0;
   const uint32_t lds_cap = std::max<uint32_t>(1, (160u * 1024u) / std::max(1u, B->lds_bytes));
   if (wpc == 0) wpc = std::min<uint32_t>(16, lds_cap);
-  wpc = std::max<uint32_t>(1, std::min(wpc, lds_cap));
+  wpc = std::max<uint32_t>(1, std::min(std::min(wpc, lds_cap), kMaxBatchWavesPerCU));
   if (tb == 0) tb = 1;
   make_items(B, tb);
   QCF_TRY(upload_items(B));
@@ -871,7 +885,7 @@ extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
       delete B;
       return r;
     }
-    const size_t slots = (size_t)st->cu_count * 16;  // the largest grid set_tune picks
+    const size_t slots = (size_t)st->cu_count * kMaxBatchWavesPerCU;  // the largest grid
     if (hipMalloc(&B->d_frames, slots * gl_planes * 256u) != hipSuccess) {
       delete B;
       return set_error(QCF_ERR_ALLOC, "batch frame buffer (%zu bytes)", slots * gl_planes * 256u);
@@ -977,6 +991,9 @@ extern "C" int qcf_autotune(const qcf_program *const *progs, size_t n,
   // wave, more resident waves; the grid takes as many waves as LDS allows
   const uint32_t caps[] = {0u, 72u * 1024u, 36u * 1024u, 14u * 1024u};
   const uint32_t tbs[] = {1, 4};
+  // waves per CU: the default (16 or what LDS allows), or the most that
+  // small on-chip frames allow (waves on global frames wait on L2 latency)
+  const uint32_t wpcs[] = {0u, kMaxBatchWavesPerCU};
   for (uint32_t cap : caps) {
     if (st != QCF_OK) break;
     qcf_batch *B = nullptr;
@@ -984,8 +1001,9 @@ extern "C" int qcf_autotune(const qcf_program *const *progs, size_t n,
     if ((st = qcf_batch_create(progs, n, shots, seeds.data(), bufs.data(), lds.data(), &t0, &B)) !=
         QCF_OK)
       break;
-    for (uint32_t tb : tbs) {
-      qcf_tune t{0, tb, cap};
+    for (uint32_t k = 0; k < 4 && st == QCF_OK; ++k) {
+      const uint32_t tb = tbs[k & 1], wpc = wpcs[k >> 1];
+      qcf_tune t{wpc, tb, cap};
       if ((st = qcf::set_tune(B, &t)) != QCF_OK) break;
       if ((st = qcf::batch_launch(B, s)) != QCF_OK) break;  // warm
       if (hipEventRecord(e0, s) != hipSuccess) { st = set_error(QCF_ERR_HIP, "event"); break; }
