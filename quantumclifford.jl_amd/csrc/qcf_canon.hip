@@ -169,11 +169,40 @@ __global__ __launch_bounds__(kFactorThreads) void canon_factor(
 
 // Snapshots: pivot k's full row at its step = original row times the earlier
 // pivots (k' < k) of its set, in order.  One lane per word position, 64 words
-// per workgroup; the lane's column of the 64 snapshots stays in LDS
-// (s_x/s_z[k*64 + lane], 64 KB), loads batched eight at a time ahead of the
-// dependent products.  Per-pivot phase counts go to cnt[] (zeroed by the
-// factor kernel); the last workgroup to finish resolves the snapshot phases.
-__global__ __launch_bounds__(64) void canon_snap(
+// per workgroup of 8 waves; the 64 snapshots of a lane's word stay in LDS
+// (s_x/s_z[k*64 + lane], 64 KB).  The pivots go in 8 groups of 8: for group
+// g, wave j multiplies pivot 8g+j by the snapshots of earlier groups in its
+// set (all final, so the 8 rows proceed in parallel), then wave 0 finishes
+// the group's rows with the products inside the group, in pivot order.  That
+// is each row's products in increasing pivot order, as the reference applies
+// them, on a critical path of about a quarter of the products.  Per-pivot
+// phase counts go to cnt[] (zeroed by the factor kernel); the last workgroup
+// to finish resolves the snapshot phases.
+constexpr uint32_t kSnapWaves = 8;
+
+__device__ __forceinline__ void snap_products(const uint64_t *s_x, const uint64_t *s_z,
+                                              uint32_t t, uint64_t early, uint64_t &x,
+                                              uint64_t &z, uint64_t &c1, uint64_t &c2) {
+  while (early) {
+    uint64_t ax[8], az[8];
+    int n = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (early) {
+        const int k2 = __ffsll((unsigned long long)early) - 1;
+        early &= early - 1ull;
+        ax[q] = s_x[k2 * 64 + t];
+        az[q] = s_z[k2 * 64 + t];
+        n = q + 1;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < n) mul_word_c(ax[q], az[q], x, z, c1, c2);
+  }
+}
+
+__global__ __launch_bounds__(64 * kSnapWaves) void canon_snap(
     const uint64_t *__restrict__ xzs, const uint8_t *__restrict__ phases, size_t ld, size_t W,
     const uint64_t *__restrict__ mask, const uint32_t *__restrict__ piv, CanonState *st,
     uint64_t *__restrict__ snap, uint32_t *__restrict__ cnt, int phases_on) {
@@ -184,69 +213,72 @@ __global__ __launch_bounds__(64) void canon_snap(
   __shared__ uint32_t l_last;
   const uint32_t npiv = st->npiv;
   if (npiv == 0) return;  // uniform
-  const uint32_t t = threadIdx.x;
+  const uint32_t t = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   uint64_t *s_x = s_snap, *s_z = s_snap + 64 * 64;
-  l_cnt[t] = 0;
-  if (t < npiv) {
-    l_piv[t] = piv[t];
-    l_mask[t] = mask[piv[t]];
+  if (threadIdx.x < 64) {
+    l_cnt[t] = 0;
+    if (t < npiv) {
+      l_piv[t] = piv[t];
+      l_mask[t] = mask[piv[t]];
+    }
   }
   __syncthreads();
   const size_t w = (size_t)blockIdx.x * 64 + t;
   const bool ok = w < W;
   const size_t row = 2 * W;
-  // the original pivot rows first: all loads in flight at once
-  for (uint32_t k = 0; k < npiv; ++k) {
+  // the original pivot rows: all loads in flight at once
+  for (uint32_t k = wave; k < npiv; k += kSnapWaves) {
     const size_t ps = l_piv[k];
     s_x[k * 64 + t] = ok ? xzs[ps * ld + w] : 0ull;
     s_z[k * 64 + t] = ok ? xzs[ps * ld + W + w] : 0ull;
   }
-  for (uint32_t k = 0; k < npiv; ++k) {
-    uint64_t x = s_x[k * 64 + t], z = s_z[k * 64 + t];
-    uint64_t early = l_mask[k] & ((1ull << k) - 1ull);
-    uint64_t c1 = 0, c2 = 0;
-    while (early) {
-      uint64_t ax[8], az[8];
-      int n = 0;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        if (early) {
-          const int k2 = __ffsll((unsigned long long)early) - 1;
-          early &= early - 1ull;
-          ax[q] = s_x[k2 * 64 + t];
-          az[q] = s_z[k2 * 64 + t];
-          n = q + 1;
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (q < n) mul_word_c(ax[q], az[q], x, z, c1, c2);
-    }
-    s_x[k * 64 + t] = x;
-    s_z[k * 64 + t] = z;
-    if (ok) {
-      snap[k * row + w] = x;
-      snap[k * row + W + w] = z;
-    }
-    if (phases_on) {
-      const uint32_t f = (__popcll(c1) + 2u * __popcll(c2)) & 3u;
-      if (f) atomicAdd(&l_cnt[k], f);
-    }
-  }
-  if (!phases_on) return;  // uniform
   __syncthreads();
-  if (t < npiv && l_cnt[t]) atomicAdd(&cnt[t], l_cnt[t]);
+  for (uint32_t g0 = 0; g0 < npiv; g0 += kSnapWaves) {
+    const uint64_t before = (1ull << g0) - 1ull;  // pivots of earlier groups
+    const uint32_t k = g0 + wave;
+    if (k < npiv) {
+      uint64_t x = s_x[k * 64 + t], z = s_z[k * 64 + t], c1 = 0, c2 = 0;
+      snap_products(s_x, s_z, t, l_mask[k] & before, x, z, c1, c2);
+      s_x[k * 64 + t] = x;
+      s_z[k * 64 + t] = z;
+      const uint32_t f = (__popcll(c1) + 2u * __popcll(c2)) & 3u;
+      if (phases_on && f) atomicAdd(&l_cnt[k], f);
+    }
+    __syncthreads();
+    if (wave == 0) {
+      const uint32_t g1 = min(npiv, g0 + kSnapWaves);
+      for (uint32_t kk = g0 + 1; kk < g1; ++kk) {
+        const uint64_t inside = l_mask[kk] & ((1ull << kk) - 1ull) & ~before;
+        if (!inside) continue;  // uniform
+        uint64_t x = s_x[kk * 64 + t], z = s_z[kk * 64 + t], c1 = 0, c2 = 0;
+        snap_products(s_x, s_z, t, inside, x, z, c1, c2);
+        s_x[kk * 64 + t] = x;
+        s_z[kk * 64 + t] = z;
+        const uint32_t f = (__popcll(c1) + 2u * __popcll(c2)) & 3u;
+        if (phases_on && f) atomicAdd(&l_cnt[kk], f);
+      }
+    }
+    __syncthreads();
+  }
+  if (ok)
+    for (uint32_t k = wave; k < npiv; k += kSnapWaves) {
+      snap[k * row + w] = s_x[k * 64 + t];
+      snap[k * row + W + w] = s_z[k * 64 + t];
+    }
+  if (!phases_on) return;  // uniform
+  if (threadIdx.x < npiv && l_cnt[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], l_cnt[threadIdx.x]);
   __threadfence();
   __syncthreads();
-  if (t == 0) l_last = atomicAdd(&st->done, 1u) == gridDim.x - 1 ? 1u : 0u;
+  if (threadIdx.x == 0) l_last = atomicAdd(&st->done, 1u) == gridDim.x - 1 ? 1u : 0u;
   __syncthreads();
   if (!l_last) return;  // uniform
   __threadfence();
   // snapshot phases: phases[p_k] + its product counters + the phases of the
   // earlier snapshots in its set, as two bit masks (low and high phase bits)
-  if (t < npiv) l_cnt[t] = (uint32_t)phases[l_piv[t]] + atomicAdd(&cnt[t], 0u);
+  if (threadIdx.x < npiv)
+    l_cnt[threadIdx.x] = (uint32_t)phases[l_piv[threadIdx.x]] + atomicAdd(&cnt[threadIdx.x], 0u);
   __syncthreads();
-  if (t == 0) {
+  if (threadIdx.x == 0) {
     uint64_t lo = 0, hi = 0;
     for (uint32_t k = 0; k < npiv; ++k) {
       const uint64_t early = l_mask[k] & ((1ull << k) - 1ull);
@@ -494,8 +526,8 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
       else if (J <= 12) QCF_FACTOR(12);
       else QCF_FACTOR(16);
 #undef QCF_FACTOR
-      hipLaunchKernelGGL(canon_snap, dim3(snap_grid), dim3(64), kSnapLds, stream, xzs, phases, ld,
-                         W, mask, piv, st, snap, cnt, phases_on);
+      hipLaunchKernelGGL(canon_snap, dim3(snap_grid), dim3(64 * kSnapWaves), kSnapLds, stream,
+                         xzs, phases, ld, W, mask, piv, st, snap, cnt, phases_on);
       switch (G) {
         case 64:
           launch_apply<64>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream);
