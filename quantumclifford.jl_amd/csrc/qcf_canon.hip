@@ -47,8 +47,7 @@ struct CanonState {
   uint64_t ph_lo, ph_hi;  // bit k: low / high bit of snapshot k's phase
 };
 
-constexpr int kFactorThreads = 1024;
-constexpr uint32_t kMaxCanonRows = 16 * kFactorThreads;  // 16 slots per thread
+constexpr uint32_t kMaxCanonRows = 16 * 1024;  // 16 slots per thread of 1024
 
 __device__ __forceinline__ void mul_word_c(uint64_t x1, uint64_t z1, uint64_t &x2, uint64_t &z2,
                                            uint64_t &c1, uint64_t &c2) {
@@ -82,26 +81,26 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 }
 
 // One panel: word `c` of every row (x word w, or z word W+w), `nb` valid bits.
-// Slot s (physical row) belongs to thread s % 1024, register j = s / 1024; the
+// Slot s (physical row) belongs to thread s % NT, register j = s / NT; the
 // panel words also live in LDS (s_word[s], dynamic) so that every thread can
 // read the pivot's word after the step's one barrier.  Keys are
 // (logical row << 14) | slot, so nrows <= 16384.  The step's shared values are
 // double-buffered by step parity: a buffer is rewritten two steps later, after
 // a barrier that every reader of it has passed.
-template <int J>
-__global__ __launch_bounds__(kFactorThreads) void canon_factor(
+template <int NT, int J>
+__global__ __launch_bounds__(NT) void canon_factor(
     const uint64_t *__restrict__ xzs, size_t ld, uint32_t nrows, uint32_t c, uint32_t nb,
     uint32_t *__restrict__ inv, uint64_t *__restrict__ mask, uint32_t *__restrict__ piv,
     CanonState *st, uint32_t *__restrict__ cnt, int mark_rx) {
   extern __shared__ uint64_t s_word[];
-  __shared__ uint32_t s_part[2][kFactorThreads / 64];
+  __shared__ uint32_t s_part[2][NT / 64];
   __shared__ uint32_t s_si[2];
   const uint32_t t = threadIdx.x;
   uint64_t word[J], m[J];
   uint32_t lid[J];
 #pragma unroll
   for (int j = 0; j < J; ++j) {
-    const uint32_t s = t + j * kFactorThreads;
+    const uint32_t s = t + j * NT;
     const bool ok = s < nrows;
     word[j] = ok ? xzs[(size_t)s * ld + c] : 0ull;
     lid[j] = ok ? inv[s] : UINT_MAX;
@@ -118,7 +117,7 @@ __global__ __launch_bounds__(kFactorThreads) void canon_factor(
     uint32_t key = UINT_MAX;
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-      const uint32_t s = t + j * kFactorThreads;
+      const uint32_t s = t + j * NT;
       const uint32_t kk = (lid[j] << 14) | s;
       if (((word[j] >> b) & 1ull) && lid[j] >= i && kk < key) key = kk;
       if (lid[j] == i) s_si[par] = s;
@@ -128,14 +127,14 @@ __global__ __launch_bounds__(kFactorThreads) void canon_factor(
     __syncthreads();
     uint32_t best = UINT_MAX;
 #pragma unroll
-    for (int w = 0; w < kFactorThreads / 64; ++w) best = min(best, s_part[par][w]);
+    for (int w = 0; w < NT / 64; ++w) best = min(best, s_part[par][w]);
     if (best == UINT_MAX) continue;  // no pivot in this column (uniform)
     const uint32_t ps = best & 0x3FFFu, pl = best >> 14;
     const uint32_t si = s_si[par];
     const uint64_t pw = s_word[ps];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-      const uint32_t s = t + j * kFactorThreads;
+      const uint32_t s = t + j * NT;
       if (s == ps) {
         lid[j] = i;  // the pivot row moves to logical row i ...
       } else {
@@ -153,7 +152,7 @@ __global__ __launch_bounds__(kFactorThreads) void canon_factor(
   }
 #pragma unroll
   for (int j = 0; j < J; ++j) {
-    const uint32_t s = t + j * kFactorThreads;
+    const uint32_t s = t + j * NT;
     if (s < nrows) {
       inv[s] = lid[j];
       mask[s] = m[j];
@@ -419,17 +418,49 @@ size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 constexpr uint32_t kSnapLds = 2 * 64 * 64 * sizeof(uint64_t);  // 64 KB, the default limit
 
-template <int J>
+template <int NT, int J>
 int launch_factor(const uint64_t *xzs, size_t ld, uint32_t nrows, uint32_t c, uint32_t nb,
                   uint32_t *inv, uint64_t *mask, uint32_t *piv, CanonState *st, uint32_t *cnt,
                   int mark_rx, hipStream_t stream) {
   const uint32_t lds = nrows * (uint32_t)sizeof(uint64_t);  // at most 128 KB
   if (lds > 64u * 1024u)
-    QCF_HIP_CHECK(hipFuncSetAttribute((const void *)canon_factor<J>,
+    QCF_HIP_CHECK(hipFuncSetAttribute((const void *)canon_factor<NT, J>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(canon_factor<J>, dim3(1), dim3(kFactorThreads), lds, stream, xzs, ld, nrows,
-                     c, nb, inv, mask, piv, st, cnt, mark_rx);
+  hipLaunchKernelGGL((canon_factor<NT, J>), dim3(1), dim3(NT), lds, stream, xzs, ld, nrows, c, nb,
+                     inv, mask, piv, st, cnt, mark_rx);
   return QCF_OK;
+}
+
+// Threads of the elimination workgroup.  A pivot step is a chain of
+// dependent LDS round trips, a DPP minimum and a barrier; 1024 threads (fewest
+// rows per thread) measured fastest: 1000 rows 1.62 / 1.47 / 1.47 ms and 6000
+// rows - / 25.2 / 22.3 ms for 256 / 512 / 1024 threads.  QCF_CANON_THREADS
+// (256, 512 or 1024) overrides the choice.
+uint32_t factor_threads(uint32_t nrows) {
+  uint32_t nt = 1024;
+  if (const char *e = getenv("QCF_CANON_THREADS")) {
+    const int v = atoi(e);
+    if (v == 256 || v == 512 || v == 1024) nt = (uint32_t)v;
+  }
+  while (nt < 1024 && (nrows + nt - 1) / nt > 16) nt *= 2;  // at most 16 rows per thread
+  return nt;
+}
+
+template <int NT>
+int dispatch_factor(uint32_t J, const uint64_t *xzs, size_t ld, uint32_t n, uint32_t c,
+                    uint32_t nb, uint32_t *inv, uint64_t *mask, uint32_t *piv, CanonState *st,
+                    uint32_t *cnt, int mark, hipStream_t stream) {
+#define QCF_LAUNCH_FACTOR(JJ) \
+  launch_factor<NT, JJ>(xzs, ld, n, c, nb, inv, mask, piv, st, cnt, mark, stream)
+  if (J <= 1) return QCF_LAUNCH_FACTOR(1);
+  if (J <= 2) return QCF_LAUNCH_FACTOR(2);
+  if (J <= 3) return QCF_LAUNCH_FACTOR(3);
+  if (J <= 4) return QCF_LAUNCH_FACTOR(4);
+  if (J <= 6) return QCF_LAUNCH_FACTOR(6);
+  if (J <= 8) return QCF_LAUNCH_FACTOR(8);
+  if (J <= 12) return QCF_LAUNCH_FACTOR(12);
+  return QCF_LAUNCH_FACTOR(16);
+#undef QCF_LAUNCH_FACTOR
 }
 
 template <int G>
@@ -506,7 +537,8 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
   const uint32_t apply_grid =
       std::max<uint32_t>(1, std::min<uint32_t>((n + rows_per_block - 1) / rows_per_block,
                                                (uint32_t)ds->cu_count * 8));
-  const uint32_t J = (n + kFactorThreads - 1) / kFactorThreads;
+  const uint32_t NT = factor_threads(n);
+  const uint32_t J = (n + NT - 1) / NT;
   const uint32_t snap_grid = (uint32_t)((W + 63) / 64);
   for (int zstage = 0; zstage < 2; ++zstage) {
     for (size_t w = 0; w < W; ++w) {
@@ -515,17 +547,12 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
       const uint32_t nb = (uint32_t)std::min<size_t>(64, nqubits - first);
       const uint32_t c = (uint32_t)(zstage ? W + w : w);
       const int mark = (zstage == 1 && w == 0) ? 1 : 0;
-#define QCF_FACTOR(JJ) \
-  QCF_TRY(launch_factor<JJ>(xzs, ld, n, c, nb, inv, mask, piv, st, cnt, mark, stream))
-      if (J <= 1) QCF_FACTOR(1);
-      else if (J <= 2) QCF_FACTOR(2);
-      else if (J <= 3) QCF_FACTOR(3);
-      else if (J <= 4) QCF_FACTOR(4);
-      else if (J <= 6) QCF_FACTOR(6);
-      else if (J <= 8) QCF_FACTOR(8);
-      else if (J <= 12) QCF_FACTOR(12);
-      else QCF_FACTOR(16);
-#undef QCF_FACTOR
+      if (NT == 256)
+        QCF_TRY(dispatch_factor<256>(J, xzs, ld, n, c, nb, inv, mask, piv, st, cnt, mark, stream));
+      else if (NT == 512)
+        QCF_TRY(dispatch_factor<512>(J, xzs, ld, n, c, nb, inv, mask, piv, st, cnt, mark, stream));
+      else
+        QCF_TRY(dispatch_factor<1024>(J, xzs, ld, n, c, nb, inv, mask, piv, st, cnt, mark, stream));
       hipLaunchKernelGGL(canon_snap, dim3(snap_grid), dim3(64 * kSnapWaves), kSnapLds, stream,
                          xzs, phases, ld, W, mask, piv, st, snap, cnt, phases_on);
       switch (G) {
