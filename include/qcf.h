@@ -114,7 +114,8 @@ int qcf_mul_rows(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nwords,
  * them) and allows phases == NULL.  When `ranks` (device, 2 x uint32) is not
  * NULL it receives the pivots of the X stage and of both stages (the
  * reference's rx-1 and i-1 of ranks=true).  Stream-ordered; the workspace is
- * library-owned (per device, grown on demand).  nrows <= 16384.
+ * library-owned (per device, grown on demand) and shared by every call on
+ * that device, so calls that overlap on one device race.  nrows <= 16384.
  */
 int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nwords,
                      size_t nrows, size_t nqubits, int phases_on, uint32_t *ranks,

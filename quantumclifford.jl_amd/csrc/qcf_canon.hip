@@ -16,17 +16,21 @@
 // rows, and the row order as a slot -> logical-row map (swaps move no data).
 // canon_snap then computes, for each pivot k, the full pivot row as it was at
 // step k (its original row times the earlier pivot rows in its set); every
-// word of that is independent of the others, so one thread per word runs the
-// products in order.  canon_apply finally replays each row's set against those
-// snapshots: every row is read and written once per panel instead of once per
-// pivot.  Phases: each product's phase is a sum of per-position mod-4
+// word of that is independent of the others, so one lane per word runs the
+// products in order (eight waves share the chain by pivot groups).
+// canon_apply finally replays each row's set against those snapshots: every
+// row is read and written once per panel instead of once per pivot.  Phases: each product's phase is a sum of per-position mod-4
 // counters, so a lane may accumulate the counters of all products on its
 // words and the group reduces once per row.  After the last panel the rows
 // move to their logical places (canon_gather, canon_scatter_back).
 //
 // Roofline per panel: the row pass reads and writes the tableau once
 // (2 * nrows * 16 * W bytes); snapshots (64 * 16 * W bytes) are L2-resident.
-// The factor step is latency-bound (64 dependent workgroup reductions).
+// The elimination (64 dependent workgroup reductions) and the snapshot chain
+// are latency-bound and make the critical path.
+//
+// The workspace (per device, grown on demand) is shared by all calls on a
+// device: calls that overlap on one device race.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
