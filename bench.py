@@ -344,8 +344,8 @@ def main():
     ap.add_argument("--batched-dmax", type=int, default=15)
     ap.add_argument("--mlp-steps", type=int, default=24414)  # 2e8 shots of 8192
     ap.add_argument("--mlp-eval-shots", type=int, default=10**7)
-    ap.add_argument("--only", choices=("reference_circuits", "canonicalize", "mul_config",
-                                       "batched"),
+    ap.add_argument("--only", choices=("reference_circuits", "canonicalize", "mul_left_1gq",
+                                       "mul_config", "batched"),
                     help="run one extra alone and print its JSON (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -358,6 +358,7 @@ def main():
     if a.only:
         fn = {"reference_circuits": lambda: bench_reference_circuits(5, 1),
               "canonicalize": lambda: bench_canonicalize(5, 1),
+              "mul_left_1gq": lambda: bench_mul(10, 2),
               "mul_config": lambda: bench_mul_config(10, 2),
               "batched": lambda: bench_batched(dmax=a.batched_dmax)}[a.only]
         print(json.dumps({a.only: fn()}), flush=True)
