@@ -96,3 +96,18 @@ def test_sites_of_all_rates_share_one_thinned_class():
     assert p.info.nclasses == 1 and p.info.nsites == 9
     # a zero-rate site needs no class
     assert qcf.Program([qcf.PauliError(1, 0.0)]).info.nclasses == 0
+
+
+@pytest.mark.parametrize("nrows,nwords,ld,nqubits,exc", [
+    (16385, 1, 2, 64, qcf.QCFError),            # more rows than the elimination kernel holds
+    (10, 1, 2, 65, qcf.DimensionMismatch),      # 65 qubits do not fit one word per half
+    (10, 2, 3, 100, qcf.DimensionMismatch),     # row stride shorter than the row
+])
+def test_canonicalize_argument_errors(nrows, nwords, ld, nqubits, exc):
+    """qcf_canonicalize rejects bad shapes before it touches the device (so
+    host pointers are never dereferenced here)."""
+    xzs = np.zeros(max(1, nrows * ld), np.uint64)
+    ph = np.zeros(max(1, nrows), np.uint8)
+    with pytest.raises(exc):
+        _lib.call("qcf_canonicalize", xzs.ctypes.data_as(C.c_void_p),
+                  ph.ctypes.data_as(C.c_void_p), ld, nwords, nrows, nqubits, 1, None, None)
