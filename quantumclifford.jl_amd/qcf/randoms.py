@@ -70,3 +70,41 @@ def random_tableau(rng: np.random.Generator, nrows: int, n: int):
         xzs[:, w - 1] &= m
         xzs[:, 2 * w - 1] &= m
     return np.zeros(nrows, np.uint8), xzs
+
+
+def random_stabilizer_layers(rng: np.random.Generator, n: int, layers: int | None = None):
+    """A dense random stabilizer state of n qubits (the kind random_stabilizer(n)
+    returns, src/randoms.jl): |0...0> (rows Z_i) conjugated by `layers` layers,
+    each a random H or S (or nothing) on every qubit followed by CNOTs on a
+    random pairing of all qubits, with the reference's phase rules
+    (src/symbolic_cliffords.jl:105-118, 359-395).  Random pairings scramble in
+    O(log n) layers; the default 2*ceil(log2 n) + 8 leaves rows dense.
+    Returns (phases uint8[n], xzs uint64[n, 2W]) row-contiguous."""
+    w = (n + 63) // 64
+    if layers is None:
+        layers = 2 * int(np.ceil(np.log2(max(n, 2)))) + 8
+    xt = np.zeros((n, n), np.uint8)   # xt[q, row]
+    zt = np.eye(n, dtype=np.uint8)
+    ph = np.zeros(n, np.int64)
+    for _ in range(layers):
+        kind = rng.integers(0, 3, n)  # 0: H, 1: S, 2: nothing
+        h = np.nonzero(kind == 0)[0]
+        sg = np.nonzero(kind == 1)[0]
+        ph = (ph + 2 * (xt[h] & zt[h]).sum(axis=0, dtype=np.int64)) & 3
+        xt[h], zt[h] = zt[h].copy(), xt[h].copy()
+        ph = (ph + 2 * (xt[sg] & zt[sg]).sum(axis=0, dtype=np.int64)) & 3
+        zt[sg] ^= xt[sg]
+        perm = rng.permutation(n)
+        a, b = perm[: n // 2], perm[n // 2: 2 * (n // 2)]
+        x1, z1, x2, z2 = xt[a], zt[a], xt[b], zt[b]
+        flip = (x1 & z1 & x2 & z2) | (x1 & z2 & (1 - (z1 | x2)))
+        ph = (ph + 2 * flip.sum(axis=0, dtype=np.int64)) & 3
+        zt[a] = z1 ^ z2
+        xt[b] = x2 ^ x1
+    xs = np.zeros((n, w * 64), np.uint8)
+    zs = np.zeros((n, w * 64), np.uint8)
+    xs[:, :n] = xt.T
+    zs[:, :n] = zt.T
+    px = np.packbits(xs, axis=1, bitorder="little").view(np.uint64)
+    pz = np.packbits(zs, axis=1, bitorder="little").view(np.uint64)
+    return ph.astype(np.uint8), np.ascontiguousarray(np.concatenate([px, pz], axis=1))

@@ -55,6 +55,17 @@ def test_random_stabilizers(gpu, n):
         check(ph, xzs, n, phases=False)
 
 
+@pytest.mark.parametrize("n", [64, 333, 1000])
+def test_dense_random_stabilizers(gpu, n):
+    """Dense random stabilizers (3/4 of the entries non-identity, like the
+    reference's random_stabilizer), full and half rank, phases on and off."""
+    ph, xzs = randoms.random_stabilizer_layers(np.random.default_rng(n + 7), n)
+    print(f"canonicalize! dense random stabilizer n={n}", flush=True)
+    check(ph, xzs, n)
+    check(ph, xzs, n, phases=False)
+    check(ph[: n // 2].copy(), np.ascontiguousarray(xzs[: n // 2]), n)
+
+
 @pytest.mark.parametrize("nrows,n", [(300, 500), (700, 200), (2048 + 3, 2048 + 3), (1, 70)])
 def test_random_tableaux_ragged(gpu, nrows, n):
     """random_tableau(nrows, n): rows need not commute; more or fewer rows than

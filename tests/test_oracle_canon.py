@@ -115,6 +115,20 @@ def test_canonical_form_depends_only_on_the_group(n):
     assert np.array_equal(xzs, xzs2) and np.array_equal(ph, ph2)
 
 
+def test_dense_generator_gives_dense_valid_stabilizers():
+    """random_stabilizer_layers: commuting rows with real phases, full rank, and
+    about 3/4 of the entries non-identity (a random stabilizer's density)."""
+    n = 200
+    ph, xzs = randoms.random_stabilizer_layers(np.random.default_rng(1), n)
+    assert set(np.unique(ph).tolist()) <= {0, 2}
+    assert all(oracle.comm(xzs[i], xzs[j]) == 0 for i in range(0, n, 7) for j in range(0, n, 5))
+    W = xzs.shape[1] // 2
+    xs = np.unpackbits(xzs[:, :W].view(np.uint8), axis=1, bitorder="little")[:, :n]
+    zs = np.unpackbits(xzs[:, W:].view(np.uint8), axis=1, bitorder="little")[:, :n]
+    assert abs(((xs | zs) != 0).mean() - 0.75) < 0.02
+    assert oracle.canonicalize(xzs.copy(), ph.copy(), n)[1] == n
+
+
 def test_phases_off_keeps_phases_and_matches_bits():
     rng = np.random.default_rng(3)
     ph, xzs = randoms.random_clifford_tableau(rng, 70)
