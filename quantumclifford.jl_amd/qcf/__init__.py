@@ -11,9 +11,9 @@ from .ops import *  # noqa: F401,F403  (reference operation names)
 from .ops import Depolarize2, PauliError, affectedbits, affectedqubits
 from .pauli import (P, PauliOperator, S, Tableau, canonicalize_, mul_left_, mul_left_row_,
                     mul_left_sweep_, mul_left_tableau_pauli_, mul_left_vec_, mul_right_)
-from .frames import (PauliFrame, Program, alloc_masks, frame_from_reference,
-                     frame_to_reference, initZ_, lower_circuit, measurements,
-                     pfmeasurements, pftrajectories, pftrajectories_state, seed_)
+from .frames import (PauliFrame, Program, alloc_masks, apply_, applynoise_,
+                     frame_from_reference, frame_to_reference, initZ_, lower_circuit,
+                     measurements, pfmeasurements, pftrajectories, pftrajectories_state, seed_)
 from . import batch, circuits, decoders, jltext, randoms
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import OP_DTYPE, ProgramInfo, call
+from ._lib import OP_DTYPE, DimensionMismatch, ProgramInfo, call
 from .ops import Operation
 
 
@@ -190,8 +190,8 @@ def pftrajectories(circuit, trajectories: int = 5000, threads: bool = True, *,
     a fresh frame sized from the circuit, initZ!, then the circuit.  `threads` is
     accepted for signature compatibility (the GPU runs all shots in parallel).
     `masks` replays pre-drawn randomness; `export_masks` records the native draws."""
-    if isinstance(circuit, PauliFrame):
-        raise TypeError("use pftrajectories_state(state, circuit)")
+    if isinstance(circuit, PauliFrame):  # pftrajectories(state, circuit)
+        return pftrajectories_state(circuit, trajectories, seed=seed, masks=masks)
     prog = circuit if isinstance(circuit, Program) else Program(circuit)
     f = PauliFrame(trajectories, prog.nq - 1, prog.nbits, device=device, init=False)
     if not keep_frame:
@@ -207,21 +207,51 @@ def pftrajectories(circuit, trajectories: int = 5000, threads: bool = True, *,
     return f
 
 
+def apply_(frame: PauliFrame, op: Operation, *, seed: int | None = None) -> PauliFrame:
+    """apply!(frame::PauliFrame, op) for any supported operation
+    (src/pauli_frames.jl:58-134): a one-operation program on the device."""
+    return pftrajectories_state(frame, [op], seed=seed)
+
+
+def applynoise_(frame: PauliFrame, noise, i: int, *, seed: int | None = None) -> PauliFrame:
+    """applynoise!(frame::PauliFrame, noise, i) (src/pauli_frames.jl:136-174):
+    the noise channel on qubit i of every frame."""
+    from .ops import NoiseOp
+    return pftrajectories_state(frame, [NoiseOp(noise, (i,))], seed=seed)
+
+
 def alloc_masks(prog: Program, shots: int, device=None) -> torch.Tensor:
     dev = torch.device(device) if device is not None else torch.device(
         "cuda", torch.cuda.current_device())
     return torch.zeros(max(1, prog.nreplay) * plane_words(shots), dtype=torch.int64, device=dev)
 
 
-def measurements(frame: PauliFrame) -> torch.Tensor:
-    """measurements(frame): shots x bits Bool matrix, column-major like the
-    reference's Matrix{Bool} (src/pauli_frames.jl:280)."""
+def _relative_measurements(frame: PauliFrame) -> torch.Tensor:
     out = torch.empty(max(1, frame.nbits) * max(1, frame.shots), dtype=torch.uint8,
                       device=frame.record.device)
     call("qcf_record_to_bool", _ptr(frame.record), frame.ld, frame.shots, frame.nbits,
          _ptr(out), _stream())
     m = out[: frame.nbits * frame.shots].view(frame.nbits, frame.shots).t()
     return m.bool()
+
+
+def measurements(*args) -> torch.Tensor:
+    """measurements(frame): shots x bits Bool matrix of flips relative to the
+    reference measurements, column-major like the reference's Matrix{Bool}
+    (src/pauli_frames.jl:280).  measurements(register, frame): the absolute
+    outcomes, the register's measurement bits XOR each shot's flips
+    (src/pauli_frames.jl:289); `register` is the noiseless reference run's bit
+    vector (the reference's bitview(register)), one Bool per classical bit."""
+    if len(args) == 1:
+        return _relative_measurements(args[0])
+    if len(args) != 2:
+        raise TypeError("measurements(frame) or measurements(register, frame)")
+    register, frame = args
+    bits = torch.as_tensor(np.asarray(register, dtype=bool), device=frame.record.device)
+    if bits.numel() != frame.nbits:
+        raise DimensionMismatch(
+            f"register holds {bits.numel()} bits, the frame records {frame.nbits}")
+    return _relative_measurements(frame) ^ bits.view(1, -1)
 
 
 pfmeasurements = measurements  # deprecated alias (src/pauli_frames.jl:291-293)

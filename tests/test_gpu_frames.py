@@ -285,6 +285,50 @@ def test_measurements_matrix_layout(gpu):
     assert np.array_equal(m.cpu().numpy().T.astype(np.uint8), bits)
 
 
+def test_absolute_measurements_from_register_bits(gpu):
+    """measurements(register, frame) = bitview(register) .xor measurements(frame)
+    (src/pauli_frames.jl:289): every column flips where the reference bit is 1."""
+    c = circuits.repetition_code(3, 3, 0.05)
+    f = qcf.pftrajectories(c.ops, 2049, seed=8)
+    ref = np.random.default_rng(3).integers(0, 2, c.nbits).astype(bool)
+    rel = qcf.measurements(f).cpu().numpy()
+    ab = qcf.measurements(ref, f).cpu().numpy()
+    assert ab.shape == rel.shape and ab.dtype == bool
+    assert np.array_equal(ab, rel ^ ref[None, :])
+    with pytest.raises(qcf.DimensionMismatch):
+        qcf.measurements(ref[:-1], f)
+
+
+def test_apply_and_applynoise_one_op_at_a_time(gpu):
+    """apply!/applynoise! on a PauliFrame op by op give the same frame and record
+    as the whole circuit through pftrajectories(state, circuit) (src/pauli_frames.jl:
+    58-174, 234-239), replaying the same masks: the noise ops draw nothing else."""
+    ops = [qcf.sHadamard(1), qcf.sCNOT(1, 2), qcf.sMZ(2, 1), qcf.sPhase(2), qcf.sMRZ(1, 2),
+           qcf.sSWAP(1, 3), qcf.sMZ(3, 3)]
+    shots = 3000
+    a = qcf.PauliFrame(shots, 3, 3, init=False)
+    b = a.copy()
+    a.frame.random_(-2**62, 2**62)
+    b.frame.copy_(a.frame)
+    # whole circuit, with the one sMRZ z draw supplied as a replay mask
+    masks = torch.randint(-2**62, 2**62, (qcf.frames.plane_words(shots),), dtype=torch.int64,
+                          device=a.frame.device)
+    qcf.pftrajectories(a, ops, masks=masks)
+    for op in ops:
+        if isinstance(op, qcf.sMRZ):
+            qcf.pftrajectories_state(b, [op], masks=masks)
+        else:
+            qcf.apply_(b, op)
+    assert np.array_equal(a.frame_host(), b.frame_host())
+    assert np.array_equal(a.record_host(), b.record_host())
+    # applynoise!: X flips with p = 1 flip every x bit of the qubit
+    c = a.copy()
+    qcf.applynoise_(c, qcf.ops.PauliNoise(1.0, 0.0, 0.0), 2)
+    fx, cx = a.frame_host(), c.frame_host()
+    bits = oracle.unpack_shots((fx ^ cx), shots)
+    assert bits[1].all() and not np.delete(bits, 1, axis=0).any()
+
+
 def test_reference_frame_layout_roundtrip(gpu):
     c = circuits.rotated_surface_code(3, 2, 0.01)
     f = qcf.pftrajectories(c.ops, 777, seed=9)
