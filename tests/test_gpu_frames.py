@@ -329,6 +329,49 @@ def test_apply_and_applynoise_one_op_at_a_time(gpu):
     assert bits[1].all() and not np.delete(bits, 1, axis=0).any()
 
 
+def test_repetition_code_syndrome_implications(gpu):
+    """test/test_pauliframe.jl:2-22: encode |111> with sX and CNOTs, X error on
+    qubit 2 with p = 0.75, syndrome CNOTs and sMZ.  Wherever the frame has X on
+    qubit 2 (and no Z), qubits 4 and 5 carry X and both syndrome bits are set;
+    the state form pftrajectories(frame, circuit) and the fresh form agree on
+    that rule."""
+    circuit = [qcf.sX(1), qcf.sCNOT(1, 2), qcf.sCNOT(1, 3), qcf.PauliError(2, 0.75),
+               qcf.sCNOT(1, 4), qcf.sCNOT(2, 4), qcf.sCNOT(2, 5), qcf.sCNOT(3, 5),
+               qcf.sMZ(4, 1), qcf.sMZ(5, 2)]
+    shots = 4096 + 11
+    frames = [qcf.pftrajectories(qcf.PauliFrame(shots, 5, 2), circuit, seed=1),
+              qcf.pftrajectories(circuit, trajectories=shots, seed=2)]
+    for f in frames:
+        fx = oracle.unpack_shots(f.frame_host(), shots)  # planes: x of 1..6, z of 1..6
+        m = qcf.measurements(f).cpu().numpy()
+        nq = f.nq
+        sel = (fx[1] == 1) & (fx[nq + 1] == 0)
+        # X or Y on qubit 2 with probability 1/2 (PauliError(2, 0.75)), and its z
+        # bit is a uniformly random gauge bit (initZ! propagated by the CNOTs):
+        # X-without-Z in a quarter of the frames (4 sigma at 4107 shots: 0.027)
+        assert abs(sel.mean() - 0.25) < 0.027
+        assert fx[3][sel].all() and fx[4][sel].all()
+        assert m[sel, 0].all() and m[sel, 1].all()
+
+
+def test_pauli_measurement_correlations(gpu):
+    """test/test_pauliframe.jl:92-115: H on three qubits, PauliMeasurement ZZ_
+    and _ZZ, then sMZ of each qubit; after decoding with the two parity
+    outcomes the three single-qubit outcomes agree in every frame."""
+    circuit = [qcf.sHadamard(1), qcf.sHadamard(2), qcf.sHadamard(3),
+               qcf.PauliMeasurement(qcf.P("ZZ_", device="cpu"), 1),
+               qcf.PauliMeasurement(qcf.P("_ZZ", device="cpu"), 2),
+               qcf.sMZ(1, 3), qcf.sMZ(2, 4), qcf.sMZ(3, 5)]
+    n = 3000
+    for m in (qcf.measurements(qcf.pftrajectories(qcf.PauliFrame(n, 3, 5), circuit, seed=3)),
+              qcf.measurements(qcf.pftrajectories(circuit, trajectories=n, seed=4))):
+        m = m.cpu().numpy().copy()
+        m[:, 3] ^= m[:, 0]
+        m[:, 4] ^= m[:, 0] ^ m[:, 1]
+        assert (m[:, 2] == m[:, 3]).all() and (m[:, 3] == m[:, 4]).all()
+        assert 0.4 < m[:, 2].mean() < 0.6  # the outcomes themselves are random
+
+
 def test_reference_frame_layout_roundtrip(gpu):
     c = circuits.rotated_surface_code(3, 2, 0.01)
     f = qcf.pftrajectories(c.ops, 777, seed=9)

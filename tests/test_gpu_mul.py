@@ -138,3 +138,19 @@ def test_pivot_sweep_and_tableau_pauli(gpu, n):
         s = oracle.mul_left_vec(r, xzs[pivot - 1].copy())
         assert np.array_equal(t4.xzs_host()[m - 1], r)
         assert t4.phases_host()[m - 1] == (s + ph[m - 1] + ph[pivot - 1]) % 4
+
+
+@pytest.mark.parametrize("per_cu", ["1", "2", "3", "7"])
+def test_result_independent_of_launch_shape(gpu, per_cu, monkeypatch):
+    """Like the reference's random block and batch sizes for its device kernels
+    (test/KernelAbstractions/implementation/definitions.jl:8-13): the product
+    and phase do not depend on how many workgroups split the words."""
+    monkeypatch.setenv("QCF_MUL_BLOCKS_PER_CU", per_cu)
+    rng = np.random.default_rng(77)
+    for n in (64 * 2049, 3_000_001):
+        (p1, x1), (p2, x2) = random_pauli(rng, n), random_pauli(rng, n)
+        r = dev_pauli(p2, x2, n)
+        qcf.mul_left_(r, dev_pauli(p1, x1, n))
+        ox, oph = x2.copy(), np.array([p2], np.uint8)
+        oracle.mul_left_pauli(ox, oph, x1.copy(), np.array([p1], np.uint8))
+        assert np.array_equal(r.xz_host(), ox) and r.phase_value() == int(oph[0])
