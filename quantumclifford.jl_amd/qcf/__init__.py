@@ -9,8 +9,11 @@ from ._lib import (ArgumentError, BoundsError, DimensionMismatch, HIPError, Meth
                    QCFError, lib)
 from .ops import *  # noqa: F401,F403  (reference operation names)
 from .ops import Depolarize2, PauliError, affectedbits, affectedqubits
-from .pauli import (P, PauliOperator, S, Tableau, canonicalize_, mul_left_, mul_left_row_,
-                    mul_left_sweep_, mul_left_tableau_pauli_, mul_left_vec_, mul_right_)
+from .pauli import (P, PauliOperator, S, Tableau, canonicalize_, mul_left_,
+                    mul_left_destabilizer_, mul_left_pauli_row_, mul_left_row_, mul_left_sweep_,
+                    mul_left_tableau_pauli_, mul_left_tableau_row_, mul_left_vec_, mul_right_,
+                    mul_right_pauli_row_, mul_right_row_, mul_right_tableau_pauli_,
+                    mul_right_tableau_row_)
 from .frames import (PauliFrame, Program, alloc_masks, apply_, applynoise_,
                      frame_from_reference, frame_to_reference, initZ_, lower_circuit,
                      measurements, pfmeasurements, pftrajectories, pftrajectories_state, seed_)

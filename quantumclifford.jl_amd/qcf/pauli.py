@@ -266,3 +266,77 @@ def mul_left_sweep_(s: Tableau, pivot: int, row_mask: torch.Tensor | None = None
     call("qcf_mul_rows", _ptr(s.xzs), _ptr(s.phases), s.ld, s.nwords, s.nrows,
          _ptr(s.xzs, p0 * s.ld), _ptr(s.phases, p0), p0, mask, int(bool(phases)), 0, _stream())
     return s
+
+
+def mul_right_row_(s: Tableau, m: int, i: int, phases: bool = True) -> Tableau:
+    """mul_right!(s::Tableau, m, i): row m := row m * row i (src/mul_leftright.jl:189-193)."""
+    m0, i0 = m - 1, i - 1
+    call("qcf_mul_right", _ptr(s.xzs, m0 * s.ld), _ptr(s.phases, m0), _ptr(s.xzs, i0 * s.ld),
+         _ptr(s.phases, i0), s.nwords, int(bool(phases)), _stream())
+    return s
+
+
+def _check_widths(a_nwords: int, b_nwords: int):
+    if a_nwords != b_nwords:
+        raise DimensionMismatch("The two Pauli operators should have the same length!")
+
+
+def mul_left_tableau_row_(s: Tableau, m: int, t: Tableau, i: int, phases: bool = True) -> Tableau:
+    """mul_left!(s::Tableau, m, t::Tableau, i): row m of s := row i of t * row m of s
+    (src/mul_leftright.jl:171-175).  As in the reference, the phase added is
+    s.phases[i] (of the first tableau), not t.phases[i]."""
+    _check_widths(s.nwords, t.nwords)
+    m0, i0 = m - 1, i - 1
+    call("qcf_mul_left", _ptr(s.xzs, m0 * s.ld), _ptr(s.phases, m0), _ptr(t.xzs, i0 * t.ld),
+         _ptr(s.phases, i0), s.nwords, int(bool(phases)), _stream())
+    return s
+
+
+def mul_right_tableau_row_(s: Tableau, m: int, t: Tableau, i: int, phases: bool = True) -> Tableau:
+    """mul_right!(s::Tableau, m, t::Tableau, i): row m of s := row m of s * row i of t
+    (src/mul_leftright.jl:183-187), adding s.phases[i] like the left form."""
+    _check_widths(s.nwords, t.nwords)
+    m0, i0 = m - 1, i - 1
+    call("qcf_mul_right", _ptr(s.xzs, m0 * s.ld), _ptr(s.phases, m0), _ptr(t.xzs, i0 * t.ld),
+         _ptr(s.phases, i0), s.nwords, int(bool(phases)), _stream())
+    return s
+
+
+def mul_left_pauli_row_(r: PauliOperator, l: Tableau, i: int, phases: bool = True) -> PauliOperator:
+    """mul_left!(r::PauliOperator, l::Tableau, i): r := row i of l * r
+    (src/mul_leftright.jl:151-155)."""
+    _check_widths(r.nwords, l.nwords)
+    i0 = i - 1
+    call("qcf_mul_left", _ptr(r.xz), _ptr(r.phase), _ptr(l.xzs, i0 * l.ld), _ptr(l.phases, i0),
+         r.nwords, int(bool(phases)), _stream())
+    return r
+
+
+def mul_right_pauli_row_(l: PauliOperator, r: Tableau, i: int, phases: bool = True) -> PauliOperator:
+    """mul_right!(l::PauliOperator, r::Tableau, i): l := l * row i of r
+    (src/mul_leftright.jl:159-163)."""
+    _check_widths(l.nwords, r.nwords)
+    i0 = i - 1
+    call("qcf_mul_right", _ptr(l.xz), _ptr(l.phase), _ptr(r.xzs, i0 * r.ld), _ptr(r.phases, i0),
+         l.nwords, int(bool(phases)), _stream())
+    return l
+
+
+def mul_right_tableau_pauli_(s: Tableau, p: PauliOperator, phases: bool = True) -> Tableau:
+    """mul_right!(s::Tableau, p::PauliOperator): every row := row * p
+    (src/mul_leftright.jl:234-240)."""
+    _check_widths(s.nwords, p.nwords)
+    call("qcf_mul_rows", _ptr(s.xzs), _ptr(s.phases), s.ld, s.nwords, s.nrows, _ptr(p.xz),
+         _ptr(p.phase), C.c_size_t(2**64 - 1), None, int(bool(phases)), 1, _stream())
+    return s
+
+
+def mul_left_destabilizer_(t: Tableau, i: int, j: int, phases: bool = True) -> Tableau:
+    """mul_left!(s::Destabilizer, i, j) on its 2n-row tableau (destabilizers
+    1..n, then stabilizers n+1..2n; src/mul_leftright.jl:200-206): destabilizer
+    j := destabilizer i * destabilizer j without phases (the indices flip to
+    keep the commutation relations), then stabilizer i := stabilizer j *
+    stabilizer i."""
+    n = t.nrows // 2
+    mul_left_row_(t, j, i, phases=False)
+    return mul_left_row_(t, i + n, j + n, phases=phases)

@@ -154,3 +154,71 @@ def test_result_independent_of_launch_shape(gpu, per_cu, monkeypatch):
         ox, oph = x2.copy(), np.array([p2], np.uint8)
         oracle.mul_left_pauli(ox, oph, x1.copy(), np.array([p1], np.uint8))
         assert np.array_equal(r.xz_host(), ox) and r.phase_value() == int(oph[0])
+
+
+def _tab(rng, nrows, n):
+    ph, xzs = randoms.random_tableau(rng, nrows, n)
+    return rng.integers(0, 4, nrows).astype(np.uint8), xzs
+
+
+@pytest.mark.parametrize("n", [63, 130, 4097])
+def test_tableau_row_forms(gpu, n):
+    """The row forms of src/mul_leftright.jl:151-193 and :234-240 against the
+    oracle's Pauli products on the same rows, including the quirk of the
+    two-tableau forms (the phase added is s.phases[i], not t.phases[i])."""
+    rng = np.random.default_rng(n)
+    sph, sxz = _tab(rng, 6, n)
+    tph, txz = _tab(rng, 6, n)
+    pph, pxz = random_pauli(rng, n)
+    W = sxz.shape[1] // 2
+
+    def row_op(dst_xz, dst_ph, src_xz, src_ph, right):
+        x, ph = dst_xz.copy(), np.array([dst_ph], np.uint8)
+        (oracle.mul_right_pauli if right else oracle.mul_left_pauli)(
+            x, ph, src_xz.copy(), np.array([src_ph], np.uint8))
+        return x, int(ph[0])
+
+    for right in (False, True):
+        # s[m] := t[i] * s[m] (or s[m] * t[i]) with s.phases[i]
+        s, t = qcf.Tableau(sph, n, sxz), qcf.Tableau(tph, n, txz)
+        (qcf.mul_right_tableau_row_ if right else qcf.mul_left_tableau_row_)(s, 2, t, 5)
+        ex, eph = row_op(sxz[1], sph[1], txz[4], sph[4], right)
+        assert np.array_equal(s.xzs_host()[1], ex) and int(s.phases_host()[1]) == eph
+        assert np.array_equal(np.delete(s.xzs_host(), 1, axis=0), np.delete(sxz, 1, axis=0))
+        # s[m] := s[i] * s[m] (or s[m] * s[i])
+        s = qcf.Tableau(sph, n, sxz)
+        (qcf.mul_right_row_ if right else qcf.mul_left_row_)(s, 3, 6)
+        ex, eph = row_op(sxz[2], sph[2], sxz[5], sph[5], right)
+        assert np.array_equal(s.xzs_host()[2], ex) and int(s.phases_host()[2]) == eph
+        # Pauli := row i * Pauli (or Pauli * row i), with the row's phase
+        p = qcf.PauliOperator(pph, n, pxz)
+        (qcf.mul_right_pauli_row_ if right else qcf.mul_left_pauli_row_)(p, t, 4)
+        ex, eph = row_op(pxz, pph, txz[3], tph[3], right)
+        assert np.array_equal(p.xz_host(), ex) and p.phase_value() == eph
+    # every row := row * p
+    s = qcf.Tableau(sph, n, sxz)
+    qcf.mul_right_tableau_pauli_(s, qcf.PauliOperator(pph, n, pxz))
+    for m in range(6):
+        ex, eph = row_op(sxz[m], sph[m], pxz, pph, True)
+        assert np.array_equal(s.xzs_host()[m], ex) and int(s.phases_host()[m]) == eph
+    with pytest.raises(qcf.DimensionMismatch):
+        qcf.mul_left_pauli_row_(qcf.PauliOperator(0, n + 64, np.zeros(2 * (W + 1), np.uint64)), t, 1)
+
+
+def test_destabilizer_index_flip(gpu):
+    """mul_left!(s::Destabilizer, i, j) (src/mul_leftright.jl:200-206):
+    destabilizer j := destabilizer i * destabilizer j without phases, then
+    stabilizer i := stabilizer j * stabilizer i with phases."""
+    n = 70
+    rng = np.random.default_rng(9)
+    ph, xzs = _tab(rng, 2 * n, n)
+    t = qcf.Tableau(ph, n, xzs)
+    i, j = 3, 11
+    qcf.mul_left_destabilizer_(t, i, j)
+    ex = xzs.copy()
+    eph = ph.copy()
+    ex[j - 1] ^= xzs[i - 1]  # phases off: a plain XOR, phase byte untouched
+    x, p = xzs[i - 1 + n].copy(), np.array([ph[i - 1 + n]], np.uint8)
+    oracle.mul_left_pauli(x, p, xzs[j - 1 + n].copy(), np.array([ph[j - 1 + n]], np.uint8))
+    ex[i - 1 + n], eph[i - 1 + n] = x, p[0]
+    assert np.array_equal(t.xzs_host(), ex) and np.array_equal(t.phases_host(), eph)
