@@ -20,6 +20,12 @@
  * the reference does with Julia's default RNG (used only to time a CPU
  * baseline; that stream is not reproducible across thread counts, like the
  * reference's Threads.@threads split, src/pauli_frames.jl:215-222).
+ *
+ * Parity unpinned: the two-qubit depolarizing channel (the reference has no
+ * PauliFrame method for it; only its closed-form rates and the base-4 digit
+ * order of src/noise.jl:112-118 check it) and the z pad bits that initZ!
+ * randomizes past the last qubit (written as zeros here; no operation reads
+ * them).
  */
 #include <math.h>
 #include <stdint.h>
