@@ -569,21 +569,24 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
     wpc = global ? (global_waves_per_cu() ? global_waves_per_cu() : kGlobalWavesPerCU)
                  : std::max<uint32_t>(1, std::min<uint32_t>(16, (uint32_t)lds_max / std::max(1u, lds_bytes)));
   const uint64_t grid = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * wpc);
-  if (global) {
-    QCF_TRY(frame_scratch(grid * (size_t)P->nplanes * 256u, &kp.scratch));
-    kp.sstride = P->nplanes * 64u;
-  }
   // Native launches of on-chip frames run the kernel specialised to this
   // program (the partial last tile included); the interpreter is the general
   // path.  Both draw the same events for the same shots.
-  // Frames in global memory stay on the interpreter unless QCF_JIT_GLOBAL=1:
-  // both are bound by L2 latency there and the specialised code measured no
-  // faster for d >= 9 while costing seconds of compilation.
-  if (!masks && !exp && !jit_disabled() && (!global || jit_global_enabled())) {
+  // Frames too large for LDS run the split kernel: the planes used most in
+  // registers, the rest in LDS, one wave per CU.  With QCF_JIT_GLOBAL=1 (or
+  // when the split kernel cannot be built) they keep every plane in global
+  // memory: the specialised kernel there is opt-in, since both it and the
+  // interpreter are bound by L2 latency and it measured no faster for d >= 9
+  // while costing seconds of compilation.
+  const bool try_split = global && !force_global_frames() && !jit_global_enabled() &&
+                         jit_split_enabled();
+  if (!masks && !exp && !jit_disabled() && (!global || try_split || jit_global_enabled())) {
     JitKernelInfo jk;
     // records under 4 GB take 32-bit byte offsets from one base
     const bool rec32 = (uint64_t)std::max<uint32_t>(P->nbits, 1u) * ld * 8ull < (1ull << 32);
-    int js = jit_get(P, variant | (global ? 4 : 0) | (rec32 ? 8 : 0), &jk);
+    int js = try_split ? jit_get(P, variant | 4 | 16 | (rec32 ? 8 : 0), &jk)
+                       : jit_get(P, variant | (global ? 4 : 0) | (rec32 ? 8 : 0), &jk);
+    const bool gjit = global && !try_split;  // specialised kernel on global planes
     if (js == QCF_OK) {
       JitParams jp{};
       jp.rec = kp.rec;
@@ -598,12 +601,12 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
       jp.has_class = kp.nclasses > 0 ? 1u : 0u;
       jp.nfull = (uint32_t)(shots / kBlockShots);
       uint32_t jwpc = waves_per_cu ? waves_per_cu : jk.blocks_per_cu;
-      if (global)
+      if (gjit)
         jwpc = global_waves_per_cu() ? global_waves_per_cu()
                                      : std::min<uint32_t>(jwpc, kGlobalJitWavesPerCU);
       uint64_t g1 = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * jwpc);
       const uint32_t nx = jit_xcds();
-      if (!global && jit_full_grid()) {
+      if (!gjit && jit_full_grid()) {
         // one workgroup per tile, rounded up to whole XCD rounds (the extra
         // workgroups find no tile and return)
         g1 = kp.nblocks;
@@ -615,7 +618,7 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
         g1 -= g1 % nx;
         jp.xcds = nx;
       }
-      if (global) {
+      if (gjit) {
         js = frame_scratch(g1 * (size_t)P->nplanes * 256u, &jp.scratch);
         jp.sstride = (uint64_t)P->nplanes * 64u;
       }
@@ -625,6 +628,10 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
     const char *req = getenv("QCF_REQUIRE_JIT");
     if (req && req[0] == '1') return js;
     clear_error();
+  }
+  if (global) {
+    QCF_TRY(frame_scratch(grid * (size_t)P->nplanes * 256u, &kp.scratch));
+    kp.sstride = P->nplanes * 64u;
   }
   if (masks && !exp) return launch_pf<kReplay>(kp, lds_bytes, (uint32_t)grid, stream);
   if (exp) return launch_pf<kExport>(kp, lds_bytes, (uint32_t)grid, stream);

@@ -73,8 +73,6 @@ struct FlipG {
 };
 )QCFJIT";
 
-std::string plane(uint32_t p) { return "L[" + std::to_string(p * 64u) + "u + l]"; }
-
 // Fields of an instruction that name record columns (shifted from one loop
 // iteration to the next), and fields the specialised kernel ignores (replay
 // plane numbers).  Everything else must repeat exactly.
@@ -232,6 +230,48 @@ size_t emitted_size(const qcf_program *P, int variant) {
 // exchange-with-zero per plane (so the buffer is zero again for the next
 // group).  Longer programs keep the planes in LDS (L[p*64+l]) as the
 // interpreter does and the events XOR straight into them.
+// Planes instruction code[i] reads or writes (a noise group: its sites').
+void instr_planes(const std::vector<Instr> &code, size_t i, std::vector<uint32_t> *out) {
+  const Instr &in = code[i];
+  auto use = [&](uint32_t p) {
+    if (p != kNone) out->push_back(p);
+  };
+  switch (in.op & 0xFFu) {
+    case I_XORN: {
+      const uint32_t *w = &in.a;
+      for (uint32_t k = 0; k < (in.op >> 8); ++k) use(w[k] >> 16), use(w[k] & 0xFFFFu);
+      break;
+    }
+    case I_NOISEG:
+      for (uint32_t k = 1; k <= (in.op >> 8) && i + k < code.size(); ++k) {
+        const Instr &s = code[i + k];
+        use(s.a & 0xFFFFu), use(s.a >> 16);
+        if (s.b) use(s.b & 0xFFFFu), use(s.b >> 16);
+      }
+      break;
+    case I_MR: use(in.a), use(in.b), use(in.e); break;
+    case I_MRX: use(in.a), use(in.b), use(in.e), use(in.f); break;
+    case I_XSTORE2: use(in.a), use(in.b), use(in.d); break;
+    case I_MEAS: use(in.a), use(in.c); break;
+    case I_RESETZ:
+    case I_INIT:
+    case I_LIN1: use(in.a), use(in.b); break;
+    case I_ACC_LDS:
+    case I_ACCX_LDS:
+    case I_LOAD:
+    case I_STORE:
+    case I_ZERO: use(in.a); break;
+    case I_ACC_STORE: use(in.b); break;
+    default: break;
+  }
+}
+
+// Split mode: LDS of one workgroup (gfx950: 160 KB) and the most planes it
+// moves to registers (the rest of the 512 a wave may hold is left to the
+// kernel's own values).
+constexpr uint32_t kSplitLdsBytes = 160u * 1024u;
+constexpr uint32_t kSplitMaxRegs = 400u;
+
 bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds_bytes) {
   std::vector<Instr> code;
   for (const Instr &in : variant_code(P, variant))
@@ -249,9 +289,11 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   } else {
     ranges = {{0, code.size(), false}};
   }
-  // planes in global memory when the launcher found the frame too large for
-  // LDS (variant bit 2), else registers for small programs, else LDS
-  const bool glob = (variant & 4) != 0;
+  // frames too large for LDS (variant bit 2): with bit 4 the planes used most
+  // stay in registers and the rest in LDS (split mode), else every plane in
+  // global memory; fitting frames: registers for small programs, else LDS
+  const bool split = (variant & 16) != 0;
+  const bool glob = !split && (variant & 4) != 0;
   // variant bit 3: the record spans less than 4 GB, so a record address is
   // one 64-bit base plus a 32-bit byte offset (one scalar multiply per
   // store instead of a 64-bit multiply-add)
@@ -265,7 +307,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
       P->classes.empty() ? 0.0 : (double)P->sites.size() * kBlockShots * P->classes[0].p;
   // noiseless programs take no exchanges at all and gain the occupancy of
   // registers (d=5 x 25: 0.295 vs 0.395 ms)
-  const bool regs = !glob && P->nplanes <= jit_reg_planes() &&
+  const bool regs = !glob && !split && P->nplanes <= jit_reg_planes() &&
                     (P->classes.empty() || ev_per_tile >= jit_reg_events() ||
                      2u * P->nq * 256u > 40u * 1024u);
   const bool nt = jit_nt_stores();
@@ -273,7 +315,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   // site names) in registers: only the qubit planes take LDS, so more waves
   // fit on a CU
   const uint32_t qpl = 2u * P->nq;
-  bool hyb = !regs && !glob && P->nplanes > qpl;
+  bool hyb = !regs && !glob && !split && P->nplanes > qpl;
   for (size_t i = 0; hyb && i < code.size(); ++i) {
     if ((code[i].op & 0xFFu) != I_NOISEG) continue;
     const uint32_t G = code[i].op >> 8;
@@ -285,8 +327,58 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     }
     i += G;
   }
-  auto in_reg = [&](uint32_t p) { return regs || (hyb && p >= qpl); };
   const uint32_t kSlots = jit_slots();  // noise slots per event pass in register mode
+  // Split mode: sreg[p] marks the planes kept in registers, lidx[p] gives the
+  // others their LDS plane (packed), and the noise slots of register planes
+  // follow the last LDS plane.  A plane's score is the operations that read
+  // or write it (each an LDS access if it stayed in LDS) less its noise sites
+  // (each an LDS exchange if it moved to a register); the highest scores take
+  // the registers, as many as the LDS budget needs and jit_split_regs() asks.
+  std::vector<uint8_t> sreg(P->nplanes, 0);
+  std::vector<uint32_t> lidx(P->nplanes, 0);
+  uint32_t nlds = 0;
+  if (split) {
+    std::vector<int64_t> score(P->nplanes, 0);
+    bool noisy = false;
+    std::vector<uint32_t> pl;
+    for (size_t i = 0; i < code.size(); ++i) {
+      pl.clear();
+      instr_planes(code, i, &pl);
+      const bool noise = (code[i].op & 0xFFu) == I_NOISEG;
+      for (uint32_t p : pl)
+        if (p < P->nplanes) score[p] += noise ? -1 : 1;
+      if (noise) {
+        noisy = true;
+        i += code[i].op >> 8;
+      }
+    }
+    const uint32_t budget = kSplitLdsBytes / 256u - (noisy ? kSlots : 0u);
+    const uint32_t kmin = P->nplanes > budget ? P->nplanes - budget : 0u;
+    const uint32_t k = std::min(P->nplanes, std::max(kmin, jit_split_regs()));
+    if (k > kSplitMaxRegs) return false;
+    std::vector<uint32_t> order(P->nplanes);
+    for (uint32_t p = 0; p < P->nplanes; ++p) order[p] = p;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return score[x] > score[y]; });
+    for (uint32_t j = 0; j < k; ++j) sreg[order[j]] = 1;
+    for (uint32_t p = 0; p < P->nplanes; ++p)
+      if (!sreg[p]) lidx[p] = nlds++;
+  }
+  auto in_reg = [&](uint32_t p) { return regs || (hyb && p >= qpl) || (split && sreg[p]); };
+  // LDS plane of an LDS-resident plane
+  auto lp_of = [&](uint32_t p) { return split ? lidx[p] : p; };
+  const uint32_t slot_base = split ? nlds : 0u;  // first noise slot
+  // LDS plane q: ds instructions take a 16-bit byte offset, so planes past
+  // the first 64 KB window (256 planes) are addressed from an opaque per-window
+  // base (lb<w> = w * 16384 + l); a single base register per window then
+  // serves every access in it, where per-access addresses would be hoisted
+  // out of the loops and spilled
+  auto plane = [](uint32_t q) {
+    const uint32_t w = q / 256u, off = (q % 256u) * 64u;
+    if (w == 0) return "L[" + std::to_string(off) + "u + l]";
+    return "L[lb" + std::to_string(w) + " + " + std::to_string(off) + "u]";
+  };
+  const size_t split_every = jit_split_barrier();
   constexpr size_t kBarrierOps = 30;  // operands per compiler barrier
   constexpr size_t kEntryRegs = 48;  // site-entry registers a kernel may hold
   struct Group {
@@ -308,11 +400,20 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
         Group g{(uint32_t)sites.size(), 0u, in.a + k, 0u};
         while (k < G) {
           const Instr &s = code[i + 1 + k];
-          const uint32_t need = s.b ? 4u : 2u;
-          if (regs && g.slots + need > kSlots) break;
+          const uint32_t np = s.b ? 4u : 2u;
+          const uint32_t spl[4] = {s.a & 0xFFFFu, s.a >> 16, s.b & 0xFFFFu, s.b >> 16};
+          uint32_t need = regs ? np : 0u;
+          if (split)
+            for (uint32_t j = 0; j < np; ++j) need += sreg[spl[j]];
+          if ((regs || split) && g.count > 0 && g.slots + need > kSlots) break;
           if (regs) {
             const uint32_t b0 = g.slots;
             sw.push_back({b0 | (b0 + 1u) << 16, s.b ? ((b0 + 2u) | (b0 + 3u) << 16) : 0u, s.d, s.e});
+          } else if (split) {
+            // LDS plane, or (register planes) the next slot after the LDS planes
+            uint32_t ix[4] = {0u, 0u, 0u, 0u}, nx = nlds + g.slots;
+            for (uint32_t j = 0; j < np; ++j) ix[j] = sreg[spl[j]] ? nx++ : lidx[spl[j]];
+            sw.push_back({ix[0] | ix[1] << 16, s.b ? (ix[2] | ix[3] << 16) : 0u, s.d, s.e});
           } else {
             sw.push_back({s.a, s.b, s.d, s.e});
           }
@@ -394,7 +495,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   auto PL = [&](uint32_t p) -> std::string {
     if (in_reg(p)) return "r" + std::to_string(p);
     if (glob) return "gld(" + gaddr(p) + ")";
-    return plane(p);
+    return plane(lp_of(p));
   };
   auto WR = [&](uint32_t p, const std::string &v) -> std::string {
     if (glob) return "gst(" + gaddr(p) + ", " + v + ");";
@@ -403,9 +504,9 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   auto XOR = [&](uint32_t p, const std::string &v) -> std::string {
     if (in_reg(p)) return PL(p) + " ^= " + v + ";";
     if (glob) return "gxr(" + gaddr(p) + ", " + v + ");";
-    return "atomicXor(&" + plane(p) + ", " + v + ");";
+    return "atomicXor(&" + plane(lp_of(p)) + ", " + v + ");";
   };
-  auto slot = [](uint32_t k) { return "L[" + std::to_string(k * 64u) + "u + l]"; };
+  auto slot = [&](uint32_t k) { return plane(slot_base + k); };
   auto load_group = [&](size_t g, const char *pfx) {
     std::ostringstream o;
     const uint32_t off = groups[g].site0, G = groups[g].count;
@@ -415,11 +516,14 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     return o.str();
   };
   auto emit = [&](bool tail, std::ostringstream &body) -> bool {
-  if (regs || hyb) {
-    const uint32_t p0 = regs ? 0u : qpl;
-    body << "    uint32_t r" << p0 << " = 0u";
-    for (uint32_t p = p0 + 1; p < P->nplanes; ++p) body << ", r" << p << " = 0u";
-    body << ";\n";
+  if (regs || hyb || split) {
+    const char *sep = "    uint32_t ";
+    for (uint32_t p = 0; p < P->nplanes; ++p)
+      if (in_reg(p)) {
+        body << sep << "r" << p << " = 0u";
+        sep = ", ";
+      }
+    if (*sep == ',') body << ";\n";
   }
   if (!groups.empty() && !preload) body << "    uint32_t t0 = 0u, t1 = 0u, t2 = 0u, t3 = 0u;\n";
   for (size_t ri = 0; ri < ranges.size(); ++ri) {
@@ -457,6 +561,28 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
       const Instr &in = code[i];
       const uint32_t op = in.op & 0xFFu;
       const uint32_t a = in.a, b = in.b, c = in.c;
+      // split mode: every so many operations all register planes pass a
+      // compiler barrier, so the scheduler cannot stretch live ranges (early
+      // LDS reads, reassociated XOR chains) over a whole straight-line stretch
+      if (split && split_every && i > rg.b && (i - rg.b) % split_every == 0) {
+        std::vector<uint32_t> rp;
+        if (jit_split_touched()) {
+          std::vector<uint32_t> pl;
+          for (size_t j = i - split_every; j < i; ++j) instr_planes(code, j, &pl);
+          for (uint32_t p : pl)
+            if (p < P->nplanes && sreg[p] && std::find(rp.begin(), rp.end(), p) == rp.end())
+              rp.push_back(p);
+        } else {
+          for (uint32_t p = 0; p < P->nplanes; ++p)
+            if (sreg[p]) rp.push_back(p);
+        }
+        for (size_t k0 = 0; k0 < rp.size(); k0 += kBarrierOps) {
+          body << "    asm volatile(\"\" :";
+          for (size_t k = k0; k < std::min(rp.size(), k0 + kBarrierOps); ++k)
+            body << (k > k0 ? ", " : " ") << "\"+v\"(r" << rp[k] << ")";
+          body << ");\n";
+        }
+      }
       switch (op) {
         case I_XORN: {
           const uint32_t n = in.op >> 8;
@@ -490,16 +616,17 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
                    << "u, 0u, t0, t1, t2, t3, 0u, P.c, flip);\n";
               if (gi + 1 < gend) body << "    t0 = n0; t1 = n1; t2 = n2; t3 = n3;\n";
             }
-            if (regs) {
+            if (regs || split) {
               std::vector<uint32_t> touched;
               for (uint32_t k = 0; k < g.count; ++k) {
                 const Instr &s = *sites[g.site0 + k];
-                const uint32_t s0 = site_slot[g.site0 + k];
+                uint32_t s0 = site_slot[g.site0 + k];
                 const uint32_t pl[4] = {s.a & 0xFFFFu, s.a >> 16, s.b & 0xFFFFu, s.b >> 16};
                 const uint32_t np = s.b ? 4u : 2u;
                 body << "   ";
                 for (uint32_t j = 0; j < np; ++j) {
-                  body << " " << PL(pl[j]) << " ^= atomicExch(&" << slot(s0 + j) << ", 0u);";
+                  if (!in_reg(pl[j])) continue;  // split mode: events went straight to LDS
+                  body << " " << PL(pl[j]) << " ^= atomicExch(&" << slot(s0++) << ", 0u);";
                   if (std::find(touched.begin(), touched.end(), pl[j]) == touched.end())
                     touched.push_back(pl[j]);
                 }
@@ -607,7 +734,14 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
       << (nsite ? tab.str() : std::string("0u,0u,0u,0u")) << "};\n";
   src << "extern \"C\" __global__ void __launch_bounds__(64) qcf_jit_kernel(KJ P) {\n"
       << "  extern __shared__ uint32_t L[];\n"
-      << "  const uint32_t l = threadIdx.x;\n"
+      << "  const uint32_t l = threadIdx.x;\n";
+  {
+    const uint32_t nl = glob ? 0u : split ? nlds + max_slots : regs ? max_slots : hyb ? qpl : P->nplanes;
+    for (uint32_t w = 1; w * 256u < nl; ++w)
+      src << "  uint32_t lb" << w << " = " << w * 16384u << "u + l;\n  asm volatile(\"\" : \"+v\"(lb"
+          << w << "));\n";
+  }
+  src
       << (glob ? "  uint32_t *const F = P.scratch + (uint64_t)blockIdx.x * P.sstride;\n"
                  "  (void)L;\n  const FlipG flip{F};\n"
                : "  const FlipLDS flip{L};\n");
@@ -622,8 +756,8 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
       src << ");\n";
     }
   }
-  if (regs)  // the slot buffer starts at zero and every pass leaves it at zero
-    src << "  for (uint32_t k = 0; k < " << std::max<uint32_t>(max_slots, 1u)
+  if (regs || (split && max_slots))  // the slot buffer starts at zero and every pass leaves it at zero
+    src << "  for (uint32_t k = " << slot_base << "u; k < " << slot_base + std::max<uint32_t>(max_slots, 1u)
         << "u; ++k) L[k * 64u + l] = 0u;\n";
   // Tiles are dealt out per XCD: with X = P.xcds > 1 the grid is a multiple of
   // X, workgroup i runs on XCD i mod X, and XCD x takes the x-th contiguous
@@ -661,8 +795,9 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   src << "  }\n}\n";
   *out = src.str();
   if (lds_bytes)
-    *lds_bytes = glob   ? 0u
-                 : regs ? std::max<uint32_t>(max_slots, 1u) * 256u
+    *lds_bytes = glob    ? 0u
+                 : split ? (nlds + max_slots) * 256u
+                 : regs  ? std::max<uint32_t>(max_slots, 1u) * 256u
                  : hyb  ? qpl * 256u
                         : P->nplanes * 256u;
   return true;
@@ -726,6 +861,35 @@ uint32_t jit_reg_planes() {
 size_t jit_max_instr() {
   const char *e = getenv("QCF_JIT_MAX_INSTR");
   return e ? (size_t)strtoull(e, nullptr, 10) : (size_t)12000;
+}
+
+// Split mode (frames larger than LDS): planes kept in registers, at least
+// what the LDS budget needs (QCF_JIT_SPLIT_REGS raises it).
+uint32_t jit_split_regs() {
+  const char *e = getenv("QCF_JIT_SPLIT_REGS");
+  return e ? (uint32_t)strtoul(e, nullptr, 10) : 0u;
+}
+
+// Split mode: operations between compiler barriers over the register planes
+// (QCF_JIT_SPLIT_BARRIER, 0 = none; the d=11 reference circuit at 2^19 shots
+// took 0.30 ms at 16 and 64, 0.33 ms at 32, 0.38 ms at 128).
+size_t jit_split_barrier() {
+  const char *e = getenv("QCF_JIT_SPLIT_BARRIER");
+  return e ? (size_t)strtoull(e, nullptr, 10) : (size_t)16;
+}
+
+// A split-mode barrier names only the register planes of the operations
+// since the last one (QCF_JIT_SPLIT_TOUCHED=0: every register plane; d=13 x 2
+// rounds: 15 s instead of 119 s of compilation, and no scratch spills).
+bool jit_split_touched() {
+  const char *e = getenv("QCF_JIT_SPLIT_TOUCHED");
+  return !(e && e[0] == '0');
+}
+
+// QCF_JIT_SPLIT=0 leaves frames larger than LDS to the global-memory paths.
+bool jit_split_enabled() {
+  const char *e = getenv("QCF_JIT_SPLIT");
+  return !(e && e[0] == '0');
 }
 
 // QCF_JIT_GLOBAL=1 also specialises programs whose frames live in global memory.
@@ -849,7 +1013,7 @@ extern "C" int qcf_debug_jit_source(const qcf_program *prog, int variant, char *
   qcf::clear_error();
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src;
-  if (!qcf::generate(prog, variant & 15, &src, nullptr))
+  if (!qcf::generate(prog, variant & 31, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   if (needed) *needed = src.size() + 1;
   if (buf && cap) {
@@ -865,7 +1029,7 @@ extern "C" int qcf_debug_jit_compile(const qcf_program *prog, int variant, size_
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src, log;
   std::vector<char> bin;
-  if (!qcf::generate(prog, variant & 15, &src, nullptr))
+  if (!qcf::generate(prog, variant & 31, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   QCF_TRY(qcf::compile(src, &bin, &log));
   if (code_bytes) *code_bytes = bin.size();

@@ -31,6 +31,10 @@ uint32_t jit_reg_planes();
 double jit_reg_events();
 bool jit_no_loops();
 bool jit_global_enabled();
+bool jit_split_enabled();
+uint32_t jit_split_regs();
+size_t jit_split_barrier();
+bool jit_split_touched();
 bool jit_nt_stores();
 uint32_t jit_xcds();
 bool jit_full_grid();

@@ -112,3 +112,49 @@ def test_global_frames_specialised_equals_interpreter(gpu, monkeypatch, d, round
     (ra, fa), (rb, fb) = _both(monkeypatch, run)
     assert np.array_equal(ra, rb)
     assert np.array_equal(fa, fb)
+
+
+@pytest.mark.parametrize("case", ["unrotated_11x3", "rotated_13x2"])
+def test_split_frames_specialised_equals_interpreter(gpu, monkeypatch, case):
+    """Frames too large for LDS run the split kernel (most-used planes in
+    registers, the rest in LDS, noise on register planes through slots): bit
+    for bit the interpreter's global-memory result, partial tile and kept
+    frame included."""
+    if case == "unrotated_11x3":
+        ops, shots = circuits.unrotated_surface_code_benchmark(11, 3), 3 * 2048 + 5
+    else:
+        ops, shots = circuits.rotated_surface_code(13, 2, 5e-3).ops, 2 * 2048 + 77
+    prog = qcf.Program(ops)
+    assert prog.info.lds_bytes > 160 * 1024
+    print(f"split kernel {case}, {shots} shots", flush=True)
+
+    def run():
+        f = qcf.pftrajectories(prog, shots, seed=31)
+        return f.record_host(), f.frame_host()
+
+    (ra, fa), (rb, fb) = _both(monkeypatch, run)
+    assert np.array_equal(ra, rb)
+    assert np.array_equal(fa, fb)
+    assert oracle.unpack_shots(ra, shots).any()
+
+
+def test_split_frames_in_place_equals_interpreter(gpu, monkeypatch):
+    """pftrajectories(state, circuit) on an existing 442-qubit frame."""
+    ops = circuits.unrotated_surface_code_benchmark(11, 2)
+    prog = qcf.Program(ops)
+    n, nbits, shots = prog.nq - 1, prog.nbits, 2048 + 9
+    rng = np.random.default_rng(11)
+    fb = oracle.pack_shots(rng.integers(0, 2, (2 * (n + 1), shots)).astype(np.uint8))
+    mb = oracle.pack_shots(rng.integers(0, 2, (nbits, shots)).astype(np.uint8))
+    print("split kernel in place, unrotated 11 x 2", flush=True)
+
+    def run():
+        f = qcf.PauliFrame(shots, n, nbits, init=False)
+        f.frame.view(2 * (n + 1), f.ld)[:, : fb.shape[1]].copy_(torch.from_numpy(fb.view(np.int64)))
+        f.record.view(nbits, f.ld)[:, : mb.shape[1]].copy_(torch.from_numpy(mb.view(np.int64)))
+        qcf.pftrajectories_state(f, prog, seed=78)
+        return f.record_host(), f.frame_host()
+
+    (ra, fa), (rb, fb2) = _both(monkeypatch, run)
+    assert np.array_equal(ra, rb)
+    assert np.array_equal(fa, fb2)

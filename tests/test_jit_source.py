@@ -1,0 +1,63 @@
+"""Host side of the per-circuit kernels (no GPU needed): which programs the
+split storage mode takes, that its LDS planes fit one compute unit's 160 KB,
+and that its source compiles with hiprtc for gfx950."""
+import ctypes as C
+import re
+
+import pytest
+
+import qcf
+from qcf import _lib, circuits
+
+SPLIT = 1 | 4 | 8 | 16  # fresh frame, frame larger than LDS, 32-bit record offsets, split
+
+
+def source(prog, variant):
+    n = C.c_size_t()
+    _lib.call("qcf_debug_jit_source", prog.handle, variant, None, 0, C.byref(n))
+    buf = C.create_string_buffer(n.value + 1)
+    _lib.call("qcf_debug_jit_source", prog.handle, variant, buf, n.value + 1, C.byref(n))
+    return buf.value.decode()
+
+
+def lds_planes(src):
+    """Largest LDS plane index the kernel addresses, plus one."""
+    hi = 0
+    for w, off in re.findall(r"L\[lb(\d+) \+ (\d+)u\]", src):
+        hi = max(hi, int(w) * 256 + int(off) // 64 + 1)
+    for off in re.findall(r"L\[(\d+)u \+ l\]", src):
+        hi = max(hi, int(off) // 64 + 1)
+    return hi
+
+
+def test_split_source_for_the_reference_benchmark_circuit():
+    prog = qcf.Program(circuits.unrotated_surface_code_benchmark(11, 12))
+    assert prog.info.nplanes == 884 and prog.info.lds_bytes > 160 * 1024
+    src = source(prog, SPLIT)
+    regs = set(re.findall(r"\br(\d+) = 0u", src))
+    n_lds = lds_planes(src)
+    assert n_lds <= 640  # 160 KB of 256-byte planes
+    assert len(regs) + n_lds >= prog.info.nplanes
+    assert "atomicExch" not in src  # noiseless: no slot exchanges
+
+
+def test_split_source_noisy_rotated_code_uses_slots():
+    prog = qcf.Program(circuits.rotated_surface_code(13, 2, 5e-3).ops)
+    src = source(prog, SPLIT)
+    assert lds_planes(src) <= 640
+    assert "atomicExch" in src  # events on register planes pass through slots
+
+
+def test_split_mode_declines_frames_beyond_registers_and_lds():
+    """d=15 needs 1237 planes: more than LDS plus the register budget, so the
+    generator declines and launches fall back to global-memory frames."""
+    prog = qcf.Program(circuits.rotated_surface_code(15, 15, 1e-3).ops)
+    with pytest.raises(_lib.MethodError):
+        source(prog, SPLIT)
+
+
+def test_split_source_compiles():
+    prog = qcf.Program(circuits.unrotated_surface_code_benchmark(11, 2))
+    n = C.c_size_t()
+    _lib.call("qcf_debug_jit_compile", prog.handle, SPLIT, C.byref(n))
+    assert n.value > 0
