@@ -126,15 +126,19 @@ class PauliFrame:
     """PauliFrame(frames, qubits, measurements) (src/pauli_frames.jl:9-37)."""
 
     def __init__(self, frames: int, qubits: int, measurements: int, *, device=None,
-                 init: bool = True):
+                 init: bool = True, _uninitialized: bool = False):
         self.shots = int(frames)
         self.nq = int(qubits) + 1  # hidden ancilla for PauliMeasurement
         self.nbits = int(measurements)
         self.ld = plane_words(self.shots)
         dev = torch.device(device) if device is not None else torch.device(
             "cuda", torch.cuda.current_device())
-        self.frame = torch.zeros(2 * self.nq * self.ld, dtype=torch.int64, device=dev)
-        self.record = torch.zeros(max(1, self.nbits) * self.ld, dtype=torch.int64, device=dev)
+        # a fresh launch defines every word (unwritten record columns are
+        # zeroed, every plane is stored, padding lanes store zeros), so
+        # pftrajectories skips the fill
+        alloc = torch.empty if _uninitialized else torch.zeros
+        self.frame = alloc(2 * self.nq * self.ld, dtype=torch.int64, device=dev)
+        self.record = alloc(max(1, self.nbits) * self.ld, dtype=torch.int64, device=dev)
         if init:
             initZ_(self)
 
@@ -193,7 +197,8 @@ def pftrajectories(circuit, trajectories: int = 5000, threads: bool = True, *,
     if isinstance(circuit, PauliFrame):  # pftrajectories(state, circuit)
         return pftrajectories_state(circuit, trajectories, seed=seed, masks=masks)
     prog = circuit if isinstance(circuit, Program) else Program(circuit)
-    f = PauliFrame(trajectories, prog.nq - 1, prog.nbits, device=device, init=False)
+    f = PauliFrame(trajectories, prog.nq - 1, prog.nbits, device=device, init=False,
+                   _uninitialized=True)
     if not keep_frame:
         f.frame = torch.empty(0, dtype=torch.int64, device=f.record.device)
     flags = 1

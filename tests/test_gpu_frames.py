@@ -450,3 +450,29 @@ def test_wave_scan_is_exact(gpu):
                   qcf.pauli._stream())
     got = out.cpu().numpy().view(np.uint32).reshape(97, 64)
     assert np.array_equal(got, np.cumsum(x.astype(np.uint64), axis=1).astype(np.uint32))
+
+
+@pytest.mark.parametrize("shots", [1, 2049, 5000])
+def test_fresh_launch_defines_every_word(gpu, shots):
+    """pftrajectories allocates its frame and record without filling them:
+    the launch itself writes every word (record columns no operation writes
+    are zeroed, bits past the last shot are zero), so memory left full of ones
+    by an earlier tensor gives the same result as zeroed buffers."""
+    rng = np.random.default_rng(shots)
+    n, nbits = 6, 12
+    circ = random_circuit(rng, n, 8, 100)  # bits 9..12 are never written
+    prog = qcf.Program(circ, nqubits=n, nbits=nbits)
+    ld = qcf.frames.plane_words(shots)
+    junk = torch.full((4 * (2 * (n + 1) + nbits) * ld,), -1, dtype=torch.int64, device="cuda")
+    del junk  # the caching allocator hands this block out again
+    f = qcf.pftrajectories(prog, shots, seed=12)
+    zf = torch.zeros(2 * (n + 1) * ld, dtype=torch.int64, device="cuda")
+    zr = torch.zeros(nbits * ld, dtype=torch.int64, device="cuda")
+    qcf._lib.call("qcf_pf_sample", prog.handle, shots, 12, qcf.frames._ptr(zf),
+                  qcf.frames._ptr(zr), None, ld, 1, qcf.frames._stream())
+    torch.cuda.synchronize()
+    assert torch.equal(f.record, zr) and torch.equal(f.frame, zf)
+    rec = f.record_host()
+    assert not rec[8:].any()
+    bits = np.unpackbits(rec.view(np.uint8), axis=1, bitorder="little")
+    assert not bits[:, shots:].any()
