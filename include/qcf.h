@@ -266,11 +266,16 @@ typedef struct qcf_tune {
  * per pftrajectories call): `n` compiled circuits, circuit k runs shots[k]
  * fresh frames with seed seeds[k] into meas[k] (its nbits columns of ld[k]
  * words).  qcf_batch_create uploads the per-circuit parameters and the work
- * queue once; qcf_batch_launch enqueues ONE persistent kernel whose
- * workgroups pull (circuit, tile range) items from a device queue, and
- * allocates nothing, so it can be captured in a hipGraph;
- * qcf_batch_launch_graph captures that launch once and replays the graph.
- * Results are bit-identical to n qcf_pf_sample(..., QCF_PF_FRESH) calls.
+ * queue once and compiles the circuits' specialised kernels.
+ * qcf_batch_launch first runs the specialised kernels, on four helper
+ * streams forked from `stream` and joined back into it.  It then enqueues
+ * ONE persistent kernel for the circuits without one (frames beyond what
+ * registers plus LDS hold); its workgroups pull (circuit, tile range) items
+ * from a device queue.  The launch allocates and compiles nothing, so it
+ * can be captured in a hipGraph; qcf_batch_launch_graph captures it once and
+ * replays the graph.  QCF_BATCH_JIT=0 sends every circuit to the persistent
+ * kernel.  Results are bit-identical to n qcf_pf_sample(..., QCF_PF_FRESH)
+ * calls.
  */
 typedef struct qcf_batch qcf_batch;
 int qcf_batch_create(const qcf_program *const *progs, size_t n,

@@ -518,6 +518,66 @@ bool force_global_frames() {
 
 }  // namespace
 
+// Which kernel a native launch of kp would run if specialised, with its
+// arguments and grid (nothing is launched).  Frames too large for LDS take
+// the split kernel: the planes used most in registers, the rest in LDS, one
+// wave per CU.  With QCF_JIT_GLOBAL=1 (or when the split kernel cannot be
+// built) they keep every plane in global memory: the specialised kernel
+// there is opt-in (global_planes: the caller provides the plane storage),
+// since both it and the interpreter are bound by L2 latency and it measured
+// no faster for d >= 9 while costing seconds of compilation.  Returns an
+// error (QCF_ERR_UNSUPPORTED) when the launch has no specialised kernel.
+int jit_plan(qcf_program *P, const KParams &kp, uint64_t shots, int variant, bool global,
+             uint32_t waves_per_cu, size_t ld, JitPlan *out) {
+  const bool try_split =
+      global && !force_global_frames() && !jit_global_enabled() && jit_split_enabled();
+  if (jit_disabled() || (global && !try_split && !jit_global_enabled()))
+    return set_error(QCF_ERR_UNSUPPORTED, "no specialised kernel for this launch");
+  DeviceState *st;
+  QCF_TRY(device_state(&st));
+  JitKernelInfo jk;
+  // records under 4 GB take 32-bit byte offsets from one base
+  const bool rec32 = (uint64_t)std::max<uint32_t>(P->nbits, 1u) * ld * 8ull < (1ull << 32);
+  QCF_TRY(jit_get(P, variant | (global ? 4 : 0) | (try_split ? 16 : 0) | (rec32 ? 8 : 0), &jk));
+  const bool gjit = global && !try_split;  // specialised kernel on global planes
+  JitParams jp{};
+  jp.rec = kp.rec;
+  jp.frame = kp.frame;
+  jp.seed = kp.seed;
+  jp.ld32 = kp.ld32;
+  jp.nblocks = kp.nblocks;
+  jp.nw32 = kp.nw32;
+  jp.nwbuf = kp.nwbuf;
+  jp.lastmask = kp.lastmask;
+  jp.c = kp.cls_c[0];
+  jp.has_class = kp.nclasses > 0 ? 1u : 0u;
+  jp.nfull = (uint32_t)(shots / kBlockShots);
+  uint32_t jwpc = waves_per_cu ? waves_per_cu : jk.blocks_per_cu;
+  if (gjit)
+    jwpc = global_waves_per_cu() ? global_waves_per_cu()
+                                 : std::min<uint32_t>(jwpc, kGlobalJitWavesPerCU);
+  uint64_t g1 = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * jwpc);
+  const uint32_t nx = jit_xcds();
+  if (!gjit && jit_full_grid()) {
+    // one workgroup per tile, rounded up to whole XCD rounds (the extra
+    // workgroups find no tile and return)
+    g1 = kp.nblocks;
+    if (nx > 1 && g1 >= nx) {
+      g1 += (nx - g1 % nx) % nx;
+      jp.xcds = nx;
+    }
+  } else if (nx > 1 && g1 >= nx) {
+    g1 -= g1 % nx;
+    jp.xcds = nx;
+  }
+  out->function = jk.function;
+  out->jp = jp;
+  out->grid = (uint32_t)g1;
+  out->lds_bytes = jk.lds_bytes;
+  out->global_planes = gjit;
+  return QCF_OK;
+}
+
 // Shared with the batched launcher.
 int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
                   uint64_t *frame, uint64_t *meas, uint64_t *masks, size_t ld,
@@ -569,62 +629,18 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
     wpc = global ? (global_waves_per_cu() ? global_waves_per_cu() : kGlobalWavesPerCU)
                  : std::max<uint32_t>(1, std::min<uint32_t>(16, (uint32_t)lds_max / std::max(1u, lds_bytes)));
   const uint64_t grid = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * wpc);
-  // Native launches of on-chip frames run the kernel specialised to this
-  // program (the partial last tile included); the interpreter is the general
+  // Native launches run the kernel specialised to this program when it has
+  // one (the partial last tile included); the interpreter is the general
   // path.  Both draw the same events for the same shots.
-  // Frames too large for LDS run the split kernel: the planes used most in
-  // registers, the rest in LDS, one wave per CU.  With QCF_JIT_GLOBAL=1 (or
-  // when the split kernel cannot be built) they keep every plane in global
-  // memory: the specialised kernel there is opt-in, since both it and the
-  // interpreter are bound by L2 latency and it measured no faster for d >= 9
-  // while costing seconds of compilation.
-  const bool try_split = global && !force_global_frames() && !jit_global_enabled() &&
-                         jit_split_enabled();
-  if (!masks && !exp && !jit_disabled() && (!global || try_split || jit_global_enabled())) {
-    JitKernelInfo jk;
-    // records under 4 GB take 32-bit byte offsets from one base
-    const bool rec32 = (uint64_t)std::max<uint32_t>(P->nbits, 1u) * ld * 8ull < (1ull << 32);
-    int js = try_split ? jit_get(P, variant | 4 | 16 | (rec32 ? 8 : 0), &jk)
-                       : jit_get(P, variant | (global ? 4 : 0) | (rec32 ? 8 : 0), &jk);
-    const bool gjit = global && !try_split;  // specialised kernel on global planes
-    if (js == QCF_OK) {
-      JitParams jp{};
-      jp.rec = kp.rec;
-      jp.frame = kp.frame;
-      jp.seed = kp.seed;
-      jp.ld32 = kp.ld32;
-      jp.nblocks = kp.nblocks;
-      jp.nw32 = kp.nw32;
-      jp.nwbuf = kp.nwbuf;
-      jp.lastmask = kp.lastmask;
-      jp.c = kp.cls_c[0];
-      jp.has_class = kp.nclasses > 0 ? 1u : 0u;
-      jp.nfull = (uint32_t)(shots / kBlockShots);
-      uint32_t jwpc = waves_per_cu ? waves_per_cu : jk.blocks_per_cu;
-      if (gjit)
-        jwpc = global_waves_per_cu() ? global_waves_per_cu()
-                                     : std::min<uint32_t>(jwpc, kGlobalJitWavesPerCU);
-      uint64_t g1 = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * jwpc);
-      const uint32_t nx = jit_xcds();
-      if (!gjit && jit_full_grid()) {
-        // one workgroup per tile, rounded up to whole XCD rounds (the extra
-        // workgroups find no tile and return)
-        g1 = kp.nblocks;
-        if (nx > 1 && g1 >= nx) {
-          g1 += (nx - g1 % nx) % nx;
-          jp.xcds = nx;
-        }
-      } else if (nx > 1 && g1 >= nx) {
-        g1 -= g1 % nx;
-        jp.xcds = nx;
-      }
-      if (gjit) {
-        js = frame_scratch(g1 * (size_t)P->nplanes * 256u, &jp.scratch);
-        jp.sstride = (uint64_t)P->nplanes * 64u;
-      }
-      if (js == QCF_OK) js = jit_launch(jk.function, jp, (uint32_t)g1, jk.lds_bytes, stream);
-      if (js == QCF_OK) return QCF_OK;
+  if (!masks && !exp) {
+    JitPlan pl;
+    int js = jit_plan(P, kp, shots, variant, global, waves_per_cu, ld, &pl);
+    if (js == QCF_OK && pl.global_planes) {
+      js = frame_scratch(pl.grid * (size_t)P->nplanes * 256u, &pl.jp.scratch);
+      pl.jp.sstride = (uint64_t)P->nplanes * 64u;
     }
+    if (js == QCF_OK) js = jit_launch(pl.function, pl.jp, pl.grid, pl.lds_bytes, stream);
+    if (js == QCF_OK) return QCF_OK;
     const char *req = getenv("QCF_REQUIRE_JIT");
     if (req && req[0] == '1') return js;
     clear_error();
@@ -728,10 +744,18 @@ extern "C" int qcf_frame_from_reference(const uint64_t *xzs, uint64_t shots, uin
 // ======================================================================
 // Batched persistent launch + autotuner
 // ======================================================================
+// Circuits that have a specialised kernel (frames in LDS, or split between
+// registers and LDS) run it, spread over helper streams forked from the
+// caller's stream; the persistent interpreter kernel takes the rest (frames
+// beyond split mode) on the caller's stream.  Both draw identical bits.
+constexpr int kBatchStreams = 4;
 struct qcf_batch {
   int device = -1;
   size_t n = 0;
   std::vector<qcf::KParams> params;
+  std::vector<uint8_t> on_jit;          // circuit k runs its specialised kernel
+  std::vector<qcf::JitPlan> jit;        // those launches, costliest first
+  std::vector<int> jit_stream;          // helper stream of each launch
   std::vector<qcf::WorkItem> items;
   qcf::KParams *d_params = nullptr;
   qcf::WorkItem *d_items = nullptr;
@@ -739,11 +763,18 @@ struct qcf_batch {
   uint32_t *d_frames = nullptr;
   uint32_t lds_bytes = 0;
   uint32_t grid = 0;
+  hipStream_t helpers[kBatchStreams] = {};
+  hipEvent_t fork = nullptr, join[kBatchStreams] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   ~qcf_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    for (int h = 0; h < kBatchStreams; ++h) {
+      if (helpers[h]) (void)hipStreamDestroy(helpers[h]);
+      if (join[h]) (void)hipEventDestroy(join[h]);
+    }
+    if (fork) (void)hipEventDestroy(fork);
     if (d_params) (void)hipFree(d_params);
     if (d_items) (void)hipFree(d_items);
     if (d_queue) (void)hipFree(d_queue);
@@ -785,6 +816,7 @@ void make_items(qcf_batch *B, uint32_t tile_blocks) {
   B->items.clear();
   std::vector<std::pair<double, WorkItem>> tmp;
   for (size_t k = 0; k < B->n; ++k) {
+    if (B->on_jit[k]) continue;
     const KParams &kp = B->params[k];
     for (uint32_t b0 = 0; b0 < kp.nblocks; b0 += tile_blocks) {
       const uint32_t nb = std::min(tile_blocks, kp.nblocks - b0);
@@ -832,12 +864,74 @@ int set_tune(qcf_batch *B, const qcf_tune *tune) {
   return QCF_OK;
 }
 
+// QCF_BATCH_JIT=0 keeps every circuit on the persistent interpreter kernel.
+bool batch_jit_enabled() {
+  const char *e = getenv("QCF_BATCH_JIT");
+  return !(e && e[0] == '0');
+}
+
+// QCF_BATCH_JIT_ORDER=concurrent starts the persistent kernel beside the
+// specialised launches instead of after them.
+bool batch_concurrent() {
+  const char *e = getenv("QCF_BATCH_JIT_ORDER");
+  return e && e[0] == 'c';
+}
+
+// Specialised kernels of the circuits that have one, costliest first, each
+// on the helper stream with the least work so far.  No allocation and no
+// compilation happen here (both are done when the batch is created), so the
+// sequence can be captured into a graph.
+int plan_jit(qcf_batch *B, const std::vector<double> &cost) {
+  std::vector<size_t> order;
+  for (size_t k = 0; k < B->n; ++k)
+    if (B->on_jit[k]) order.push_back(k);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+  std::vector<qcf::JitPlan> plans(B->jit);  // indexed by circuit while planning
+  B->jit.clear();
+  B->jit_stream.clear();
+  double load[kBatchStreams] = {};
+  for (size_t k : order) {
+    int h = 0;
+    for (int j = 1; j < kBatchStreams; ++j)
+      if (load[j] < load[h]) h = j;
+    load[h] += cost[k];
+    B->jit.push_back(plans[k]);
+    B->jit_stream.push_back(h);
+  }
+  if (!B->jit.empty()) {
+    for (int h = 0; h < kBatchStreams; ++h) {
+      QCF_HIP_CHECK(hipStreamCreateWithFlags(&B->helpers[h], hipStreamNonBlocking));
+      QCF_HIP_CHECK(hipEventCreateWithFlags(&B->join[h], hipEventDisableTiming));
+    }
+    QCF_HIP_CHECK(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+  }
+  return QCF_OK;
+}
+
 int batch_launch(qcf_batch *B, hipStream_t s) {
-  if (B->items.empty()) return QCF_OK;
-  QCF_TRY(allow_lds((const void *)pf_batch_kernel, B->lds_bytes));
-  hipLaunchKernelGGL(pf_batch_kernel, dim3(B->grid), dim3(64), B->lds_bytes, s, B->d_params,
-                     B->d_items, (uint32_t)B->items.size(), B->d_queue);
-  QCF_HIP_CHECK(hipGetLastError());
+  const bool conc = batch_concurrent();
+  auto persistent = [&]() -> int {
+    if (B->items.empty()) return QCF_OK;
+    QCF_TRY(allow_lds((const void *)pf_batch_kernel, B->lds_bytes));
+    hipLaunchKernelGGL(pf_batch_kernel, dim3(B->grid), dim3(64), B->lds_bytes, s, B->d_params,
+                       B->d_items, (uint32_t)B->items.size(), B->d_queue);
+    QCF_HIP_CHECK(hipGetLastError());
+    return QCF_OK;
+  };
+  if (B->jit.empty()) return persistent();
+  QCF_HIP_CHECK(hipEventRecord(B->fork, s));
+  for (int h = 0; h < kBatchStreams; ++h) QCF_HIP_CHECK(hipStreamWaitEvent(B->helpers[h], B->fork, 0));
+  for (size_t i = 0; i < B->jit.size(); ++i) {
+    const JitPlan &pl = B->jit[i];
+    QCF_TRY(jit_launch(pl.function, pl.jp, pl.grid, pl.lds_bytes, B->helpers[B->jit_stream[i]]));
+  }
+  if (conc) QCF_TRY(persistent());
+  for (int h = 0; h < kBatchStreams; ++h) {
+    QCF_HIP_CHECK(hipEventRecord(B->join[h], B->helpers[h]));
+    QCF_HIP_CHECK(hipStreamWaitEvent(s, B->join[h], 0));
+  }
+  if (!conc) QCF_TRY(persistent());
   return QCF_OK;
 }
 
@@ -863,6 +957,10 @@ extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
     return st;
   }
   B->params.resize(n);
+  B->on_jit.assign(n, 0);
+  B->jit.assign(n, JitPlan{});
+  std::vector<double> cost(n, 0.0);
+  const bool use_jit = batch_jit_enabled() && !force_global_frames();
   for (size_t k = 0; k < n; ++k) {
     if (!progs[k]) {
       delete B;
@@ -873,6 +971,24 @@ extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
       delete B;
       return st;
     }
+    const KParams &kp = B->params[k];
+    if (!use_jit || kp.nblocks == 0) continue;
+    // compiled here (once per distinct kernel source), launched later
+    qcf_program *P = const_cast<qcf_program *>(progs[k]);
+    const bool big = (int)(P->nplanes * 256u) > lds_max;
+    JitPlan pl;
+    if (jit_plan(P, kp, shots[k], 1, big, 0, ld[k], &pl) == QCF_OK && !pl.global_planes) {
+      B->on_jit[k] = 1;
+      B->jit[k] = pl;
+      // tile passes of the launch times the program length
+      cost[k] = (double)kp.ninstr * ((kp.nblocks + pl.grid - 1) / std::max(1u, pl.grid));
+    } else {
+      clear_error();
+    }
+  }
+  if (int st = plan_jit(B, cost); st != QCF_OK) {
+    delete B;
+    return st;
   }
   // circuits whose frame does not fit in LDS keep their planes in a
   // batch-owned global buffer: one slot of the largest such frame per wave
@@ -880,6 +996,7 @@ extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
   if (tune && tune->lds_cap) lds_max = std::min<int>(lds_max, (int)tune->lds_cap);
   uint32_t gl_planes = 0;
   for (size_t k = 0; k < n; ++k) {
+    if (B->on_jit[k]) continue;
     const uint32_t lb = progs[k]->nplanes * 256u;
     if ((int)lb > lds_max || force_gl)
       gl_planes = std::max(gl_planes, progs[k]->nplanes);
@@ -899,7 +1016,7 @@ extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
     }
     for (size_t k = 0; k < n; ++k) {
       const uint32_t lb = progs[k]->nplanes * 256u;
-      if ((int)lb > lds_max || force_gl) {
+      if (!B->on_jit[k] && ((int)lb > lds_max || force_gl)) {
         B->params[k].scratch = B->d_frames;
         B->params[k].sstride = gl_planes * 64u;
       }

@@ -51,6 +51,15 @@ struct JitKernelInfo {
 // use, cached in the program).  Returns an error if it cannot be built.
 int jit_get(qcf_program *P, int variant, JitKernelInfo *info);
 
+// A specialised launch ready to go: kernel, arguments, grid and LDS.
+struct JitPlan {
+  void *function = nullptr;
+  JitParams jp{};
+  uint32_t grid = 0;
+  uint32_t lds_bytes = 0;
+  bool global_planes = false;  // planes in global memory: jp.scratch still to be set
+};
+
 int jit_launch(void *function, const JitParams &p, uint32_t grid, uint32_t lds_bytes,
                hipStream_t stream);
 

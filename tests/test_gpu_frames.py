@@ -383,7 +383,12 @@ def test_reference_frame_layout_roundtrip(gpu):
     assert np.array_equal(g.frame_host(), f.frame_host())
 
 
-def test_batched_equals_single_launches(gpu):
+@pytest.mark.parametrize("batch_jit", ["1", "0"])
+def test_batched_equals_single_launches(gpu, monkeypatch, batch_jit):
+    """Batched records equal single launches, with the circuits' specialised
+    kernels on helper streams (the default) and with every circuit on the
+    persistent interpreter kernel (QCF_BATCH_JIT=0)."""
+    monkeypatch.setenv("QCF_BATCH_JIT", batch_jit)
     rng = np.random.default_rng(4)
     progs, shots, seeds = [], [], []
     for k in range(12):
@@ -423,7 +428,9 @@ def test_large_circuit_global_frames_export_replay(gpu):
     assert np.array_equal(gf, of)
 
 
-def test_batched_with_large_circuits(gpu):
+@pytest.mark.parametrize("batch_jit", ["1", "0"])
+def test_batched_with_large_circuits(gpu, monkeypatch, batch_jit):
+    monkeypatch.setenv("QCF_BATCH_JIT", batch_jit)
     rng = np.random.default_rng(8)
     progs, shots, seeds = [], [], []
     for d in (3, 15, 5, 13, 11, 3):
