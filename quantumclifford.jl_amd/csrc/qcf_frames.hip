@@ -574,6 +574,8 @@ int jit_plan(qcf_program *P, const KParams &kp, uint64_t shots, int variant, boo
   out->jp = jp;
   out->grid = (uint32_t)g1;
   out->lds_bytes = jk.lds_bytes;
+  out->blocks_per_cu = jk.blocks_per_cu;
+  out->key = jk.key;
   out->global_planes = gjit;
   return QCF_OK;
 }
@@ -754,8 +756,9 @@ struct qcf_batch {
   size_t n = 0;
   std::vector<qcf::KParams> params;
   std::vector<uint8_t> on_jit;          // circuit k runs its specialised kernel
-  std::vector<qcf::JitPlan> jit;        // those launches, costliest first
+  std::vector<qcf::JitPlan> jit;        // one launch per kernel, costliest first
   std::vector<int> jit_stream;          // helper stream of each launch
+  void *d_jit = nullptr;                // per-circuit parameters and tile offsets
   std::vector<qcf::WorkItem> items;
   qcf::KParams *d_params = nullptr;
   qcf::WorkItem *d_items = nullptr;
@@ -775,6 +778,7 @@ struct qcf_batch {
       if (join[h]) (void)hipEventDestroy(join[h]);
     }
     if (fork) (void)hipEventDestroy(fork);
+    if (d_jit) (void)hipFree(d_jit);
     if (d_params) (void)hipFree(d_params);
     if (d_items) (void)hipFree(d_items);
     if (d_queue) (void)hipFree(d_queue);
@@ -877,35 +881,90 @@ bool batch_concurrent() {
   return e && e[0] == 'c';
 }
 
-// Specialised kernels of the circuits that have one, costliest first, each
-// on the helper stream with the least work so far.  No allocation and no
-// compilation happen here (both are done when the batch is created), so the
-// sequence can be captured into a graph.
-int plan_jit(qcf_batch *B, const std::vector<double> &cost) {
-  std::vector<size_t> order;
-  for (size_t k = 0; k < B->n; ++k)
-    if (B->on_jit[k]) order.push_back(k);
-  std::stable_sort(order.begin(), order.end(),
-                   [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-  std::vector<qcf::JitPlan> plans(B->jit);  // indexed by circuit while planning
+// Specialised launches of the circuits that have a kernel.  Circuits whose
+// kernels have the same source (the same code: for memory experiments, the
+// same distance at any noise rate) share ONE launch over all their tiles --
+// the kernel finds each tile's circuit in a table of per-circuit parameters
+// -- so a distance with many small circuits still fills the GPU.  Launches go
+// costliest first, each to the helper stream with the least work so far.
+// No allocation and no compilation happen at launch time (both are done
+// here), so the launch sequence can be captured into a graph.
+int plan_jit(qcf_batch *B) {
+  std::vector<JitPlan> plans(B->jit);  // indexed by circuit
   B->jit.clear();
   B->jit_stream.clear();
+  std::vector<uint64_t> keys;
+  std::vector<std::vector<size_t>> members;
+  for (size_t k = 0; k < B->n; ++k) {
+    if (!B->on_jit[k]) continue;
+    size_t g = 0;
+    while (g < keys.size() && keys[g] != plans[k].key) ++g;
+    if (g == keys.size()) {
+      keys.push_back(plans[k].key);
+      members.emplace_back();
+    }
+    members[g].push_back(k);
+  }
+  if (keys.empty()) return QCF_OK;
+  // device table: every member's parameters, then every group's tile offsets
+  size_t nparams = 0, nfirst = 0;
+  for (const auto &m : members) nparams += m.size(), nfirst += m.size() + 1;
+  const size_t pbytes = (nparams * sizeof(JitParams) + 255) / 256 * 256;
+  std::vector<char> host(pbytes + nfirst * sizeof(uint32_t), 0);
+  JitParams *hp = (JitParams *)host.data();
+  uint32_t *hf = (uint32_t *)(host.data() + pbytes);
+  QCF_HIP_CHECK(hipMalloc(&B->d_jit, host.size()));
+  char *dbase = (char *)B->d_jit;
+  DeviceState *st;
+  QCF_TRY(device_state(&st));
+  std::vector<std::pair<double, JitPlan>> launches;
+  size_t pi = 0, fi = 0;
+  for (const auto &m : members) {
+    const size_t p0 = pi, f0 = fi;
+    uint32_t tiles = 0;
+    double work = 0.0;
+    for (size_t k : m) {
+      hp[pi] = plans[k].jp;
+      hp[pi].ncirc = 0;
+      hp[pi].xcds = 0;
+      hf[fi++] = tiles;
+      tiles += B->params[k].nblocks;
+      work += (double)B->params[k].ninstr * B->params[k].nblocks;
+      ++pi;
+    }
+    hf[fi++] = tiles;
+    JitPlan gp = plans[m[0]];
+    gp.jp.multi = (const JitParams *)(dbase + p0 * sizeof(JitParams));
+    gp.jp.first = (const uint32_t *)(dbase + pbytes + f0 * sizeof(uint32_t));
+    gp.jp.ncirc = (uint32_t)m.size();
+    gp.jp.ntiles = tiles;
+    uint64_t g1 = std::min<uint64_t>(tiles, (uint64_t)st->cu_count * std::max(1u, gp.blocks_per_cu));
+    const uint32_t nx = jit_xcds();
+    gp.jp.xcds = 0;
+    if (nx > 1 && g1 >= nx) {
+      g1 -= g1 % nx;
+      gp.jp.xcds = nx;
+    }
+    gp.grid = (uint32_t)std::max<uint64_t>(1, g1);
+    launches.push_back({work / gp.grid, gp});
+  }
+  QCF_HIP_CHECK(hipMemcpy(B->d_jit, host.data(), host.size(), hipMemcpyHostToDevice));
+  std::stable_sort(launches.begin(), launches.end(),
+                   [](const auto &a, const auto &b) { return a.first > b.first; });
   double load[kBatchStreams] = {};
-  for (size_t k : order) {
+  for (const auto &l : launches) {
     int h = 0;
     for (int j = 1; j < kBatchStreams; ++j)
       if (load[j] < load[h]) h = j;
-    load[h] += cost[k];
-    B->jit.push_back(plans[k]);
+    load[h] += l.first;
+    B->jit.push_back(l.second);
     B->jit_stream.push_back(h);
   }
-  if (!B->jit.empty()) {
-    for (int h = 0; h < kBatchStreams; ++h) {
-      QCF_HIP_CHECK(hipStreamCreateWithFlags(&B->helpers[h], hipStreamNonBlocking));
-      QCF_HIP_CHECK(hipEventCreateWithFlags(&B->join[h], hipEventDisableTiming));
-    }
-    QCF_HIP_CHECK(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+  for (int h = 0; h < kBatchStreams; ++h) {
+    QCF_HIP_CHECK(hipStreamCreateWithFlags(&B->helpers[h], hipStreamNonBlocking));
+    QCF_HIP_CHECK(hipEventCreateWithFlags(&B->join[h], hipEventDisableTiming));
   }
+  QCF_HIP_CHECK(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
   return QCF_OK;
 }
 
@@ -959,7 +1018,6 @@ extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
   B->params.resize(n);
   B->on_jit.assign(n, 0);
   B->jit.assign(n, JitPlan{});
-  std::vector<double> cost(n, 0.0);
   const bool use_jit = batch_jit_enabled() && !force_global_frames();
   for (size_t k = 0; k < n; ++k) {
     if (!progs[k]) {
@@ -980,13 +1038,11 @@ extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
     if (jit_plan(P, kp, shots[k], 1, big, 0, ld[k], &pl) == QCF_OK && !pl.global_planes) {
       B->on_jit[k] = 1;
       B->jit[k] = pl;
-      // tile passes of the launch times the program length
-      cost[k] = (double)kp.ninstr * ((kp.nblocks + pl.grid - 1) / std::max(1u, pl.grid));
     } else {
       clear_error();
     }
   }
-  if (int st = plan_jit(B, cost); st != QCF_OK) {
+  if (int st = plan_jit(B); st != QCF_OK) {
     delete B;
     return st;
   }

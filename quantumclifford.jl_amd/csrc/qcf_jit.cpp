@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -45,6 +46,11 @@ struct KJ {
   uint32_t has_class, nfull, xcds;
   uint32_t *scratch;  // per-wave planes (global mode)
   uint64_t sstride;   // words per wave
+  // several circuits with this kernel in one launch (ncirc > 0): circuit i
+  // has parameters multi[i] and owns tiles [first[i], first[i+1]) of ntiles
+  const KJ *multi;
+  const uint32_t *first;
+  uint32_t ncirc, ntiles;
 };
 struct FlipLDS {
   uint32_t *L;
@@ -732,7 +738,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   src << kDeviceSrc << kKernelPrelude;
   src << "__constant__ uint32_t QS[" << std::max<uint32_t>(4, ntab) << "] = {\n"
       << (nsite ? tab.str() : std::string("0u,0u,0u,0u")) << "};\n";
-  src << "extern \"C\" __global__ void __launch_bounds__(64) qcf_jit_kernel(KJ P) {\n"
+  src << "extern \"C\" __global__ void __launch_bounds__(64) qcf_jit_kernel(KJ P0) {\n"
       << "  extern __shared__ uint32_t L[];\n"
       << "  const uint32_t l = threadIdx.x;\n";
   {
@@ -742,7 +748,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
           << w << "));\n";
   }
   src
-      << (glob ? "  uint32_t *const F = P.scratch + (uint64_t)blockIdx.x * P.sstride;\n"
+      << (glob ? "  uint32_t *const F = P0.scratch + (uint64_t)blockIdx.x * P0.sstride;\n"
                  "  (void)L;\n  const FlipG flip{F};\n"
                : "  const FlipLDS flip{L};\n");
   if (preload) {
@@ -759,17 +765,47 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   if (regs || (split && max_slots))  // the slot buffer starts at zero and every pass leaves it at zero
     src << "  for (uint32_t k = " << slot_base << "u; k < " << slot_base + std::max<uint32_t>(max_slots, 1u)
         << "u; ++k) L[k * 64u + l] = 0u;\n";
-  // Tiles are dealt out per XCD: with X = P.xcds > 1 the grid is a multiple of
+  // Tiles are dealt out per XCD: with X = P0.xcds > 1 the grid is a multiple of
   // X, workgroup i runs on XCD i mod X, and XCD x takes the x-th contiguous
   // run of tiles, so each XCD's L2 writes whole contiguous stretches of every
   // record column.
   src << R"QCFJIT(
-  const uint32_t nx = P.xcds ? P.xcds : 1u;
+  const uint32_t ntiles = P0.ncirc ? P0.ntiles : P0.nblocks;
+  const uint32_t nx = P0.xcds ? P0.xcds : 1u;
   const uint32_t per = gridDim.x / nx, xi = blockIdx.x % nx, wi = blockIdx.x / nx;
-  const uint32_t span = (P.nblocks + nx - 1u) / nx;
+  const uint32_t span = (ntiles + nx - 1u) / nx;
   for (uint32_t t = wi; t < span; t += per) {
-    const uint32_t blk = xi * span + t;
-    if (blk >= P.nblocks) break;
+    const uint32_t tile = xi * span + t;
+    if (tile >= ntiles) break;
+    // the circuit of this tile in a multi-circuit launch: the last i with
+    // first[i] <= tile (uniform, so scalar loads, which never wait behind
+    // the vector record stores)
+    KJ P = P0;
+    uint32_t blk = tile;
+    if (P0.ncirc) {
+      // the table is read-only during the launch: constant address space
+      typedef __attribute__((address_space(4))) const uint32_t cu32;
+      typedef __attribute__((address_space(4))) const KJ cKJ;
+      const cu32 *const first = (const cu32 *)P0.first;
+      uint32_t lo = 0u, hi = P0.ncirc;
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (first[mid] <= tile) lo = mid;
+        else hi = mid;
+      }
+      const cKJ *const Q = (const cKJ *)P0.multi + lo;
+      P.rec = Q->rec;
+      P.frame = Q->frame;
+      P.seed = Q->seed;
+      P.ld32 = Q->ld32;
+      P.nw32 = Q->nw32;
+      P.nwbuf = Q->nwbuf;
+      P.lastmask = Q->lastmask;
+      P.c = Q->c;
+      P.has_class = Q->has_class;
+      P.nfull = Q->nfull;
+      blk = tile - first[lo];
+    }
     const uint32_t gw = blk * 64u + l;
     // an opaque per-tile copy of the plane stride keeps the compiler from
     // hoisting every record address out of the tile loop
@@ -939,6 +975,7 @@ int jit_get(qcf_program *P, int variant, JitKernelInfo *info) {
       info->function = j.function;
       info->lds_bytes = j.lds_bytes;
       info->blocks_per_cu = j.blocks_per_cu;
+      info->key = j.key;
       return QCF_OK;
     }
   qcf_program::JitKernel jk;
@@ -952,6 +989,7 @@ int jit_get(qcf_program *P, int variant, JitKernelInfo *info) {
   else if (!generate(P, variant, &src, &jk.lds_bytes))
     st = set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   if (st == QCF_OK) {
+    jk.key = std::hash<std::string>{}(src);
     // programs with the same source (the same circuit built twice) share one
     // compilation
     auto it = g_code_cache.find(src);
@@ -990,6 +1028,7 @@ int jit_get(qcf_program *P, int variant, JitKernelInfo *info) {
   info->function = jk.function;
   info->lds_bytes = jk.lds_bytes;
   info->blocks_per_cu = jk.blocks_per_cu;
+  info->key = jk.key;
   return st;
 }
 

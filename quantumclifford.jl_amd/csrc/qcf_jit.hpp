@@ -20,6 +20,11 @@ struct JitParams {
   uint32_t has_class, nfull, xcds;
   uint32_t *scratch;  // per-wave planes when the frame does not fit in LDS
   uint64_t sstride;   // words per wave
+  // multi-circuit launch (ncirc > 0): circuit i has parameters multi[i] and
+  // tiles [first[i], first[i+1]) of ntiles; a single launch leaves them zero
+  const JitParams *multi;
+  const uint32_t *first;
+  uint32_t ncirc, ntiles;
 };
 
 // QCF_NO_JIT=1 keeps every launch on the interpreter.
@@ -45,6 +50,7 @@ struct JitKernelInfo {
   void *function = nullptr;     // hipFunction_t
   uint32_t lds_bytes = 0;       // dynamic LDS per wave
   uint32_t blocks_per_cu = 0;   // resident waves per compute unit
+  uint64_t key = 0;             // hash of the kernel source: equal keys, equal code
 };
 
 // The compiled kernel of `variant` for the current device (compiled on first
@@ -57,6 +63,8 @@ struct JitPlan {
   JitParams jp{};
   uint32_t grid = 0;
   uint32_t lds_bytes = 0;
+  uint32_t blocks_per_cu = 0;
+  uint64_t key = 0;            // circuits with equal keys can share one launch
   bool global_planes = false;  // planes in global memory: jp.scratch still to be set
 };
 

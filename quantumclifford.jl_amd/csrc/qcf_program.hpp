@@ -121,6 +121,7 @@ struct qcf_program {
     void *function = nullptr; // hipFunction_t
     uint32_t lds_bytes = 0;
     uint32_t blocks_per_cu = 0;
+    uint64_t key = 0;  // hash of the kernel source
     bool failed = false;
   };
   std::vector<JitKernel> jit;

@@ -17,11 +17,13 @@ def _stream():
 
 
 class Batch:
-    """Fresh-frame sampling of several compiled circuits in ONE persistent kernel.
+    """Fresh-frame sampling of several compiled circuits in one batched launch:
+    circuits with a specialised kernel run it (one launch per distinct kernel,
+    on helper streams), the rest share ONE persistent interpreter kernel.
 
     Records live in `self.records[k]` (nbits_k planes of plane_words(shots_k) words).
-    `launch()` enqueues the kernel (no allocation, capturable); `launch_graph()`
-    replays a captured hipGraph of it."""
+    `launch()` enqueues the launches (no allocation, capturable); `launch_graph()`
+    replays a captured hipGraph of them."""
 
     def __init__(self, progs: Sequence[Program], shots: Sequence[int], seeds: Sequence[int],
                  tune: Tune | None = None, device=None):
