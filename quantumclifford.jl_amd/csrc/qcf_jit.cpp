@@ -358,7 +358,10 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
         i += code[i].op >> 8;
       }
     }
-    const uint32_t budget = kSplitLdsBytes / 256u - (noisy ? kSlots : 0u);
+    // variant bit 5: half of a CU's LDS per wave, so that two waves share it
+    const uint32_t lds_planes = kSplitLdsBytes / 256u / ((variant & 32) ? 2u : 1u);
+    if (noisy && lds_planes <= kSlots) return false;
+    const uint32_t budget = lds_planes - (noisy ? kSlots : 0u);
     const uint32_t kmin = P->nplanes > budget ? P->nplanes - budget : 0u;
     const uint32_t k = std::min(P->nplanes, std::max(kmin, jit_split_regs()));
     if (k > kSplitMaxRegs) return false;
@@ -1052,7 +1055,7 @@ extern "C" int qcf_debug_jit_source(const qcf_program *prog, int variant, char *
   qcf::clear_error();
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src;
-  if (!qcf::generate(prog, variant & 31, &src, nullptr))
+  if (!qcf::generate(prog, variant & 63, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   if (needed) *needed = src.size() + 1;
   if (buf && cap) {
@@ -1068,7 +1071,7 @@ extern "C" int qcf_debug_jit_compile(const qcf_program *prog, int variant, size_
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src, log;
   std::vector<char> bin;
-  if (!qcf::generate(prog, variant & 31, &src, nullptr))
+  if (!qcf::generate(prog, variant & 63, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   QCF_TRY(qcf::compile(src, &bin, &log));
   if (code_bytes) *code_bytes = bin.size();

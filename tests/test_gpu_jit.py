@@ -114,18 +114,24 @@ def test_global_frames_specialised_equals_interpreter(gpu, monkeypatch, d, round
     assert np.array_equal(fa, fb)
 
 
-@pytest.mark.parametrize("case", ["unrotated_11x3", "rotated_13x2"])
+@pytest.mark.parametrize("case", ["unrotated_11x3", "rotated_13x2", "rotated_11x3", "rotated_9x3"])
 def test_split_frames_specialised_equals_interpreter(gpu, monkeypatch, case):
     """Frames too large for LDS run the split kernel (most-used planes in
-    registers, the rest in LDS, noise on register planes through slots): bit
-    for bit the interpreter's global-memory result, partial tile and kept
-    frame included."""
+    registers, the rest in LDS, noise on register planes through slots), with
+    half of a CU's LDS per wave where the registers allow (rotated d=11, and
+    d=9, whose frame fits but would run one wave per CU): bit for bit the
+    interpreter's result, partial tile and kept frame included."""
     if case == "unrotated_11x3":
         ops, shots = circuits.unrotated_surface_code_benchmark(11, 3), 3 * 2048 + 5
-    else:
+    elif case == "rotated_13x2":
         ops, shots = circuits.rotated_surface_code(13, 2, 5e-3).ops, 2 * 2048 + 77
+    elif case == "rotated_11x3":
+        ops, shots = circuits.rotated_surface_code(11, 3, 5e-3).ops, 3 * 2048 + 100
+    else:
+        ops, shots = circuits.rotated_surface_code(9, 3, 1e-2).ops, 4 * 2048 + 1
     prog = qcf.Program(ops)
-    assert prog.info.lds_bytes > 160 * 1024
+    if case != "rotated_9x3":
+        assert prog.info.lds_bytes > 160 * 1024
     print(f"split kernel {case}, {shots} shots", flush=True)
 
     def run():

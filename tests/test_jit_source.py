@@ -61,3 +61,14 @@ def test_split_source_compiles():
     n = C.c_size_t()
     _lib.call("qcf_debug_jit_compile", prog.handle, SPLIT, C.byref(n))
     assert n.value > 0
+
+
+def test_two_wave_split_halves_the_lds():
+    """Variant bit 5: split mode within half of a CU's LDS (two waves per CU);
+    rotated d=9 (324 qubit planes, one wave per CU in LDS mode) fits it."""
+    prog = qcf.Program(circuits.rotated_surface_code(9, 9, 1e-3).ops)
+    src = source(prog, 1 | 8 | 16 | 32)
+    assert lds_planes(src) <= 320
+    big = qcf.Program(circuits.rotated_surface_code(13, 13, 1e-3).ops)
+    with pytest.raises(_lib.MethodError):  # 929 planes: too many for 320 in LDS + 400 registers
+        source(big, 1 | 4 | 8 | 16 | 32)
