@@ -543,13 +543,24 @@ int jit_plan(qcf_program *P, const KParams &kp, uint64_t shots, int variant, boo
   // wave per CU; split mode with half of the LDS each (variant bit 5) lets two
   // waves share a CU when the rest fits the register budget (rotated d=9 x 9,
   // p=1e-3, 10^7 shots: 2.0 ms instead of 3.6 ms in LDS mode).
+  // Frames beyond split mode that fit in LDS at half width (up to 1280
+  // planes) run half-tile passes (variant bit 6): lanes 0..31 hold one half
+  // of the tile's shot words, two passes per tile.
   const bool try_two = split_ok && (global || 2u * P->nq * 256u > 80u * 1024u);
-  bool two = false;
+  int js = QCF_ERR_UNSUPPORTED;
   if (try_two) {
-    two = jit_get(P, v | 16 | 32, &jk) == QCF_OK;
-    if (!two) clear_error();
+    js = jit_get(P, v | 16 | 32, &jk);
+    if (js != QCF_OK) clear_error();
   }
-  if (!two) QCF_TRY(jit_get(P, v | (try_split ? 16 : 0), &jk));
+  if (js != QCF_OK && try_split) {
+    js = jit_get(P, v | 16, &jk);
+    if (js != QCF_OK) {
+      clear_error();
+      js = jit_get(P, v | 64, &jk);
+    }
+  }
+  if (js != QCF_OK && !try_split) js = jit_get(P, v, &jk);
+  QCF_TRY(js);
   const bool gjit = global && !try_split;  // specialised kernel on global planes
   JitParams jp{};
   jp.rec = kp.rec;
@@ -885,6 +896,14 @@ bool batch_jit_enabled() {
   return !(e && e[0] == '0');
 }
 
+// Helper streams the specialised launches are spread over (QCF_BATCH_STREAMS,
+// 1..4).
+int batch_streams() {
+  const char *e = getenv("QCF_BATCH_STREAMS");
+  const int v = e ? atoi(e) : kBatchStreams;
+  return std::max(1, std::min(kBatchStreams, v));
+}
+
 // QCF_BATCH_JIT_ORDER=concurrent starts the persistent kernel beside the
 // specialised launches instead of after them.
 bool batch_concurrent() {
@@ -963,9 +982,10 @@ int plan_jit(qcf_batch *B) {
   std::stable_sort(launches.begin(), launches.end(),
                    [](const auto &a, const auto &b) { return a.first > b.first; });
   double load[kBatchStreams] = {};
+  const int nstreams = batch_streams();
   for (const auto &l : launches) {
     int h = 0;
-    for (int j = 1; j < kBatchStreams; ++j)
+    for (int j = 1; j < nstreams; ++j)
       if (load[j] < load[h]) h = j;
     load[h] += l.first;
     B->jit.push_back(l.second);

@@ -59,6 +59,15 @@ struct FlipLDS {
     atomicXor(&L[p * 64u + tl], bit);
   }
 };
+// half-tile passes: planes of 32 words; a pass keeps the flips of its half
+struct FlipHalf {
+  uint32_t *L;
+  uint32_t half;
+  __device__ __forceinline__ void operator()(uint32_t p, uint32_t tl, uint32_t bit, uint32_t,
+                                             uint32_t) const {
+    if ((tl >> 5) == half) atomicXor(&L[p * 32u + (tl & 31u)], bit);
+  }
+};
 // planes in global memory: every access at L2 (agent scope) so that the
 // event flips other lanes make are seen by the next read
 __device__ __forceinline__ uint32_t gld(const uint32_t *a) {
@@ -298,8 +307,13 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   // frames too large for LDS (variant bit 2): with bit 4 the planes used most
   // stay in registers and the rest in LDS (split mode), else every plane in
   // global memory; fitting frames: registers for small programs, else LDS
-  const bool split = (variant & 16) != 0;
-  const bool glob = !split && (variant & 4) != 0;
+  // variant bit 6: half-tile passes -- every plane in LDS at 128 bytes (the
+  // 32 shot words of one half of the tile), two passes per tile, for frames
+  // beyond split mode that fit in LDS at half width
+  const bool halfm = (variant & 64) != 0;
+  if (halfm && P->nplanes * 128u > kSplitLdsBytes) return false;
+  const bool split = !halfm && (variant & 16) != 0;
+  const bool glob = !split && !halfm && (variant & 4) != 0;
   // variant bit 3: the record spans less than 4 GB, so a record address is
   // one 64-bit base plus a 32-bit byte offset (one scalar multiply per
   // store instead of a 64-bit multiply-add)
@@ -313,7 +327,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
       P->classes.empty() ? 0.0 : (double)P->sites.size() * kBlockShots * P->classes[0].p;
   // noiseless programs take no exchanges at all and gain the occupancy of
   // registers (d=5 x 25: 0.295 vs 0.395 ms)
-  const bool regs = !glob && !split && P->nplanes <= jit_reg_planes() &&
+  const bool regs = !glob && !split && !halfm && P->nplanes <= jit_reg_planes() &&
                     (P->classes.empty() || ev_per_tile >= jit_reg_events() ||
                      2u * P->nq * 256u > 40u * 1024u);
   const bool nt = jit_nt_stores();
@@ -321,7 +335,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   // site names) in registers: only the qubit planes take LDS, so more waves
   // fit on a CU
   const uint32_t qpl = 2u * P->nq;
-  bool hyb = !regs && !glob && !split && P->nplanes > qpl;
+  bool hyb = !regs && !glob && !split && !halfm && P->nplanes > qpl;
   for (size_t i = 0; hyb && i < code.size(); ++i) {
     if ((code[i].op & 0xFFu) != I_NOISEG) continue;
     const uint32_t G = code[i].op >> 8;
@@ -382,8 +396,10 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   // base (lb<w> = w * 16384 + l); a single base register per window then
   // serves every access in it, where per-access addresses would be hoisted
   // out of the loops and spilled
-  auto plane = [](uint32_t q) {
-    const uint32_t w = q / 256u, off = (q % 256u) * 64u;
+  const uint32_t pw = halfm ? 32u : 64u;      // words per LDS plane
+  const uint32_t per_window = 16384u / pw;     // planes per 64 KB window
+  auto plane = [pw, per_window](uint32_t q) {
+    const uint32_t w = q / per_window, off = (q % per_window) * pw;
     if (w == 0) return "L[" + std::to_string(off) + "u + l]";
     return "L[lb" + std::to_string(w) + " + " + std::to_string(off) + "u]";
   };
@@ -566,10 +582,17 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     size_t gi = range_groups[ri];
     const size_t gend = range_groups[ri + 1];
     if (gi < gend && !preload) body << load_group(gi, "t");
+    bool lh_open = false;  // half mode: inside a run guarded by "lanes 0..31"
     for (size_t i = rg.b; i < rg.e; ++i) {
       const Instr &in = code[i];
       const uint32_t op = in.op & 0xFFu;
       const uint32_t a = in.a, b = in.b, c = in.c;
+      // half mode: only lanes 0..31 hold planes; the event passes (noise
+      // groups) need every lane of the wave
+      if (halfm && (op == I_NOISEG) == lh_open) {
+        body << (lh_open ? "    }\n" : "    if (lh) {\n");
+        lh_open = !lh_open;
+      }
       // split mode: every so many operations all register planes pass a
       // compiler barrier, so the scheduler cannot stretch live ranges (early
       // LDS reads, reassociated XOR chains) over a whole straight-line stretch
@@ -728,12 +751,13 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
           return false;
       }
     }
+    if (lh_open) body << "    }\n";
     if (in_loop) body << "    }\n";
   }
     return true;
   };
   std::ostringstream full, part;
-  if (!emit(false, full) || !emit(true, part)) return false;
+  if ((!halfm && !emit(false, full)) || !emit(true, part)) return false;
   std::ostringstream src;
   src << "#define QCF_RTC 1\n";
   src << "// folded: prologue " << lp.s << ", loop " << lp.R << " x " << lp.L << ", of "
@@ -746,14 +770,14 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
       << "  const uint32_t l = threadIdx.x;\n";
   {
     const uint32_t nl = glob ? 0u : split ? nlds + max_slots : regs ? max_slots : hyb ? qpl : P->nplanes;
-    for (uint32_t w = 1; w * 256u < nl; ++w)
+    for (uint32_t w = 1; w * per_window < nl; ++w)
       src << "  uint32_t lb" << w << " = " << w * 16384u << "u + l;\n  asm volatile(\"\" : \"+v\"(lb"
           << w << "));\n";
   }
   src
       << (glob ? "  uint32_t *const F = P0.scratch + (uint64_t)blockIdx.x * P0.sstride;\n"
                  "  (void)L;\n  const FlipG flip{F};\n"
-               : "  const FlipLDS flip{L};\n");
+               : halfm ? "" : "  const FlipLDS flip{L};\n");
   if (preload) {
     // entry registers, made opaque so that they stay in registers
     for (size_t r = 0; r < ereg.size(); ++r)
@@ -809,7 +833,17 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
       P.nfull = Q->nfull;
       blk = tile - first[lo];
     }
-    const uint32_t gw = blk * 64u + l;
+)QCFJIT";
+  // half mode: two passes over the tile, each with the shot words of one half
+  // in lanes 0..31 (the event stream is drawn for the whole tile in both)
+  if (halfm)
+    src << "    for (uint32_t hf = 0; hf < 2u; ++hf) {\n"
+           "    const bool lh = l < 32u;\n"
+           "    const uint32_t gw = blk * 64u + hf * 32u + (l & 31u);\n"
+           "    const FlipHalf flip{L, hf};\n";
+  else
+    src << "    const uint32_t gw = blk * 64u + l;\n";
+  src << R"QCFJIT(
     // an opaque per-tile copy of the plane stride keeps the compiler from
     // hoisting every record address out of the tile loop
     uint64_t ld = P.ld32;
@@ -825,16 +859,23 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     uint32_t acc = 0u;
     (void)acc;
 )QCFJIT";
-  src << "    if (blk < P.nfull) {  // every lane's 32 shots exist\n" << full.str()
-      << "    } else {\n"
-      << "    const bool live = gw < P.nwbuf;\n"
-      << "    const uint32_t lm = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1u ? P.lastmask : "
-         "0xFFFFFFFFu);\n"
-      << part.str() << "    }\n";
+  if (halfm)
+    src << "    const bool live = lh && gw < P.nwbuf;\n"
+        << "    const uint32_t lm = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1u ? P.lastmask : "
+           "0xFFFFFFFFu);\n"
+        << part.str() << "    }\n";
+  else
+    src << "    if (blk < P.nfull) {  // every lane's 32 shots exist\n" << full.str()
+        << "    } else {\n"
+        << "    const bool live = gw < P.nwbuf;\n"
+        << "    const uint32_t lm = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1u ? P.lastmask : "
+           "0xFFFFFFFFu);\n"
+        << part.str() << "    }\n";
   src << "  }\n}\n";
   *out = src.str();
   if (lds_bytes)
     *lds_bytes = glob    ? 0u
+                 : halfm ? P->nplanes * 128u
                  : split ? (nlds + max_slots) * 256u
                  : regs  ? std::max<uint32_t>(max_slots, 1u) * 256u
                  : hyb  ? qpl * 256u
@@ -1055,7 +1096,7 @@ extern "C" int qcf_debug_jit_source(const qcf_program *prog, int variant, char *
   qcf::clear_error();
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src;
-  if (!qcf::generate(prog, variant & 63, &src, nullptr))
+  if (!qcf::generate(prog, variant & 127, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   if (needed) *needed = src.size() + 1;
   if (buf && cap) {
@@ -1071,7 +1112,7 @@ extern "C" int qcf_debug_jit_compile(const qcf_program *prog, int variant, size_
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src, log;
   std::vector<char> bin;
-  if (!qcf::generate(prog, variant & 63, &src, nullptr))
+  if (!qcf::generate(prog, variant & 127, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   QCF_TRY(qcf::compile(src, &bin, &log));
   if (code_bytes) *code_bytes = bin.size();

@@ -114,14 +114,18 @@ def test_global_frames_specialised_equals_interpreter(gpu, monkeypatch, d, round
     assert np.array_equal(fa, fb)
 
 
-@pytest.mark.parametrize("case", ["unrotated_11x3", "rotated_13x2", "rotated_11x3", "rotated_9x3"])
+@pytest.mark.parametrize("case", ["unrotated_11x3", "rotated_13x2", "rotated_11x3", "rotated_9x3",
+                                  "rotated_15x2"])
 def test_split_frames_specialised_equals_interpreter(gpu, monkeypatch, case):
     """Frames too large for LDS run the split kernel (most-used planes in
     registers, the rest in LDS, noise on register planes through slots), with
     half of a CU's LDS per wave where the registers allow (rotated d=11, and
     d=9, whose frame fits but would run one wave per CU): bit for bit the
-    interpreter's result, partial tile and kept frame included."""
-    if case == "unrotated_11x3":
+    interpreter's result, partial tile and kept frame included.  Rotated d=15
+    (1237 planes) is beyond split mode and runs half-tile passes."""
+    if case == "rotated_15x2":
+        ops, shots = circuits.rotated_surface_code(15, 2, 5e-3).ops, 3 * 2048 + 1000
+    elif case == "unrotated_11x3":
         ops, shots = circuits.unrotated_surface_code_benchmark(11, 3), 3 * 2048 + 5
     elif case == "rotated_13x2":
         ops, shots = circuits.rotated_surface_code(13, 2, 5e-3).ops, 2 * 2048 + 77

@@ -72,3 +72,20 @@ def test_two_wave_split_halves_the_lds():
     big = qcf.Program(circuits.rotated_surface_code(13, 13, 1e-3).ops)
     with pytest.raises(_lib.MethodError):  # 929 planes: too many for 320 in LDS + 400 registers
         source(big, 1 | 4 | 8 | 16 | 32)
+
+
+def test_half_tile_source_stays_in_lds():
+    """Variant bit 6: every plane in LDS at 32 words (one half of the tile's
+    shot words), lanes 0..31; rotated d=15 (1237 planes, beyond split mode)
+    takes 158 KB.  Every constant LDS address lies inside the planes."""
+    prog = qcf.Program(circuits.rotated_surface_code(15, 2, 5e-3).ops)
+    src = source(prog, 1 | 4 | 8 | 64)
+    hi = 0
+    for w, off in re.findall(r"L\[lb(\d+) \+ (\d+)u\]", src):
+        hi = max(hi, int(w) * 16384 + int(off) + 32)
+    for off in re.findall(r"L\[(\d+)u \+ l\]", src):
+        hi = max(hi, int(off) + 32)
+    assert 0 < hi <= prog.info.nplanes * 32 <= 160 * 1024 // 4
+    assert "if (lh) {" in src and "FlipHalf flip{L, hf}" in src
+    with pytest.raises(_lib.MethodError):  # 2,004 planes do not fit even at half width
+        source(qcf.Program(circuits.x_diag_circuit(1000)), 1 | 4 | 8 | 64)
