@@ -543,9 +543,10 @@ int jit_plan(qcf_program *P, const KParams &kp, uint64_t shots, int variant, boo
   // wave per CU; split mode with half of the LDS each (variant bit 5) lets two
   // waves share a CU when the rest fits the register budget (rotated d=9 x 9,
   // p=1e-3, 10^7 shots: 2.0 ms instead of 3.6 ms in LDS mode).
-  // Frames beyond split mode that fit in LDS at half width (up to 1280
-  // planes) run half-tile passes (variant bit 6): lanes 0..31 hold one half
-  // of the tile's shot words, two passes per tile.
+  // Frames beyond split mode that fit in LDS at half or quarter width (up to
+  // 1280 / 2560 planes) run part-tile passes (variant bits 6 / 7): lanes
+  // 0..31 (0..15) hold one half (quarter) of the tile's shot words, two
+  // (four) passes per tile.
   const bool try_two = split_ok && (global || 2u * P->nq * 256u > 80u * 1024u);
   int js = QCF_ERR_UNSUPPORTED;
   if (try_two) {
@@ -557,6 +558,10 @@ int jit_plan(qcf_program *P, const KParams &kp, uint64_t shots, int variant, boo
     if (js != QCF_OK) {
       clear_error();
       js = jit_get(P, v | 64, &jk);
+    }
+    if (js != QCF_OK) {
+      clear_error();
+      js = jit_get(P, v | 128, &jk);  // quarter tiles: up to 2560 planes
     }
   }
   if (js != QCF_OK && !try_split) js = jit_get(P, v, &jk);

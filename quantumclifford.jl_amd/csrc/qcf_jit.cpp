@@ -59,13 +59,15 @@ struct FlipLDS {
     atomicXor(&L[p * 64u + tl], bit);
   }
 };
-// half-tile passes: planes of 32 words; a pass keeps the flips of its half
-struct FlipHalf {
+// part-tile passes: planes of PW words (32: halves, 16: quarters); a pass
+// keeps the flips that land in its part of the tile
+template <unsigned PW>
+struct FlipPart {
   uint32_t *L;
-  uint32_t half;
+  uint32_t part;
   __device__ __forceinline__ void operator()(uint32_t p, uint32_t tl, uint32_t bit, uint32_t,
                                              uint32_t) const {
-    if ((tl >> 5) == half) atomicXor(&L[p * 32u + (tl & 31u)], bit);
+    if (tl / PW == part) atomicXor(&L[p * PW + (tl % PW)], bit);
   }
 };
 // planes in global memory: every access at L2 (agent scope) so that the
@@ -286,6 +288,8 @@ void instr_planes(const std::vector<Instr> &code, size_t i, std::vector<uint32_t
 // kernel's own values).
 constexpr uint32_t kSplitLdsBytes = 160u * 1024u;
 constexpr uint32_t kSplitMaxRegs = 400u;
+// Noise groups (event passes) a split or part-tile kernel may emit.
+constexpr size_t kMaxLargeFrameGroups = 256;
 
 bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds_bytes) {
   std::vector<Instr> code;
@@ -307,11 +311,12 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   // frames too large for LDS (variant bit 2): with bit 4 the planes used most
   // stay in registers and the rest in LDS (split mode), else every plane in
   // global memory; fitting frames: registers for small programs, else LDS
-  // variant bit 6: half-tile passes -- every plane in LDS at 128 bytes (the
-  // 32 shot words of one half of the tile), two passes per tile, for frames
-  // beyond split mode that fit in LDS at half width
-  const bool halfm = (variant & 64) != 0;
-  if (halfm && P->nplanes * 128u > kSplitLdsBytes) return false;
+  // variant bits 6 / 7: part-tile passes -- every plane in LDS at 128 / 64
+  // bytes (the shot words of one half / quarter of the tile), two / four
+  // passes per tile, for frames beyond split mode that fit in LDS that way
+  const bool halfm = (variant & (64 | 128)) != 0;
+  const uint32_t pw = (variant & 128) ? 16u : (variant & 64) ? 32u : 64u;  // words per LDS plane
+  if (halfm && P->nplanes * pw * 4u > kSplitLdsBytes) return false;
   const bool split = !halfm && (variant & 16) != 0;
   const bool glob = !split && !halfm && (variant & 4) != 0;
   // variant bit 3: the record spans less than 4 GB, so a record address is
@@ -396,8 +401,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   // base (lb<w> = w * 16384 + l); a single base register per window then
   // serves every access in it, where per-access addresses would be hoisted
   // out of the loops and spilled
-  const uint32_t pw = halfm ? 32u : 64u;      // words per LDS plane
-  const uint32_t per_window = 16384u / pw;     // planes per 64 KB window
+  const uint32_t per_window = 16384u / pw;  // planes per 64 KB window
   auto plane = [pw, per_window](uint32_t q) {
     const uint32_t w = q / per_window, off = (q % per_window) * pw;
     if (w == 0) return "L[" + std::to_string(off) + "u + l]";
@@ -456,6 +460,11 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     }
   }
   range_groups.push_back(groups.size());
+  // Frames beyond LDS: programs with many separate noise groups (each an
+  // inlined event pass) compile for minutes -- noisy x_diag_circuit(1000):
+  // 1,003 passes, 218 s of hiprtc -- while the interpreter needs none; the
+  // memory experiments emit at most about 100
+  if ((split || halfm) && groups.size() > kMaxLargeFrameGroups) return false;
   const uint32_t nsite = (uint32_t)sites.size();
   // Site entries held in registers for the whole kernel: lane i of a group's
   // registers holds its site i.  Planes pack into one register when every
@@ -834,13 +843,15 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
       blk = tile - first[lo];
     }
 )QCFJIT";
-  // half mode: two passes over the tile, each with the shot words of one half
-  // in lanes 0..31 (the event stream is drawn for the whole tile in both)
+  // part-tile mode: 64/pw passes over the tile, each with the shot words of
+  // one part in lanes 0..pw-1 (the event stream is drawn for the whole tile
+  // in every pass); passes past the plane stride hold no shots and end it
   if (halfm)
-    src << "    for (uint32_t hf = 0; hf < 2u; ++hf) {\n"
-           "    const bool lh = l < 32u;\n"
-           "    const uint32_t gw = blk * 64u + hf * 32u + (l & 31u);\n"
-           "    const FlipHalf flip{L, hf};\n";
+    src << "    for (uint32_t hf = 0; hf < " << 64u / pw << "u; ++hf) {\n"
+        << "    if (blk * 64u + hf * " << pw << "u >= P.nwbuf) break;\n"
+        << "    const bool lh = l < " << pw << "u;\n"
+        << "    const uint32_t gw = blk * 64u + hf * " << pw << "u + (l & " << pw - 1 << "u);\n"
+        << "    const FlipPart<" << pw << "> flip{L, hf};\n";
   else
     src << "    const uint32_t gw = blk * 64u + l;\n";
   src << R"QCFJIT(
@@ -875,7 +886,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   *out = src.str();
   if (lds_bytes)
     *lds_bytes = glob    ? 0u
-                 : halfm ? P->nplanes * 128u
+                 : halfm ? P->nplanes * pw * 4u
                  : split ? (nlds + max_slots) * 256u
                  : regs  ? std::max<uint32_t>(max_slots, 1u) * 256u
                  : hyb  ? qpl * 256u
@@ -1096,7 +1107,7 @@ extern "C" int qcf_debug_jit_source(const qcf_program *prog, int variant, char *
   qcf::clear_error();
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src;
-  if (!qcf::generate(prog, variant & 127, &src, nullptr))
+  if (!qcf::generate(prog, variant & 255, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   if (needed) *needed = src.size() + 1;
   if (buf && cap) {
@@ -1112,7 +1123,7 @@ extern "C" int qcf_debug_jit_compile(const qcf_program *prog, int variant, size_
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src, log;
   std::vector<char> bin;
-  if (!qcf::generate(prog, variant & 127, &src, nullptr))
+  if (!qcf::generate(prog, variant & 255, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   QCF_TRY(qcf::compile(src, &bin, &log));
   if (code_bytes) *code_bytes = bin.size();

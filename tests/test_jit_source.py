@@ -86,6 +86,31 @@ def test_half_tile_source_stays_in_lds():
     for off in re.findall(r"L\[(\d+)u \+ l\]", src):
         hi = max(hi, int(off) + 32)
     assert 0 < hi <= prog.info.nplanes * 32 <= 160 * 1024 // 4
-    assert "if (lh) {" in src and "FlipHalf flip{L, hf}" in src
+    assert "if (lh) {" in src and "FlipPart<32> flip{L, hf}" in src
     with pytest.raises(_lib.MethodError):  # 2,004 planes do not fit even at half width
         source(qcf.Program(circuits.x_diag_circuit(1000)), 1 | 4 | 8 | 64)
+
+
+def test_quarter_tile_source_stays_in_lds():
+    """Variant bit 7: planes of 16 words (a quarter of the tile), lanes 0..15,
+    four passes: x_diag_circuit(1000) (benchmark/benchmarks.jl:103-110, 2,004
+    planes) fits in 128 KB."""
+    prog = qcf.Program(circuits.x_diag_circuit(1000))
+    src = source(prog, 1 | 4 | 8 | 128)
+    hi = 0
+    for w, off in re.findall(r"L\[lb(\d+) \+ (\d+)u\]", src):
+        hi = max(hi, int(w) * 16384 + int(off) + 16)
+    for off in re.findall(r"L\[(\d+)u \+ l\]", src):
+        hi = max(hi, int(off) + 16)
+    assert 0 < hi <= prog.info.nplanes * 16 <= 160 * 1024 // 4
+    assert "const bool lh = l < 16u;" in src and "FlipPart<16> flip{L, hf}" in src
+
+
+def test_large_frames_with_many_noise_groups_stay_on_the_interpreter():
+    """x_diag_circuit_noisy_measurement(1000) (benchmark/benchmarks.jl:161-171):
+    1,000 separate noise groups would compile for minutes; split and part-tile
+    kernels decline it."""
+    prog = qcf.Program(circuits.x_diag_circuit_noisy_measurement(1000))
+    for variant in (1 | 4 | 8 | 16, 1 | 4 | 8 | 64, 1 | 4 | 8 | 128):
+        with pytest.raises(_lib.MethodError):
+            source(prog, variant)
