@@ -304,12 +304,19 @@ int qcf_autotune(const qcf_program *const *progs, size_t n,
 int qcf_debug_philox(const uint32_t *in, uint32_t *out, size_t n,
                      void *stream);
 
-/* Source of the kernel specialised to `prog` for a launch variant (bit 0:
- * fresh frame, bit 1: frame stored at the end, bit 2: frame larger than LDS,
- * bit 3: record under 4 GB, bit 4 with bit 2: split storage, registers plus
- * LDS) and its run-time compilation with hiprtc, without loading it on a
- * device (self-check; needs no GPU).  QCF_ERR_UNSUPPORTED if the generator
- * does not handle the program in that variant. */
+/* Source of the kernel specialised to `prog` for a launch variant and its
+ * run-time compilation with hiprtc, without loading it on a device
+ * (self-check; needs no GPU).  Variant bits:
+ *   0  fresh frame;
+ *   1  frame stored at the end;
+ *   2  frame larger than LDS;
+ *   3  record under 4 GB;
+ *   4  split storage (registers plus LDS);
+ *   5  with 4, half of a CU's LDS per wave;
+ *   6  half-tile passes;
+ *   7  quarter-tile passes.
+ * QCF_ERR_UNSUPPORTED if the generator does not handle the program in that
+ * variant. */
 int qcf_debug_jit_source(const qcf_program *prog, int variant, char *buf,
                          size_t cap, size_t *needed);
 int qcf_debug_jit_compile(const qcf_program *prog, int variant,
