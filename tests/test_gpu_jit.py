@@ -151,15 +151,21 @@ def test_split_frames_specialised_equals_interpreter(gpu, monkeypatch, case):
     assert oracle.unpack_shots(ra, shots).any()
 
 
-def test_split_frames_in_place_equals_interpreter(gpu, monkeypatch):
-    """pftrajectories(state, circuit) on an existing 442-qubit frame."""
-    ops = circuits.unrotated_surface_code_benchmark(11, 2)
+@pytest.mark.parametrize("case", ["unrotated_11x2", "rotated_15x1"])
+def test_split_frames_in_place_equals_interpreter(gpu, monkeypatch, case):
+    """pftrajectories(state, circuit) on an existing 442-qubit frame (split
+    kernel) and 450-qubit frame (half-tile passes load and store the frame
+    one half of each tile at a time)."""
+    if case == "unrotated_11x2":
+        ops = circuits.unrotated_surface_code_benchmark(11, 2)
+    else:
+        ops = circuits.rotated_surface_code(15, 1, 5e-3).ops
     prog = qcf.Program(ops)
     n, nbits, shots = prog.nq - 1, prog.nbits, 2048 + 9
     rng = np.random.default_rng(11)
     fb = oracle.pack_shots(rng.integers(0, 2, (2 * (n + 1), shots)).astype(np.uint8))
     mb = oracle.pack_shots(rng.integers(0, 2, (nbits, shots)).astype(np.uint8))
-    print("split kernel in place, unrotated 11 x 2", flush=True)
+    print(f"specialised kernel in place, {case}", flush=True)
 
     def run():
         f = qcf.PauliFrame(shots, n, nbits, init=False)
