@@ -267,8 +267,9 @@ typedef struct qcf_tune {
  * fresh frames with seed seeds[k] into meas[k] (its nbits columns of ld[k]
  * words).  qcf_batch_create uploads the per-circuit parameters and the work
  * queue once and compiles the circuits' specialised kernels.
- * qcf_batch_launch first runs the specialised kernels, on four helper
- * streams forked from `stream` and joined back into it.  It then enqueues
+ * qcf_batch_launch first runs the specialised kernels, one launch per
+ * distinct kernel, on helper streams forked from `stream` and joined back
+ * into it (two by default; QCF_BATCH_STREAMS, 1..4).  It then enqueues
  * ONE persistent kernel for the circuits without one (frames beyond what
  * registers plus LDS hold); its workgroups pull (circuit, tile range) items
  * from a device queue.  The launch allocates and compiles nothing, so it

@@ -773,9 +773,9 @@ extern "C" int qcf_frame_from_reference(const uint64_t *xzs, uint64_t shots, uin
 // ======================================================================
 // Batched persistent launch + autotuner
 // ======================================================================
-// Circuits that have a specialised kernel (frames in LDS, or split between
-// registers and LDS) run it, spread over helper streams forked from the
-// caller's stream; the persistent interpreter kernel takes the rest (frames
+// Circuits that have a specialised kernel (frames in LDS, split between
+// registers and LDS, or part tiles) run it, spread over helper streams forked
+// from the caller's stream; the persistent interpreter kernel takes the rest (frames
 // beyond split mode) on the caller's stream.  Both draw identical bits.
 constexpr int kBatchStreams = 4;
 struct qcf_batch {
@@ -902,10 +902,12 @@ bool batch_jit_enabled() {
 }
 
 // Helper streams the specialised launches are spread over (QCF_BATCH_STREAMS,
-// 1..4).
+// 1..4; default 2: the 512-circuit batch took 181, 157 and 162 ms on 1, 2
+// and 4 -- the per-distance launches each fill the GPU, so two overlap the
+// tail of one with the start of the next without splitting CUs further).
 int batch_streams() {
   const char *e = getenv("QCF_BATCH_STREAMS");
-  const int v = e ? atoi(e) : kBatchStreams;
+  const int v = e ? atoi(e) : 2;
   return std::max(1, std::min(kBatchStreams, v));
 }
 
