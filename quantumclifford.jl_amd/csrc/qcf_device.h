@@ -71,6 +71,10 @@ __device__ __forceinline__ uint32_t bperm(uint32_t src_lane, uint32_t v) {
   return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(src_lane << 2), (int)v);
 }
 
+// A value barrier: v must be present here, so a load into v is waited for at
+// this point rather than wherever the register is next written.
+__device__ __forceinline__ void consume(uint32_t &v) { asm volatile("" : "+v"(v)); }
+
 // Inclusive prefix sum over the 64 lanes with DPP moves only: shifts by
 // 1, 2, 4, 8 inside each row of 16 lanes, then lane 15 -> next row and lane 31
 // -> upper half.  Lanes without a source read 0.
@@ -104,29 +108,39 @@ __device__ __forceinline__ EvState ev_init(uint64_t seed, uint32_t tile, uint32_
   return e;
 }
 
-// Draw the next 64 events: one geometric gap per lane (failures before the
-// next event, floor(log2(u) * c) with c = 1/log2(1-p)) and a wave prefix sum.
-// Gaps beyond 2^24 trials advance without an event (memorylessness keeps the
-// process exact).
-__device__ __forceinline__ EvState gen_batch(EvState ev, float c) {
-  const uint32_t r0 = ev.rng.next();
-  const uint32_t r1 = ev.rng.next();
-  const uint32_t r2 = ev.rng.next();
+// One lane's draws for the next batch: the geometric gap to its event
+// (failures before it, floor(log2(u) * c) with c = 1/log2(1-p)), the uniform
+// for the Pauli choice and the 31-bit uniform for thinning.  Gaps beyond 2^24
+// trials advance without an event (null, uacc = 0xFFFFFFFF; memorylessness
+// keeps the process exact).
+struct Draw {
+  uint32_t incr, upau, uacc;
+};
+__device__ __forceinline__ Draw draw_event(Xoshiro &rng, float c) {
+  const uint32_t r0 = rng.next();
+  const uint32_t r1 = rng.next();
+  const uint32_t r2 = rng.next();
   const float uf = ((float)(r0 >> 8) + 0.5f) * (1.0f / 16777216.0f);
   const float g = __log2f(uf) * c;
-  uint32_t incr;
-  bool null_ev;
+  Draw d;
   if (!(g < 16777216.0f)) {
-    incr = 1u << 24;
-    null_ev = true;
+    d.incr = 1u << 24;
+    d.uacc = 0xFFFFFFFFu;
   } else {
-    incr = (uint32_t)g + 1u;
-    null_ev = false;
+    d.incr = (uint32_t)g + 1u;
+    d.uacc = r2 >> 1;
   }
-  ev.pos = ev.last + wave_inclusive_scan(incr);
+  d.upau = r1;
+  return d;
+}
+
+// Draw the next 64 events: one gap per lane and a wave prefix sum.
+__device__ __forceinline__ EvState gen_batch(EvState ev, float c) {
+  const Draw d = draw_event(ev.rng, c);
+  ev.pos = ev.last + wave_inclusive_scan(d.incr);
   ev.last = readlane(ev.pos, 63);
-  ev.upau = r1;
-  ev.uacc = null_ev ? 0xFFFFFFFFu : (r2 >> 1);
+  ev.upau = d.upau;
+  ev.uacc = d.uacc;
   return ev;
 }
 
@@ -164,9 +178,14 @@ __device__ __forceinline__ void group_events(EvState &ev, uint32_t ord0, uint32_
     const bool inr = rel < span;
     if (__builtin_amdgcn_ballot_w64(inr && ev.uacc != 0xFFFFFFFFu) != 0) {
       const uint32_t src = base + (inr ? (rel >> 11) : 0u);
-      const uint32_t w0 = bperm(src, e0), w1 = bperm(src, e1);
-      const uint32_t w2 = bperm(src, e2), w3 = bperm(src, e3);
-      const uint32_t py = bperm(src, pay);
+      uint32_t w0 = bperm(src, e0), w1 = bperm(src, e1);
+      uint32_t w2 = bperm(src, e2), w3 = bperm(src, e3);
+      uint32_t py = bperm(src, pay);
+      consume(w0);
+      consume(w1);
+      consume(w2);
+      consume(w3);
+      consume(py);
       if (inr && ev.uacc < w3) {
         const uint32_t within = rel & (kTileShots - 1u);
         const uint32_t tl = within >> 5;
@@ -190,50 +209,98 @@ __device__ __forceinline__ void group_events(EvState &ev, uint32_t ord0, uint32_
 // W2L/W3L, else the group's common value.  TWO / ONE: every site of the
 // group acts on two / on one qubit.  Same events and flips as group_events;
 // the flip callback must accept a zero bit (a no-op XOR).
+//
+// The site entries of the first batch come from prefetch_r, called with the
+// same event state earlier in the kernel (before the gates that precede the
+// pass): LDS operations complete in order, so that permute is back without
+// waiting for the gate XORs issued after it.  Later batches permute here.
+template <bool P8>
+struct PrefR {
+  uint32_t a, b, t, acc;
+};
+
+template <bool TWO, bool ONE, bool W2L, bool W3L, bool P8>
+__device__ __forceinline__ PrefR<P8> prefetch_r(const EvState &ev, uint32_t ord0, uint32_t G,
+                                                uint32_t pa, uint32_t pb, uint32_t w2,
+                                                uint32_t w3) {
+  const uint32_t rel = ev.pos - ord0 * kTileShots;
+  const uint32_t src = rel < G * kTileShots ? (rel >> 11) : 0u;
+  PrefR<P8> p;
+  p.a = bperm(src, pa);
+  p.b = P8 ? 0u : bperm(src, pb);
+  p.t = W2L ? bperm(src, w2) : w2;
+  p.acc = W3L ? bperm(src, w3) : w3;
+  // keep the scheduler from sinking the permutes to their use, behind the
+  // gates that follow
+  __builtin_amdgcn_sched_barrier(0);
+  return p;
+}
+
+template <bool TWO, bool ONE, bool W2L, bool W3L, bool P8, class Flip>
+__device__ __forceinline__ void flips_r(const EvState &ev, uint32_t rel, uint32_t a, uint32_t b,
+                                        uint32_t t, Flip flip) {
+  const uint32_t within = rel & (kTileShots - 1u);
+  const uint32_t tl = within >> 5;
+  const uint32_t bit = 1u << (within & 31u);
+  const uint32_t x1 = P8 ? (a & 0xFFu) : (a & 0xFFFFu);
+  const uint32_t z1 = P8 ? ((a >> 8) & 0xFFu) : (a >> 16);
+  const uint32_t q2 = P8 ? (a >> 16) : b;
+  const bool two = TWO ? true : ONE ? false : q2 != 0u;
+  uint32_t fl;
+  if (two) {
+    fl = __umulhi(ev.upau, 15u) + 1u;
+  } else {
+    const uint32_t u15 = ev.upau >> 17, t1 = t & 0xFFFFu, t2 = t >> 16;
+    fl = (u15 < t1 || u15 >= t2 ? 1u : 0u) | (u15 >= t1 ? 2u : 0u);
+  }
+  // every plane of the site takes an XOR, with zero where the Pauli leaves
+  // it alone: no divergent branch per plane
+  flip(x1, tl, (fl & 1u) ? bit : 0u, 0u, 0u);
+  flip(z1, tl, (fl & 2u) ? bit : 0u, 1u, 0u);
+  if (!ONE) {
+    const uint32_t x2 = P8 ? (q2 & 0xFFu) : (q2 & 0xFFFFu);
+    const uint32_t z2 = P8 ? (q2 >> 8) : (q2 >> 16);
+    flip(x2, tl, (fl & 4u) ? bit : 0u, 2u, 0u);
+    flip(z2, tl, (fl & 8u) ? bit : 0u, 3u, 0u);
+  }
+}
+
 template <bool TWO, bool ONE, bool W2L, bool W3L, bool P8, class Flip>
 __device__ __forceinline__ void group_events_r(EvState &ev, uint32_t ord0, uint32_t G,
                                                uint32_t pa, uint32_t pb, uint32_t w2,
-                                               uint32_t w3, float c, Flip flip) {
+                                               uint32_t w3, float c, PrefR<P8> pf, Flip flip) {
   const uint32_t lo = ord0 * kTileShots;
   const uint32_t span = G * kTileShots;
-  while (true) {
+  // the prefetched values are used on every path, so no permute is left
+  // pending where control flow joins
+  consume(pf.a);
+  if (!P8) consume(pf.b);
+  if (W2L) consume(pf.t);
+  if (W3L) consume(pf.acc);
+  {
+    const uint32_t rel = ev.pos - lo;
+    if (rel < span && ev.uacc < pf.acc)
+      flips_r<TWO, ONE, W2L, W3L, P8>(ev, rel, pf.a, pf.b, pf.t, flip);
+  }
+  while (ev.last < lo + span) {
+    ev = gen_batch(ev, c);
     const uint32_t rel = ev.pos - lo;
     const bool inr = rel < span;
     if (__builtin_amdgcn_ballot_w64(inr && ev.uacc != 0xFFFFFFFFu) != 0) {
       const uint32_t src = inr ? (rel >> 11) : 0u;
-      const uint32_t a = bperm(src, pa);
-      const uint32_t b = P8 ? 0u : bperm(src, pb);
-      const uint32_t t = W2L ? bperm(src, w2) : w2;
-      const uint32_t acc = W3L ? bperm(src, w3) : w3;
-      if (inr && ev.uacc < acc) {
-        const uint32_t within = rel & (kTileShots - 1u);
-        const uint32_t tl = within >> 5;
-        const uint32_t bit = 1u << (within & 31u);
-        const uint32_t x1 = P8 ? (a & 0xFFu) : (a & 0xFFFFu);
-        const uint32_t z1 = P8 ? ((a >> 8) & 0xFFu) : (a >> 16);
-        const uint32_t q2 = P8 ? (a >> 16) : b;
-        const bool two = TWO ? true : ONE ? false : q2 != 0u;
-        uint32_t fl;
-        if (two) {
-          fl = __umulhi(ev.upau, 15u) + 1u;
-        } else {
-          const uint32_t u15 = ev.upau >> 17, t1 = t & 0xFFFFu, t2 = t >> 16;
-          fl = (u15 < t1 || u15 >= t2 ? 1u : 0u) | (u15 >= t1 ? 2u : 0u);
-        }
-        // every plane of the site takes an XOR, with zero where the Pauli
-        // leaves it alone: no divergent branch per plane
-        flip(x1, tl, (fl & 1u) ? bit : 0u, 0u, 0u);
-        flip(z1, tl, (fl & 2u) ? bit : 0u, 1u, 0u);
-        if (!ONE) {
-          const uint32_t x2 = P8 ? (q2 & 0xFFu) : (q2 & 0xFFFFu);
-          const uint32_t z2 = P8 ? (q2 >> 8) : (q2 >> 16);
-          flip(x2, tl, (fl & 4u) ? bit : 0u, 2u, 0u);
-          flip(z2, tl, (fl & 8u) ? bit : 0u, 3u, 0u);
-        }
-      }
+      uint32_t a = bperm(src, pa);
+      uint32_t b = P8 ? 0u : bperm(src, pb);
+      uint32_t t = W2L ? bperm(src, w2) : w2;
+      uint32_t acc = W3L ? bperm(src, w3) : w3;
+      // the permuted values are used here, on every path: no LDS result is
+      // left pending where the loop branches back, so the loop head (and the
+      // code after the pass) does not wait for every outstanding LDS write
+      consume(a);
+      if (!P8) consume(b);
+      if (W2L) consume(t);
+      if (W3L) consume(acc);
+      if (inr && ev.uacc < acc) flips_r<TWO, ONE, W2L, W3L, P8>(ev, rel, a, b, t, flip);
     }
-    if (ev.last >= lo + span) break;
-    ev = gen_batch(ev, c);
   }
 }
 

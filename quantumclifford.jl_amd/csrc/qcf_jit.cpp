@@ -541,6 +541,101 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     return "atomicXor(&" + plane(lp_of(p)) + ", " + v + ");";
   };
   auto slot = [&](uint32_t k) { return plane(slot_base + k); };
+  // Read hoisting (LDS-resident planes): hz maps a plane to the register that
+  // holds the value read at the start of the current run of instructions.
+  const uint32_t hoist_cap = glob ? 0u : jit_hoist();
+  std::unordered_map<uint32_t, std::string> hz;
+  auto RD = [&](uint32_t p) -> std::string {
+    const auto it = hz.find(p);
+    return it != hz.end() ? it->second : PL(p);
+  };
+  // planes an instruction reads and writes; false for instructions that end
+  // a run (noise groups)
+  auto plane_rw = [](const Instr &in, std::vector<uint32_t> *rd, std::vector<uint32_t> *wr) {
+    auto R = [&](uint32_t p) { if (p != kNone) rd->push_back(p); };
+    auto W = [&](uint32_t p) { if (p != kNone) wr->push_back(p); };
+    switch (in.op & 0xFFu) {
+      case I_XORN: {
+        const uint32_t *w = &in.a;
+        for (uint32_t k = 0; k < (in.op >> 8); ++k) R(w[k] >> 16), W(w[k] & 0xFFFFu);
+        return true;
+      }
+      case I_MR: R(in.a), W(in.a), W(in.b), W(in.e); return true;
+      case I_MRX: R(in.a), R(in.f), W(in.a), W(in.b), W(in.e); return true;
+      case I_MEAS: R(in.a), W(in.c); return true;
+      case I_XSTORE2: R(in.a), R(in.b), W(in.d); return true;
+      case I_RESETZ:
+      case I_INIT: W(in.a), W(in.b); return true;
+      case I_ACC_LDS:
+      case I_ACCX_LDS:
+      case I_STORE: R(in.a); return true;
+      case I_ACC_STORE: W(in.b); return true;
+      case I_LIN1: R(in.a), R(in.b), W(in.a), W(in.b); return true;
+      case I_ZERO:
+      case I_LOAD: W(in.a); return true;
+      case I_REC_ZERO:
+      case I_ACC_REC:
+      case I_ACCX_REC: return true;
+      default: return false;
+    }
+  };
+  // The run starting at code[i] (ending before `end`): the longest stretch of
+  // hoistable instructions in which no LDS plane is read after a write of it,
+  // within hoist_cap planes.  Emits the reads and fills hz; returns the end.
+  auto open_run = [&](size_t i, size_t end, std::ostringstream &body) -> size_t {
+    std::vector<uint32_t> order, written, rd, wr;
+    size_t j = i;
+    for (; j < end; ++j) {
+      rd.clear();
+      wr.clear();
+      if (!plane_rw(code[j], &rd, &wr)) break;
+      bool ok = true;
+      std::vector<uint32_t> add;
+      for (uint32_t p : rd) {
+        if (p >= P->nplanes || in_reg(p)) continue;
+        if (std::find(written.begin(), written.end(), p) != written.end()) ok = false;
+        else if (std::find(order.begin(), order.end(), p) == order.end() &&
+                 std::find(add.begin(), add.end(), p) == add.end())
+          add.push_back(p);
+      }
+      if (!ok || order.size() + add.size() > hoist_cap) break;
+      order.insert(order.end(), add.begin(), add.end());
+      for (uint32_t p : wr)
+        if (p < P->nplanes && !in_reg(p)) written.push_back(p);
+    }
+    if (j == i || order.empty()) return i;
+    body << "    {";
+    for (uint32_t p : order) {
+      const std::string h = "h" + std::to_string(p);
+      body << " const uint32_t " << h << " = " << PL(p) << ";";
+      hz[p] = h;
+    }
+    body << "\n";
+    return j;
+  };
+  // event passes over register-held site entries: template arguments, call
+  // arguments, and the permute of the first batch (prefetch_r)
+  const bool prefetch_on = jit_prefetch();
+  auto q = [](uint32_t r) { return "q" + std::to_string(r); };
+  auto pass_tmpl = [&](size_t gi) {
+    const GDesc &d = gd[gi];
+    std::ostringstream o;
+    o << "<" << d.two << ", " << d.one << ", " << d.w2l << ", " << d.w3l << ", " << p8 << ">";
+    return o.str();
+  };
+  auto pass_args = [&](size_t gi, bool in_loop) {
+    const GDesc &d = gd[gi];
+    std::ostringstream o;
+    o << "ev, " << groups[gi].ord0 << (in_loop ? "u + ob, " : "u, ") << groups[gi].count << "u, "
+      << q(d.ra) << ", " << (p8 ? "0u" : q(d.rb)) << ", "
+      << (d.w2l ? q(d.r2) : std::to_string(d.w2c) + "u") << ", "
+      << (d.w3l ? q(d.r3) : std::to_string(d.w3c) + "u");
+    return o.str();
+  };
+  auto emit_prefetch = [&](size_t gi, bool in_loop, std::ostringstream &body) {
+    body << "    const qcfd::PrefR<" << p8 << "> pf" << gi << " = qcfd::prefetch_r" << pass_tmpl(gi)
+         << "(" << pass_args(gi, in_loop) << ");\n";
+  };
   auto load_group = [&](size_t g, const char *pfx) {
     std::ostringstream o;
     const uint32_t off = groups[g].site0, G = groups[g].count;
@@ -591,7 +686,14 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     size_t gi = range_groups[ri];
     const size_t gend = range_groups[ri + 1];
     if (gi < gend && !preload) body << load_group(gi, "t");
+    bool prefetched = false;  // the pass of group gi has its prefetch emitted
+    if (gi < gend && preload && prefetch_on) {
+      emit_prefetch(gi, in_loop, body);
+      prefetched = true;
+    }
     bool lh_open = false;  // half mode: inside a run guarded by "lanes 0..31"
+    size_t run_end = rg.b;  // end of the current read-hoisting run
+    bool run_open = false;
     for (size_t i = rg.b; i < rg.e; ++i) {
       const Instr &in = code[i];
       const uint32_t op = in.op & 0xFFu;
@@ -624,13 +726,17 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
           body << ");\n";
         }
       }
+      if (hoist_cap && !run_open) {
+        run_end = open_run(i, rg.e, body);
+        run_open = run_end > i;
+      }
       switch (op) {
         case I_XORN: {
           const uint32_t n = in.op >> 8;
           const uint32_t *w = &in.a;
           body << "    {";
           for (uint32_t k = 0; k < n; ++k)
-            body << " const uint32_t v" << k << " = " << PL(w[k] >> 16) << ";";
+            body << " const uint32_t v" << k << " = " << RD(w[k] >> 16) << ";";
           for (uint32_t k = 0; k < n; ++k)
             body << " " << XOR(w[k] & 0xFFFFu, "v" + std::to_string(k));
           body << " }\n";
@@ -642,13 +748,10 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
             const Group &g = groups[gi];
             body << "    {\n";
             if (preload) {
-              const GDesc &d = gd[gi];
-              auto q = [](uint32_t r) { return "q" + std::to_string(r); };
-              body << "    if (P.has_class) qcfd::group_events_r<" << d.two << ", " << d.one
-                   << ", " << d.w2l << ", " << d.w3l << ", " << p8 << ">(ev, " << g.ord0
-                   << (in_loop ? "u + ob, " : "u, ") << g.count << "u, " << q(d.ra) << ", "
-                   << (p8 ? "0u" : q(d.rb)) << ", " << (d.w2l ? q(d.r2) : std::to_string(d.w2c) + "u")
-                   << ", " << (d.w3l ? q(d.r3) : std::to_string(d.w3c) + "u") << ", P.c, flip);\n";
+              if (!prefetched) emit_prefetch(gi, in_loop, body);
+              body << "    if (P.has_class) qcfd::group_events_r" << pass_tmpl(gi) << "("
+                   << pass_args(gi, in_loop) << ", P.c, pf" << gi << ", flip);\n";
+              prefetched = false;
             } else {
               if (gi + 1 < gend)
                 body << "    uint32_t n0, n1, n2, n3;\n" << load_group(gi + 1, "n");
@@ -685,6 +788,12 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
             }
             body << "    }\n";
             done += g.count;
+            // the next pass's first permute goes here, ahead of the gates
+            // between the two passes
+            if (preload && prefetch_on && gi + 1 < gend) {
+              emit_prefetch(gi + 1, in_loop, body);
+              prefetched = true;
+            }
           }
           i += G;
           break;
@@ -692,21 +801,21 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
         case I_MR:
         case I_MRX: {
           const uint32_t col = in.d, keep = in.e;
-          body << "    { const uint32_t v = " << PL(a) << ";";
+          body << "    { const uint32_t v = " << RD(a) << ";";
           if (col != kNone) body << " " << st_rec(col, "v");
-          if (op == I_MRX) body << " " << st_rec(in.g, "v ^ " + PL(in.f));
+          if (op == I_MRX) body << " " << st_rec(in.g, "v ^ " + RD(in.f));
           if (keep != kNone) body << " " << WR(keep, "v");
           body << " " << WR(a, "0u") << " " << XOR(b, "g.next()") << " }\n";
           break;
         }
         case I_XSTORE2:
-          body << "    { const uint32_t v = " << PL(a) << " ^ " << PL(b) << "; "
+          body << "    { const uint32_t v = " << RD(a) << " ^ " << RD(b) << "; "
                << st_rec(c, "v");
           if (in.d != kNone) body << " " << WR(in.d, "v");
           body << " }\n";
           break;
         case I_MEAS:
-          body << "    { const uint32_t v = " << PL(a) << "; " << st_rec(b, "v");
+          body << "    { const uint32_t v = " << RD(a) << "; " << st_rec(b, "v");
           if (c != kNone) body << " " << WR(c, "v");
           body << " }\n";
           break;
@@ -714,10 +823,10 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
           body << "    " << WR(a, "0u") << " " << XOR(b, "g.next()") << "\n";
           break;
         case I_ACC_LDS:
-          body << "    acc = " << PL(a) << ";\n";
+          body << "    acc = " << RD(a) << ";\n";
           break;
         case I_ACCX_LDS:
-          body << "    acc ^= " << PL(a) << ";\n";
+          body << "    acc ^= " << RD(a) << ";\n";
           break;
         case I_ACC_REC:
           body << "    acc = " << ld_rec(a) << ";\n";
@@ -743,14 +852,14 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
                << "\n";
           break;
         case I_STORE:
-          body << "    " << (tail ? "if (live) " : "") << at("P.frame", b, false) << " = " << PL(a)
+          body << "    " << (tail ? "if (live) " : "") << at("P.frame", b, false) << " = " << RD(a)
                << (tail ? " & lm;\n" : ";\n");
           break;
         case I_ZERO:
           body << "    " << WR(a, "0u") << "\n";
           break;
         case I_LIN1:
-          body << "    { const uint32_t x = " << PL(a) << ", z = " << PL(b) << "; "
+          body << "    { const uint32_t x = " << RD(a) << ", z = " << RD(b) << "; "
                << WR(a, std::string((c & 1u) ? "x" : "0u") + " ^ " + ((c & 2u) ? "z" : "0u"))
                << " "
                << WR(b, std::string((c & 4u) ? "x" : "0u") + " ^ " + ((c & 8u) ? "z" : "0u"))
@@ -758,6 +867,11 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
           break;
         default:
           return false;
+      }
+      if (run_open && i + 1 >= run_end) {
+        body << "    }\n";
+        hz.clear();
+        run_open = false;
       }
     }
     if (lh_open) body << "    }\n";
@@ -865,11 +979,10 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     const uint32_t gw4 = gw * 4u;
     (void)ld4; (void)recb; (void)gw4;
     qcfd::Xoshiro g = qcfd::gauge_stream(P.seed, gw);
-    qcfd::EvState ev = qcfd::ev_init(P.seed, blk, l);
-    if (P.has_class) ev = qcfd::gen_batch(ev, P.c);
-    uint32_t acc = 0u;
-    (void)acc;
 )QCFJIT";
+  src << "    qcfd::EvState ev = qcfd::ev_init(P.seed, blk, l);\n"
+         "    if (P.has_class) ev = qcfd::gen_batch(ev, P.c);\n"
+         "    uint32_t acc = 0u;\n    (void)acc;\n";
   if (halfm)
     src << "    const bool live = lh && gw < P.nwbuf;\n"
         << "    const uint32_t lm = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1u ? P.lastmask : "
@@ -1019,13 +1132,47 @@ bool jit_disabled() {
   return e && e[0] == '1';
 }
 
+// LDS-resident planes read by a run of gates and measurements go first, all
+// of them before the run's first write, as long as no read follows a write
+// of the same plane inside the run: one LDS round trip per run instead of
+// one per 7-XOR bundle and per measurement.  QCF_JIT_HOIST caps the planes
+// one run may hold in registers (0 = off; default 64).
+uint32_t jit_hoist() {
+  const char *e = getenv("QCF_JIT_HOIST");
+  return e ? (uint32_t)strtoul(e, nullptr, 10) : 64u;
+}
+
+// QCF_JIT_PREFETCH=0 leaves the first site-entry permute of every noise pass
+// at the pass instead of after the previous pass.
+bool jit_prefetch() {
+  const char *e = getenv("QCF_JIT_PREFETCH");
+  return !(e && e[0] == '0');
+}
+
+// The values of every knob generate() reads: a kernel compiled under other
+// values is not reused.
+std::string jit_knob_signature() {
+  static const char *const kKnobs[] = {
+      "QCF_JIT_REG_EVENTS", "QCF_JIT_SLOTS", "QCF_JIT_REG_PLANES", "QCF_JIT_MAX_INSTR",
+      "QCF_JIT_SPLIT_REGS", "QCF_JIT_SPLIT_BARRIER", "QCF_JIT_SPLIT_TOUCHED", "QCF_JIT_NT",
+      "QCF_JIT_NO_LOOPS", "QCF_JIT_HOIST", "QCF_JIT_PREFETCH"};
+  std::string s;
+  for (const char *k : kKnobs) {
+    const char *e = getenv(k);
+    s += e ? e : "-";
+    s += '|';
+  }
+  return s;
+}
+
 int jit_get(qcf_program *P, int variant, JitKernelInfo *info) {
   *info = JitKernelInfo{};
   int dev = 0;
   QCF_HIP_CHECK(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lock(g_jit_mu);
+  const std::string knobs = jit_knob_signature();
   for (auto &j : P->jit)
-    if (j.device == dev && j.variant == variant) {
+    if (j.device == dev && j.variant == variant && j.knobs == knobs) {
       if (j.failed) return set_error(QCF_ERR_UNSUPPORTED, "specialised kernel unavailable");
       info->function = j.function;
       info->lds_bytes = j.lds_bytes;
@@ -1036,6 +1183,7 @@ int jit_get(qcf_program *P, int variant, JitKernelInfo *info) {
   qcf_program::JitKernel jk;
   jk.device = dev;
   jk.variant = variant;
+  jk.knobs = knobs;
   std::string src, log;
   std::vector<char> bin;
   int st = QCF_OK;

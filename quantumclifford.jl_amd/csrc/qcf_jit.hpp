@@ -45,6 +45,8 @@ uint32_t jit_xcds();
 bool jit_full_grid();
 uint32_t jit_slots();
 size_t jit_max_instr();
+uint32_t jit_hoist();
+bool jit_prefetch();
 
 struct JitKernelInfo {
   void *function = nullptr;     // hipFunction_t

@@ -10,6 +10,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "../../include/qcf.h"
@@ -122,6 +123,7 @@ struct qcf_program {
     uint32_t lds_bytes = 0;
     uint32_t blocks_per_cu = 0;
     uint64_t key = 0;  // hash of the kernel source
+    std::string knobs;  // generator knob values it was built under
     bool failed = false;
   };
   std::vector<JitKernel> jit;
