@@ -66,7 +66,7 @@ def bench_frames(steps, warmup, shots=10**7):
 
     def step():
         seed[0] += 1
-        qcf._lib.call("qcf_pf_sample", prog.handle, shots, seed[0], None,
+        qcf._lib.call("qcf_pf_sample", prog.handle, shots, 0, seed[0], None,
                       C.c_void_p(rec.data_ptr()), None, ld, 1,
                       C.c_void_p(torch.cuda.current_stream().cuda_stream))
 
@@ -158,7 +158,7 @@ def bench_batched(ncirc=512, seed=0, dmax=15):
     def seq():
         st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
         for p, s, sd, r in zip(progs, shots, seeds, recs):
-            qcf._lib.call("qcf_pf_sample", p.handle, s, sd, None, C.c_void_p(r.data_ptr()),
+            qcf._lib.call("qcf_pf_sample", p.handle, s, 0, sd, None, C.c_void_p(r.data_ptr()),
                           None, qcf.frames.plane_words(s), 1, st)
 
     ms_s = time_launches(seq, 2, 1)
@@ -189,7 +189,7 @@ def bench_reference_circuits(steps, warmup):
 
         def step():
             seed[0] += 1
-            qcf._lib.call("qcf_pf_sample", prog.handle, shots, seed[0], None,
+            qcf._lib.call("qcf_pf_sample", prog.handle, shots, 0, seed[0], None,
                           C.c_void_p(rec.data_ptr()), None, ld, 1,
                           C.c_void_p(torch.cuda.current_stream().cuda_stream))
 

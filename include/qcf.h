@@ -214,6 +214,11 @@ int qcf_program_get_info(const qcf_program *prog, qcf_program_info *info);
                                   record zeroed (pftrajectories(circuit; trajectories)) */
 #define QCF_PF_EXPORT_MASKS 2u /* native RNG, and write every drawn mask to `masks`
                                   (which must be zeroed by the caller)            */
+#define QCF_PF_VIEW 4u         /* the planes' stride also holds shots outside this
+                                  launch (view(frame, r)): write only the launch's own
+                                  64-shot words, never the zero padding up to the
+                                  stride.  `shots` must then end on a 64-shot word or
+                                  at the end of the stride's shots.                 */
 
 /*
  * Run the circuit on `shots` frames: pftrajectories(state::PauliFrame, circuit)
@@ -223,21 +228,35 @@ int qcf_program_get_info(const qcf_program *prog, qcf_program_info *info);
  *   frame  : 2*info.nqubits planes of `ld` words (in/out; may be NULL with
  *            QCF_PF_FRESH when the final frame is not wanted)
  *   meas   : info.nbits columns of `ld` words (in/out)
- *   masks  : NULL for the native counter-based RNG (Philox4x32-10 keyed on
- *            `seed` and the absolute shot word); otherwise info.nreplay planes
- *            of `ld` words that replace every random draw in program order --
- *            the replayed-mask mode.  Planes 0..nqubits-1 are the initZ! z
- *            words of qubits 1..nqubits (read only with QCF_PF_FRESH); then
- *            one plane per sMRZ z-flip, two (x flip, z flip) per
- *            single-qubit noise site and four (x1,z1,x2,z2) per DEPOLARIZE2
- *            site, in op order.  With QCF_PF_EXPORT_MASKS the native draws
- *            are written there instead.
+ *   masks  : NULL for the native counter-based RNG; otherwise info.nreplay
+ *            planes of `ld` words that replace every random draw in program
+ *            order -- the replayed-mask mode.  Planes 0..nqubits-1 are the
+ *            initZ! z words of qubits 1..nqubits (read only with
+ *            QCF_PF_FRESH); then one plane per sMRZ z-flip, two (x flip,
+ *            z flip) per single-qubit noise site and four (x1,z1,x2,z2) per
+ *            DEPOLARIZE2 site, in op order.  With QCF_PF_EXPORT_MASKS the
+ *            native draws are written there instead.
  *   ld     : words per plane, >= ceil(shots/64); qcf_plane_words(shots)
  *            is the recommended stride
- * Results do not depend on launch shape: the RNG stream of a shot depends only
- * on (seed, shot index).
+ *   shot_offset : absolute index of this launch's first shot in the caller's
+ *            experiment (a multiple of 2048 for native draws; ignored when
+ *            masks are replayed).  The buffers hold only this launch's shots.
+ *
+ * Native draws (no reference counterpart: the reference draws from Julia's
+ * task-local Xoshiro, src/pauli_frames.jl:47-56,110-113,136-174):
+ *   - gauge words (initZ!, the sMRZ z-flip): one xoshiro128** stream per
+ *     absolute 32-shot word, seeded by Philox4x32-10(seed, word);
+ *   - noise events: one stream of event positions per absolute 2048-shot
+ *     tile along its (site ordinal, shot) sequence, drawn 64 at a time (one
+ *     geometric gap per lane from a per-lane xoshiro128** stream seeded by
+ *     Philox4x32-10(seed, tile, lane), then a wave prefix sum).
+ * So the bits of a shot depend on (seed, its absolute tile) and not on the
+ * launch shape, batching or kernel: a run split at multiples of 2048 shots
+ * (with shot_offset) reproduces the one-run bits.
+ * Errors: QCF_ERR_ARGUMENT for a native shot_offset that is not a multiple
+ * of 2048, QCF_ERR_DIMENSION for ld < ceil(shots/64).
  */
-int qcf_pf_sample(const qcf_program *prog, uint64_t shots, uint64_t seed,
+int qcf_pf_sample(const qcf_program *prog, uint64_t shots, uint64_t shot_offset, uint64_t seed,
                   uint64_t *frame, uint64_t *meas, uint64_t *masks, size_t ld,
                   uint32_t flags, void *stream);
 

@@ -390,31 +390,11 @@ __global__ void canon_ranks(const CanonState *st, uint32_t *ranks) {
   }
 }
 
-// Workspace per device, grown on demand (owned by the library).
-struct CanonScratch {
-  void *ptr = nullptr;
-  size_t bytes = 0;
-};
-
-int canon_scratch(size_t bytes, char **out) {
-  static std::mutex mu;
-  static std::vector<CanonScratch> per_dev;
-  int dev = 0;
-  QCF_HIP_CHECK(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lock(mu);
-  if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1);
-  CanonScratch &cs = per_dev[dev];
-  if (cs.bytes < bytes) {
-    if (cs.ptr) {
-      QCF_HIP_CHECK(hipDeviceSynchronize());
-      QCF_HIP_CHECK(hipFree(cs.ptr));
-      cs.ptr = nullptr;
-      cs.bytes = 0;
-    }
-    QCF_HIP_CHECK(hipMalloc(&cs.ptr, bytes));
-    cs.bytes = bytes;
-  }
-  *out = (char *)cs.ptr;
+// Workspace of the launch's stream (library-owned, grown on demand).
+int canon_scratch(hipStream_t stream, size_t bytes, char **out) {
+  void *p = nullptr;
+  QCF_TRY(stream_scratch(kScratchCanon, stream, bytes, &p));
+  *out = (char *)p;
   return QCF_OK;
 }
 
@@ -523,7 +503,7 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
   const size_t o_tph = off;
   off = align256(off + nrows);
   char *base = nullptr;
-  QCF_TRY(canon_scratch(off, &base));
+  QCF_TRY(canon_scratch(stream, off, &base));
   CanonState *st = (CanonState *)(base + o_st);
   uint32_t *inv = (uint32_t *)(base + o_inv);
   uint64_t *mask = (uint64_t *)(base + o_mask);

@@ -42,6 +42,15 @@ struct DeviceState {
 constexpr uint32_t kCounterSlots = 4096;
 int device_state(DeviceState **out);
 
+// Library-owned device workspace, one per (device, stream, kind): calls on one
+// stream run in order, so they never use a workspace at the same time, and
+// calls on different streams have different ones.  It grows on demand behind
+// a synchronization of that stream only; it cannot grow while the stream is
+// being captured into a graph (QCF_ERR_ARGUMENT: make one uncaptured call of
+// the same size first).  Workspaces live until the process ends.
+enum ScratchKind : int { kScratchFrames = 0, kScratchCanon = 1 };
+int stream_scratch(int kind, hipStream_t stream, size_t bytes, void **out);
+
 // ------------------------------------------------------------------ RNG --
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
 // as 1, 2, 3", SC'11): 10 rounds of the 4x32 multiply-xorshift round with the

@@ -39,6 +39,7 @@ struct KParams {
   uint32_t ninstr;
   uint32_t nblocks;    // 2048-shot tiles (last tile index + 1)
   uint32_t blk0;       // first tile of this launch
+  uint32_t tile0;      // absolute index of tile 0 (shot_offset / 2048): keys the RNG
   uint32_t nw32;       // valid 32-bit shot words
   uint32_t nwbuf;      // 32-bit words of a plane the last tile may store (<= stride)
   uint32_t lastmask;   // valid bits of word nw32-1
@@ -115,8 +116,8 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
   const uint32_t lmask = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1 ? P.lastmask : 0xFFFFFFFFu);
   const uint64_t goff = gw;
   qcfd::Xoshiro gauge{0, 0, 0, 1};
-  if (MODE != kReplay) gauge = qcfd::gauge_stream(P.seed, gw);
-  EvState ev0 = qcfd::ev_init(P.seed, blk, lane);
+  if (MODE != kReplay) gauge = qcfd::gauge_stream(P.seed, gw + P.tile0 * 64u);
+  EvState ev0 = qcfd::ev_init(P.seed, blk + P.tile0, lane);
   if (MODE != kReplay && P.nclasses > 0) ev0 = qcfd::gen_batch(ev0, P.cls_c[0]);
   // Every global load in the instruction loop is waited for on the spot, so
   // no load is ever pending where control flow merges (which would make the
@@ -468,37 +469,20 @@ int launch_pf(const KParams &kp, uint32_t lds_bytes, uint32_t grid, hipStream_t 
   return QCF_OK;
 }
 
-// Frames too large for LDS live in a per-wave region of this device buffer
-// (grown on demand; only single-circuit launches use it, batches own theirs).
+// Frames too large for LDS live in a per-wave region of a library-owned
+// device workspace, one per stream (single-circuit launches; batches own
+// theirs).
 // (d=11 unrotated surface code, 884 planes, 10^7 shots: 8, 16, 24, 32 waves
 // per CU took 36.0, 29.4, 24.0, 23.9 ms; the waves wait on L2/MALL latency,
 // so more of them help until occupancy stops them)
 constexpr uint32_t kGlobalWavesPerCU = 24;
 constexpr uint32_t kMaxBatchWavesPerCU = 32;
 constexpr uint32_t kGlobalJitWavesPerCU = 16;  // specialised kernels, planes in global memory
-struct FrameScratch {
-  void *ptr = nullptr;
-  size_t bytes = 0;
-};
-int frame_scratch(size_t bytes, uint32_t **out) {
-  static std::mutex mu;
-  static std::vector<FrameScratch> per_dev;
-  int dev = 0;
-  QCF_HIP_CHECK(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lock(mu);
-  if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1);
-  FrameScratch &fs = per_dev[dev];
-  if (fs.bytes < bytes) {
-    if (fs.ptr) {
-      QCF_HIP_CHECK(hipDeviceSynchronize());
-      QCF_HIP_CHECK(hipFree(fs.ptr));
-      fs.ptr = nullptr;
-      fs.bytes = 0;
-    }
-    QCF_HIP_CHECK(hipMalloc(&fs.ptr, bytes));
-    fs.bytes = bytes;
-  }
-  *out = (uint32_t *)fs.ptr;
+// Frames in global memory use the workspace of the launch's stream.
+int frame_scratch(hipStream_t stream, size_t bytes, uint32_t **out) {
+  void *p = nullptr;
+  QCF_TRY(stream_scratch(kScratchFrames, stream, bytes, &p));
+  *out = (uint32_t *)p;
   return QCF_OK;
 }
 
@@ -575,6 +559,7 @@ int jit_plan(qcf_program *P, const KParams &kp, uint64_t shots, int variant, boo
   jp.nblocks = kp.nblocks;
   jp.nw32 = kp.nw32;
   jp.nwbuf = kp.nwbuf;
+  jp.tile0 = kp.tile0;
   jp.lastmask = kp.lastmask;
   jp.c = kp.cls_c[0];
   jp.has_class = kp.nclasses > 0 ? 1u : 0u;
@@ -608,7 +593,7 @@ int jit_plan(qcf_program *P, const KParams &kp, uint64_t shots, int variant, boo
 }
 
 // Shared with the batched launcher.
-int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
+int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t shot_offset, uint64_t seed,
                   uint64_t *frame, uint64_t *meas, uint64_t *masks, size_t ld,
                   uint32_t flags, uint32_t waves_per_cu, hipStream_t stream) {
   qcf_program *P = const_cast<qcf_program *>(prog);
@@ -626,6 +611,14 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
     return set_error(QCF_ERR_ARGUMENT, "pftrajectories: mask export needs a mask buffer");
   if (shots > (uint64_t)0xFFFFFFFFu * 16)
     return set_error(QCF_ERR_UNSUPPORTED, "pftrajectories: too many shots in one call");
+  if (!masks || exp) {  // the counter-based RNG is keyed on absolute tiles
+    if (shot_offset % kBlockShots)
+      return set_error(QCF_ERR_ARGUMENT,
+                       "pftrajectories: shot_offset %llu is not a multiple of %u shots",
+                       (unsigned long long)shot_offset, kBlockShots);
+    if ((shot_offset + shots + 31) / 32 > 0xFFFFFFFFull)
+      return set_error(QCF_ERR_UNSUPPORTED, "pftrajectories: shot_offset + shots beyond 2^37");
+  }
   int lds_max = 0;
   QCF_TRY(max_lds_per_block(&lds_max));
   const uint32_t lds_bytes = P->nplanes * 256u;
@@ -638,7 +631,11 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
   kp.nblocks = (uint32_t)((nw32 + 63) / 64);
   // lanes of the last tile that lie inside the plane stride store zeros past
   // the shots, so a padded stride leaves defined (zero) padding
-  kp.nwbuf = (uint32_t)std::min<uint64_t>(2 * (uint64_t)ld, 64ull * kp.nblocks);
+  // a view shares its plane stride with other shots: only its own words
+  kp.nwbuf = (flags & QCF_PF_VIEW)
+                 ? (uint32_t)(2 * ((shots + 63) / 64))
+                 : (uint32_t)std::min<uint64_t>(2 * (uint64_t)ld, 64ull * kp.nblocks);
+  kp.tile0 = (!masks || exp) ? (uint32_t)(shot_offset / kBlockShots) : 0u;
   const uint32_t tail = (uint32_t)(shots % 32);
   kp.lastmask = tail ? ((1u << tail) - 1u) : 0xFFFFFFFFu;
   kp.ld32 = 2 * (uint64_t)ld;
@@ -665,7 +662,7 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
     JitPlan pl;
     int js = jit_plan(P, kp, shots, variant, global, waves_per_cu, ld, &pl);
     if (js == QCF_OK && pl.global_planes) {
-      js = frame_scratch(pl.grid * (size_t)P->nplanes * 256u, &pl.jp.scratch);
+      js = frame_scratch(stream, pl.grid * (size_t)P->nplanes * 256u, &pl.jp.scratch);
       pl.jp.sstride = (uint64_t)P->nplanes * 64u;
     }
     if (js == QCF_OK) js = jit_launch(pl.function, pl.jp, pl.grid, pl.lds_bytes, stream);
@@ -675,7 +672,7 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t seed,
     clear_error();
   }
   if (global) {
-    QCF_TRY(frame_scratch(grid * (size_t)P->nplanes * 256u, &kp.scratch));
+    QCF_TRY(frame_scratch(stream, grid * (size_t)P->nplanes * 256u, &kp.scratch));
     kp.sstride = P->nplanes * 64u;
   }
   if (masks && !exp) return launch_pf<kReplay>(kp, lds_bytes, (uint32_t)grid, stream);
@@ -691,13 +688,13 @@ extern "C" size_t qcf_plane_words(uint64_t shots) {
   return (size_t)std::max<uint64_t>(1, (shots + 2047) / 2048) * 32;
 }
 
-extern "C" int qcf_pf_sample(const qcf_program *prog, uint64_t shots,
+extern "C" int qcf_pf_sample(const qcf_program *prog, uint64_t shots, uint64_t shot_offset,
                              uint64_t seed, uint64_t *frame, uint64_t *meas,
                              uint64_t *masks, size_t ld, uint32_t flags,
                              void *stream) {
   clear_error();
   if (!prog) return set_error(QCF_ERR_ARGUMENT, "null program");
-  return pf_launch_one(prog, shots, seed, frame, meas, masks, ld, flags, 0,
+  return pf_launch_one(prog, shots, shot_offset, seed, frame, meas, masks, ld, flags, 0,
                        (hipStream_t)stream);
 }
 

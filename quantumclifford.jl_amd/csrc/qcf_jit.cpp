@@ -51,6 +51,7 @@ struct KJ {
   const KJ *multi;
   const uint32_t *first;
   uint32_t ncirc, ntiles;
+  uint32_t tile0;     // absolute index of tile 0: keys the RNG
 };
 struct FlipLDS {
   uint32_t *L;
@@ -954,6 +955,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
       P.c = Q->c;
       P.has_class = Q->has_class;
       P.nfull = Q->nfull;
+      P.tile0 = Q->tile0;
       blk = tile - first[lo];
     }
 )QCFJIT";
@@ -978,9 +980,9 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     char *const recb = (char *)P.rec;
     const uint32_t gw4 = gw * 4u;
     (void)ld4; (void)recb; (void)gw4;
-    qcfd::Xoshiro g = qcfd::gauge_stream(P.seed, gw);
+    qcfd::Xoshiro g = qcfd::gauge_stream(P.seed, gw + P.tile0 * 64u);
 )QCFJIT";
-  src << "    qcfd::EvState ev = qcfd::ev_init(P.seed, blk, l);\n"
+  src << "    qcfd::EvState ev = qcfd::ev_init(P.seed, blk + P.tile0, l);\n"
          "    if (P.has_class) ev = qcfd::gen_batch(ev, P.c);\n"
          "    uint32_t acc = 0u;\n    (void)acc;\n";
   if (halfm)

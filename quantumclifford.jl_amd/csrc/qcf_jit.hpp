@@ -25,6 +25,7 @@ struct JitParams {
   const JitParams *multi;
   const uint32_t *first;
   uint32_t ncirc, ntiles;
+  uint32_t tile0;     // absolute index of tile 0: keys the RNG
 };
 
 // QCF_NO_JIT=1 keeps every launch on the interpreter.

@@ -46,6 +46,54 @@ int device_state(DeviceState **out) {
   return QCF_OK;
 }
 
+struct ScratchKey {
+  int dev, kind;
+  hipStream_t stream;
+  bool operator==(const ScratchKey &o) const {
+    return dev == o.dev && kind == o.kind && stream == o.stream;
+  }
+};
+struct ScratchBuf {
+  ScratchKey key;
+  void *ptr;
+  size_t bytes;
+};
+static std::mutex g_scratch_mu;
+static std::vector<ScratchBuf> g_scratch;
+
+int stream_scratch(int kind, hipStream_t stream, size_t bytes, void **out) {
+  int dev = 0;
+  QCF_HIP_CHECK(hipGetDevice(&dev));
+  const ScratchKey key{dev, kind, stream};
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  ScratchBuf *b = nullptr;
+  for (auto &s : g_scratch)
+    if (s.key == key) b = &s;
+  if (!b) {
+    g_scratch.push_back(ScratchBuf{key, nullptr, 0});
+    b = &g_scratch.back();
+  }
+  if (b->bytes < bytes) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    QCF_HIP_CHECK(hipStreamIsCapturing(stream, &cap));
+    if (cap != hipStreamCaptureStatusNone)
+      return set_error(QCF_ERR_ARGUMENT,
+                       "workspace of %zu bytes must grow to %zu while the stream is being "
+                       "captured; make one uncaptured call of this size first",
+                       b->bytes, bytes);
+    if (b->ptr) {
+      QCF_HIP_CHECK(hipStreamSynchronize(stream));  // the last user of it
+      QCF_HIP_CHECK(hipFree(b->ptr));
+      b->ptr = nullptr;
+      b->bytes = 0;
+    }
+    QCF_HIP_CHECK(hipMalloc(&b->ptr, bytes));
+    b->bytes = bytes;
+  }
+  *out = b->ptr;
+  return QCF_OK;
+}
+
 }  // namespace qcf
 
 extern "C" const char *qcf_last_error(void) {

@@ -16,7 +16,8 @@ from .pauli import (P, PauliOperator, S, Tableau, canonicalize_, mul_left_,
                     mul_right_tableau_row_)
 from .frames import (PauliFrame, Program, alloc_masks, apply_, applynoise_,
                      frame_from_reference, frame_to_reference, initZ_, lower_circuit,
-                     measurements, pfmeasurements, pftrajectories, pftrajectories_state, seed_)
+                     measurements, pfmeasurements, pftrajectories, pftrajectories_state, seed_,
+                     view)
 from . import batch, circuits, decoders, jltext, randoms
 
 __all__ = [n for n in dir() if not n.startswith("_")]

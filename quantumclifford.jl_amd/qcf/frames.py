@@ -139,6 +139,9 @@ class PauliFrame:
         alloc = torch.empty if _uninitialized else torch.zeros
         self.frame = alloc(2 * self.nq * self.ld, dtype=torch.int64, device=dev)
         self.record = alloc(max(1, self.nbits) * self.ld, dtype=torch.int64, device=dev)
+        # a view (view(frame, r)) shares its parent's planes: its shots start at
+        # word w0 of every plane and at absolute shot `offset`
+        self.offset, self.w0, self.is_view = 0, 0, False
         if init:
             initZ_(self)
 
@@ -149,26 +152,85 @@ class PauliFrame:
         return self.shots
 
     def copy(self) -> "PauliFrame":
+        """A frame of its own with this frame's shots (a view copies its range)."""
         f = PauliFrame.__new__(PauliFrame)
         f.shots, f.nq, f.nbits, f.ld = self.shots, self.nq, self.nbits, self.ld
-        f.frame = self.frame.clone()
-        f.record = self.record.clone()
+        f.offset, f.w0, f.is_view = self.offset, 0, False
+        if self.is_view:
+            f.ld = plane_words(self.shots)
+            f.frame = torch.zeros(2 * f.nq * f.ld, dtype=torch.int64, device=self.frame.device)
+            f.record = torch.zeros(max(1, f.nbits) * f.ld, dtype=torch.int64,
+                                   device=self.frame.device)
+            nw = words(self.shots)
+            f.frame.view(2 * f.nq, f.ld)[:, :nw].copy_(self._planes(self.frame, 2 * self.nq))
+            f.record.view(-1, f.ld)[:, :nw].copy_(self._planes(self.record, max(1, self.nbits)))
+        else:
+            f.frame = self.frame.clone()
+            f.record = self.record.clone()
         return f
+
+    def _planes(self, t: torch.Tensor, n: int) -> torch.Tensor:
+        """(n, words) view of this frame's shots in a plane buffer of the frame."""
+        nw = words(self.shots)
+        if not self.is_view:
+            return t.view(n, self.ld)[:, :nw]
+        base = t._qcf_base  # the parent's whole buffer
+        return base.view(n, self.ld)[:, self.w0:self.w0 + nw]
 
     # host views for tests ---------------------------------------------
     def frame_host(self) -> np.ndarray:
+        if self.is_view:
+            return self._planes(self.frame, 2 * self.nq).cpu().numpy().view(np.uint64)
         return self.frame.cpu().numpy().view(np.uint64).reshape(2 * self.nq, self.ld)
 
     def record_host(self) -> np.ndarray:
+        if self.is_view:
+            return self._planes(self.record, max(1, self.nbits)).cpu().numpy().view(
+                np.uint64)[: self.nbits]
         return self.record.cpu().numpy().view(np.uint64).reshape(-1, self.ld)[: self.nbits]
+
+
+def view(frame: PauliFrame, r) -> PauliFrame:
+    """view(frame, r) (src/pauli_frames.jl:17): the frames r (a 0-based range or
+    slice of shots) of `frame`, sharing its memory.  r must start on a 64-shot
+    word and end on one or at the last shot; a native pftrajectories on the
+    view keys its draws on the view's absolute shots (the offset of r within
+    the parent), so r must then also start on a multiple of 2048 shots."""
+    if isinstance(r, slice):
+        if r.step not in (None, 1):
+            raise _lib.ArgumentError("view(frame, r) needs a contiguous range of shots")
+        start, stop, _ = r.indices(frame.shots)
+    else:
+        if getattr(r, "step", 1) != 1:
+            raise _lib.ArgumentError("view(frame, r) needs a contiguous range of shots")
+        start, stop = r.start, r.stop
+    if not 0 <= start <= stop <= frame.shots:
+        raise _lib.BoundsError(f"view range {start}:{stop} outside 0:{frame.shots}")
+    if start % 64 or (stop % 64 and stop != frame.shots):
+        raise _lib.ArgumentError(
+            "view(frame, r): r must start on a multiple of 64 shots and end on one or at "
+            f"the last shot (got {start}:{stop})")
+    v = PauliFrame.__new__(PauliFrame)
+    v.shots, v.nq, v.nbits, v.ld = stop - start, frame.nq, frame.nbits, frame.ld
+    v.offset, v.w0, v.is_view = frame.offset + start, frame.w0 + start // 64, True
+    fb = frame.frame._qcf_base if frame.is_view else frame.frame
+    rb = frame.record._qcf_base if frame.is_view else frame.record
+    v.frame = fb[v.w0:]
+    v.frame._qcf_base = fb
+    v.record = rb[v.w0:]
+    v.record._qcf_base = rb
+    return v
 
 
 def initZ_(frame: PauliFrame, seed: int | None = None) -> PauliFrame:
     """initZ!: x = 0, z uniformly random on every frame qubit (src/pauli_frames.jl:47-56)."""
     prog = Program([], nqubits=frame.nq - 1, nbits=frame.nbits)
-    record = torch.empty_like(frame.record)  # initZ! leaves the record alone
-    call("qcf_pf_sample", prog.handle, frame.shots, _SEEDS.next() if seed is None else seed,
-         _ptr(frame.frame), _ptr(record), None, frame.ld, 1, _stream())
+    # initZ! leaves the record alone
+    record = torch.empty(max(1, frame.nbits) * frame.ld, dtype=torch.int64,
+                         device=frame.frame.device)
+    call("qcf_pf_sample", prog.handle, frame.shots, frame.offset,
+         _SEEDS.next() if seed is None else seed, _ptr(frame.frame), _ptr(record), None, frame.ld,
+         1 | (4 if frame.is_view else 0), _stream())
     return frame
 
 
@@ -181,19 +243,22 @@ def pftrajectories_state(state: PauliFrame, circuit, *, seed: int | None = None,
         raise _lib.BoundsError(
             f"circuit needs {prog.nq} frame qubits / {prog.nbits} bits; frame has "
             f"{state.nq} / {state.nbits}")
-    call("qcf_pf_sample", prog.handle, state.shots, _SEEDS.next() if seed is None else seed,
-         _ptr(state.frame), _ptr(state.record), _ptr(masks), state.ld, 0, _stream())
+    call("qcf_pf_sample", prog.handle, state.shots, state.offset,
+         _SEEDS.next() if seed is None else seed, _ptr(state.frame), _ptr(state.record),
+         _ptr(masks), state.ld, 4 if state.is_view else 0, _stream())
     return state
 
 
 def pftrajectories(circuit, trajectories: int = 5000, threads: bool = True, *,
                    seed: int | None = None, masks: torch.Tensor | None = None,
                    export_masks: torch.Tensor | None = None, keep_frame: bool = True,
-                   device=None) -> PauliFrame:
+                   device=None, shot_offset: int = 0) -> PauliFrame:
     """pftrajectories(circuit; trajectories, threads) (src/pauli_frames.jl:193-227):
     a fresh frame sized from the circuit, initZ!, then the circuit.  `threads` is
     accepted for signature compatibility (the GPU runs all shots in parallel).
-    `masks` replays pre-drawn randomness; `export_masks` records the native draws."""
+    `masks` replays pre-drawn randomness; `export_masks` records the native draws.
+    `shot_offset` (a multiple of 2048) places these shots in a longer run: the
+    draws are those that shots shot_offset.. of one run with this seed make."""
     if isinstance(circuit, PauliFrame):  # pftrajectories(state, circuit)
         return pftrajectories_state(circuit, trajectories, seed=seed, masks=masks)
     prog = circuit if isinstance(circuit, Program) else Program(circuit)
@@ -206,7 +271,8 @@ def pftrajectories(circuit, trajectories: int = 5000, threads: bool = True, *,
     if export_masks is not None:
         flags |= 2
         mptr = export_masks
-    call("qcf_pf_sample", prog.handle, f.shots, _SEEDS.next() if seed is None else seed,
+    f.offset = int(shot_offset)
+    call("qcf_pf_sample", prog.handle, f.shots, f.offset, _SEEDS.next() if seed is None else seed,
          _ptr(f.frame) if keep_frame else None, _ptr(f.record), _ptr(mptr), f.ld, flags,
          _stream())
     return f

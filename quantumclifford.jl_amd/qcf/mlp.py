@@ -137,7 +137,7 @@ def train(decoder: MLPDecoder, make_circuit, steps: int, p_range=(5e-4, 5e-3), l
         k = step % per
         if k == 0:
             prog = progs[int(rng.integers(n_rates))]
-            _lib.call("qcf_pf_sample", prog.handle, B * per, int(rng.integers(0, 2**62)), None,
+            _lib.call("qcf_pf_sample", prog.handle, B * per, 0, int(rng.integers(0, 2**62)), None,
                       _ptr(rec), None, ld, 1, st)
         decoder.train_step(rec, ld, lr=lr, shot0=k * B)
         if log_every and (step + 1) % log_every == 0:
@@ -176,7 +176,7 @@ def evaluate(decoder: MLPDecoder, matcher, circuit, shots: int, taus=(0.0,), see
         s = min(chunk, shots - done)
         ld = plane_words(s)
         rec = torch.empty(max(1, prog.nbits) * ld, dtype=torch.int64, device="cuda")
-        _lib.call("qcf_pf_sample", prog.handle, s, seed * 1000003 + k, None, _ptr(rec), None, ld,
+        _lib.call("qcf_pf_sample", prog.handle, s, 0, seed * 1000003 + k, None, _ptr(rec), None, ld,
                   1, _stream())
         torch.cuda.synchronize()
         t = time.perf_counter()

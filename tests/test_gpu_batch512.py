@@ -36,7 +36,7 @@ def test_512_circuit_batch_equals_single_launches(gpu):
     for k, (p, s, sd) in enumerate(zip(progs, shots, seeds)):
         ld = qcf.frames.plane_words(s)
         single = torch.empty(max(1, p.nbits) * ld, dtype=torch.int64, device="cuda")
-        qcf._lib.call("qcf_pf_sample", p.handle, s, sd, None, C.c_void_p(single.data_ptr()),
+        qcf._lib.call("qcf_pf_sample", p.handle, s, 0, sd, None, C.c_void_p(single.data_ptr()),
                       None, ld, 1, st)
         n = p.nbits * ld
         if not torch.equal(single[:n], b.records[k][:n]):

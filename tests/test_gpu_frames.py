@@ -92,11 +92,21 @@ def test_every_gate_from_random_frames(gpu, shots):
     assert np.array_equal(gm, om)
 
 
+def random_pauli_measurement(rng, n, nbits):
+    """PauliMeasurement of a random Pauli on qubits 1..m; the reference uses
+    qubit m+1 as its ancilla (src/pauli_frames.jl:118-134), the frame's hidden
+    qubit when m = n and a circuit qubit otherwise."""
+    m = n if rng.random() < 0.7 else int(rng.integers(1, n + 1))
+    letters = "".join("_XZY"[int(c)] for c in rng.integers(0, 4, m))
+    sign = ("", "-", "i", "-i")[int(rng.integers(4))]
+    return qcf.PauliMeasurement(qcf.P(sign + letters, device="cpu"), int(rng.integers(0, nbits + 1)))
+
+
 def random_circuit(rng, n, nbits, length):
     circ = []
     written = []
     for _ in range(length):
-        k = rng.integers(0, 10)
+        k = rng.integers(0, 11)
         q = int(rng.integers(1, n + 1))
         if k < 3:
             g = GATES_1Q[rng.integers(len(GATES_1Q))]
@@ -118,10 +128,12 @@ def random_circuit(rng, n, nbits, length):
             else:
                 a, b = rng.choice(np.arange(1, n + 1), 2, replace=False)
                 circ.append(qcf.Depolarize2(int(a), int(b), float(rng.uniform(0, 0.3))))
-        else:
+        elif k == 9:
             ar = int(rng.integers(1, 5))
             bits = [int(x) for x in rng.integers(1, nbits + 1, ar)]
             circ.append(qcf.ClassicalXOR(bits, int(rng.integers(1, nbits + 1))))
+        else:
+            circ.append(random_pauli_measurement(rng, n, nbits))
     return circ
 
 
@@ -179,6 +191,43 @@ def test_surface_code_native_export_replay(gpu, d, rounds, shots):
     assert gm[np.array(c.det_bits) - 1].any()
 
 
+def test_noisy_gate_and_noise_op_all_export_replay(gpu):
+    """NoisyGate (the gate, then its noise on the gate's qubits) and NoiseOpAll
+    (the noise on every frame qubit), src/noise.jl:174-199, with every noise
+    kind: native draws exported and replayed through the oracle."""
+    ops = qcf.ops
+    circ = [qcf.sHadamard(1),
+            ops.NoisyGate(qcf.sCNOT(1, 2), ops.UnbiasedUncorrelatedNoise(0.05)),
+            ops.NoisyGate(qcf.sCNOT(2, 3), ops.PauliNoise(0.01, 0.02, 0.03)),
+            ops.NoiseOpAll(ops.UnbiasedUncorrelatedNoise(0.02)),
+            ops.NoisyGate(qcf.sSWAP(1, 3), ops.DepolarizationNoise(0.1)),
+            ops.NoisyGate(qcf.sHadamard(2), ops.DepolarizationNoise(0.08)),
+            qcf.sMZ(1, 1), qcf.sMZ(2, 2), qcf.sMZ(3, 3),
+            ops.NoiseOpAll(ops.PauliNoise(0.1, 0.0, 0.0)),
+            qcf.sMRZ(1, 4), qcf.sMRZ(2, 5), qcf.sMZ(3, 6), qcf.ClassicalXOR([1, 4], 7)]
+    prog, f, gm, om, gf, of, mk = fresh_export_and_replay(circ, 6000, seed=31)
+    assert np.array_equal(gm, om[: prog.nbits])
+    assert np.array_equal(gf, of)
+    assert gm[:6].any() and mk[prog.nq:].any()
+
+
+@pytest.mark.parametrize("shots", [333, 4096 + 9])
+def test_classical_xor_of_many_bits_export_replay(gpu, shots):
+    """ClassicalXOR of 17 bits (the arity the reference's compactification no
+    longer covers, test/test_sumtypecompactification.jl:2), of one bit, and of
+    33 bits with repeats, over random gauge outcomes and noise."""
+    circ = [qcf.sHadamard(q) for q in range(1, 21)]
+    circ += [qcf.PauliError(q, 0.1) for q in range(1, 21)]
+    circ += [qcf.sMZ(q, q) for q in range(1, 21)]
+    circ += [qcf.ClassicalXOR(list(range(1, 18)), 21), qcf.ClassicalXOR([3], 22),
+             qcf.ClassicalXOR(list(range(1, 21)) + list(range(1, 14)), 23),
+             qcf.ClassicalXOR([21, 22, 23], 24)]
+    prog, f, gm, om, gf, of, mk = fresh_export_and_replay(circ, shots, seed=17)
+    assert np.array_equal(gm, om[: prog.nbits])
+    want17 = np.bitwise_xor.reduce(gm[:17], axis=0)
+    assert np.array_equal(gm[20], want17) and 0.3 < gm[20].mean() < 0.7
+
+
 def test_mixed_rates_thinning_export_replay(gpu):
     """Sites of different rates share one event stream thinned per site."""
     circ = []
@@ -195,33 +244,6 @@ def test_mixed_rates_thinning_export_replay(gpu):
     assert np.array_equal(gf, of)
 
 
-def test_native_rates_match_oracle_statistics(gpu):
-    """Native-RNG marginals of every measurement, detector and observable bit of
-    the d=5 x 25-round circuit at p=2e-3 agree with the oracle's per-shot
-    sampling (the reference's algorithm) within 5 sigma (two-sample binomial)."""
-    import os
-    c = circuits.rotated_surface_code(5, 25, 2e-3)
-    prog = qcf.Program(c.ops)
-    n1 = 10**7
-    f = qcf.pftrajectories(prog, n1, seed=2024, keep_frame=False)
-    counts1 = np.zeros(prog.nbits)
-    # popcount per plane on the GPU side via numpy on chunks
-    host = f.record_host()
-    for b in range(prog.nbits):
-        counts1[b] = np.unpackbits(host[b].view(np.uint8)).sum()
-    n2 = 10**6
-    _, om = oracle.pf_run(prog.ops, prog.args, prog.nq, prog.nbits, n2, None, True, seed=7,
-                          nthreads=min(16, os.cpu_count() or 1))
-    counts2 = om.sum(axis=1).astype(float)
-    p1, p2 = counts1 / n1, counts2 / n2
-    pp = (counts1 + counts2) / (n1 + n2)
-    se = np.sqrt(np.maximum(pp * (1 - pp), 1e-12) * (1 / n1 + 1 / n2))
-    z = np.abs(p1 - p2) / se
-    dets = np.array(c.det_bits) - 1
-    assert p1[dets].mean() > 1e-3  # detectors actually fire
-    assert z.max() < 5.0, (int(z.argmax()), float(z.max()), p1[z.argmax()], p2[z.argmax()])
-
-
 @pytest.mark.parametrize("shots", [1, 100, 2047, 2049, 5000])
 def test_padding_inside_stride_is_zero(gpu, shots):
     """A record buffer prefilled with ones: every word past the shots, up to
@@ -233,7 +255,7 @@ def test_padding_inside_stride_is_zero(gpu, shots):
     ld = qcf.frames.plane_words(shots)
     rec = torch.full((prog.nbits * ld,), -1, dtype=torch.int64, device="cuda")
     st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
-    qcf._lib.call("qcf_pf_sample", prog.handle, shots, 77, None, C.c_void_p(rec.data_ptr()),
+    qcf._lib.call("qcf_pf_sample", prog.handle, shots, 0, 77, None, C.c_void_p(rec.data_ptr()),
                   None, ld, 1, st)
     host = rec.cpu().numpy().view(np.uint64).reshape(prog.nbits, ld)
     bits = oracle.unpack_shots(host, ld * 64)
@@ -475,7 +497,7 @@ def test_fresh_launch_defines_every_word(gpu, shots):
     f = qcf.pftrajectories(prog, shots, seed=12)
     zf = torch.zeros(2 * (n + 1) * ld, dtype=torch.int64, device="cuda")
     zr = torch.zeros(nbits * ld, dtype=torch.int64, device="cuda")
-    qcf._lib.call("qcf_pf_sample", prog.handle, shots, 12, qcf.frames._ptr(zf),
+    qcf._lib.call("qcf_pf_sample", prog.handle, shots, 0, 12, qcf.frames._ptr(zf),
                   qcf.frames._ptr(zr), None, ld, 1, qcf.frames._stream())
     torch.cuda.synchronize()
     assert torch.equal(f.record, zr) and torch.equal(f.frame, zf)
@@ -483,3 +505,36 @@ def test_fresh_launch_defines_every_word(gpu, shots):
     assert not rec[8:].any()
     bits = np.unpackbits(rec.view(np.uint8), axis=1, bitorder="little")
     assert not bits[:, shots:].any()
+
+
+def test_global_frames_on_two_streams_equal_sequential_runs(gpu, monkeypatch):
+    """Frames in global memory use a workspace per stream: two replay launches
+    queued on different streams at once give the records of the same launches
+    run one after the other."""
+    monkeypatch.setenv("QCF_FORCE_GLOBAL_FRAMES", "1")
+    rng = np.random.default_rng(21)
+    circ = random_circuit(rng, 6, 8, 150)
+    prog = qcf.Program(circ, nqubits=6, nbits=8)
+    shots = 3 * 2048 + 5
+    runs = []
+    for k in range(2):
+        f = qcf.PauliFrame(shots, 6, 8, init=False)
+        f.frame.random_(-2**62, 2**62)
+        m = torch.randint(-2**62, 2**62, (max(1, prog.nreplay) * f.ld,), dtype=torch.int64,
+                          device="cuda")
+        runs.append((f, m))
+    want = []
+    for f, m in runs:
+        g = f.copy()
+        qcf.pftrajectories_state(g, prog, masks=m)
+        want.append((g.frame_host(), g.record_host()))
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for s in streams:
+        s.wait_stream(torch.cuda.current_stream())
+    for (f, m), s in zip(runs, streams):
+        with torch.cuda.stream(s):
+            qcf.pftrajectories_state(f, prog, masks=m)
+    torch.cuda.synchronize()
+    for (f, _), (wf, wr) in zip(runs, want):
+        assert np.array_equal(f.frame_host(), wf) and np.array_equal(f.record_host(), wr)

@@ -177,3 +177,26 @@ def test_split_frames_in_place_equals_interpreter(gpu, monkeypatch, case):
     (ra, fa), (rb, fb2) = _both(monkeypatch, run)
     assert np.array_equal(ra, rb)
     assert np.array_equal(fa, fb2)
+
+
+@pytest.mark.parametrize("shots", [333, 3 * 2048 + 9])
+def test_many_bit_xor_and_pauli_measurement_specialised_equals_interpreter(gpu, monkeypatch,
+                                                                          shots):
+    """ClassicalXOR of 17 and 33 bits, PauliMeasurement (its ancilla the hidden
+    qubit) and noise after every gate."""
+    circ = [qcf.sHadamard(q) for q in range(1, 21)]
+    circ += [qcf.ops.NoisyGate(qcf.sCNOT(q, q + 1), qcf.ops.UnbiasedUncorrelatedNoise(0.05))
+             for q in range(1, 20)]
+    circ += [qcf.PauliMeasurement(qcf.P("X" * 10 + "Z" * 5 + "Y" * 5, device="cpu"), 25)]
+    circ += [qcf.sMZ(q, q) for q in range(1, 21)]
+    circ += [qcf.ClassicalXOR(list(range(1, 18)), 21),
+             qcf.ClassicalXOR(list(range(1, 21)) + list(range(1, 14)), 22)]
+    prog = qcf.Program(circ)
+
+    def run():
+        f = qcf.pftrajectories(prog, shots, seed=88)
+        return f.record_host(), f.frame_host()
+
+    (ra, fa), (rb, fb) = _both(monkeypatch, run)
+    assert np.array_equal(ra, rb) and np.array_equal(fa, fb)
+    assert oracle.unpack_shots(ra, shots)[20].any()

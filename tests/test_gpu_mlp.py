@@ -56,7 +56,7 @@ def sample_record(circ, shots, seed):
     prog = qcf.Program(circ.ops)
     ld = words(shots)
     rec = torch.empty(prog.nbits * ld, dtype=torch.int64, device="cuda")
-    qcf._lib.call("qcf_pf_sample", prog.handle, shots, seed, None, C.c_void_p(rec.data_ptr()),
+    qcf._lib.call("qcf_pf_sample", prog.handle, shots, 0, seed, None, C.c_void_p(rec.data_ptr()),
                   None, ld, 1, mlp._stream())
     torch.cuda.synchronize()
     return rec, ld

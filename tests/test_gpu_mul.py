@@ -104,8 +104,12 @@ def test_vector_form(gpu):
             assert s == os_ and np.array_equal(ta.cpu().numpy().view(np.uint64), oa)
 
 
-@pytest.mark.parametrize("n", [5, 64, 1000, 1500])
+@pytest.mark.parametrize("n", [5, 64, 1000, 1500, 10000])
 def test_pivot_sweep_and_tableau_pauli(gpu, n):
+    """Pivot sweeps (plain and canonicalize!-style masked), tableau x Pauli and
+    the single-row form on random-Clifford stabilizer tableaux, including the
+    10,000-qubit tableau of the mul_left! benchmark configuration
+    (src/canonicalization.jl:105-109, src/mul_leftright.jl:177-181,195-198)."""
     rng = np.random.default_rng(n)
     ph, xzs = randoms.random_clifford_tableau(rng, n, ngates=3 * n)
     t = qcf.Tableau(ph, n, xzs)
