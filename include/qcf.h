@@ -121,6 +121,22 @@ int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nwords,
                      size_t nrows, size_t nqubits, int phases_on, uint32_t *ranks,
                      void *stream);
 
+/*
+ * qcf_canonicalize without any allocation: no reference counterpart (the
+ * reference's canonicalize! allocates nothing on the device to begin with).
+ * For callers that cannot allocate -- code under hipGraph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Same arguments, results
+ * and errors as qcf_canonicalize; the workspace is a fixed 16 MiB device
+ * array that each GPU holds until the process ends (a tableau of nrows rows
+ * of W words needs about nrows*(16*W + 13) + 1024*W bytes: 6000 x 6000 fits),
+ * and larger tableaux are rejected with QCF_ERR_ARGUMENT.  Every call on one
+ * GPU shares that array, so calls that overlap on one GPU (different
+ * streams, or host threads) race; calls in one stream are ordered.
+ */
+int qcf_canonicalize_noalloc(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nwords,
+                             size_t nrows, size_t nqubits, int phases_on, uint32_t *ranks,
+                             void *stream);
+
 /* ======================================================================= */
 /*  Pauli frames (src/pauli_frames.jl)                                      */
 /* ======================================================================= */

@@ -458,20 +458,19 @@ void launch_apply(uint64_t *xzs, uint8_t *phases, size_t ld, size_t W, uint32_t 
 }  // namespace
 }  // namespace qcf
 
-using namespace qcf;
+// Workspace of qcf_canonicalize_noalloc: 16 MiB of device memory that every
+// GPU the library runs on holds until the process ends.  Host code never
+// passes this name to a kernel (it is a host-side stand-in there); it uses the
+// device address hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocCanonBytes = 16u << 20;
+static __device__ __align__(256) char canon_noalloc_scratch[kNoAllocCanonBytes];
 
-extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nwords,
-                                size_t nrows, size_t nqubits, int phases_on, uint32_t *ranks,
-                                void *stream_) {
-  clear_error();
-  hipStream_t stream = (hipStream_t)stream_;
-  if (nrows == 0) {
-    if (ranks) {
-      hipLaunchKernelGGL(canon_ranks, dim3(1), dim3(64), 0, stream, nullptr, ranks);
-      QCF_HIP_CHECK(hipGetLastError());
-    }
-    return QCF_OK;
-  }
+namespace qcf {
+namespace {
+
+// Checks of both canonicalize! entry points; returns QCF_OK or an error code.
+int canon_check(const uint64_t *xzs, const uint8_t *phases, size_t ld, size_t nwords,
+                size_t nrows, size_t nqubits, int phases_on) {
   if (!xzs) return set_error(QCF_ERR_ARGUMENT, "canonicalize!: null tableau");
   if (phases_on && !phases) return set_error(QCF_ERR_ARGUMENT, "canonicalize!: null phases");
   if (nqubits > 64 * nwords)
@@ -483,44 +482,61 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
   if (nrows > kMaxCanonRows)
     return set_error(QCF_ERR_UNSUPPORTED, "canonicalize!: %zu rows exceed the limit of %u", nrows,
                      kMaxCanonRows);
-  const size_t W = nwords;
-  // workspace: state | inv | mask | piv | phase counters | snapshots | gathered rows | phases
+  return QCF_OK;
+}
+
+// Workspace layout: state | inv | mask | piv | phase counters | snapshots |
+// gathered rows | phases.  Every part is written by the kernels before it is
+// read, so the workspace may hold anything on entry.
+struct CanonLayout {
+  size_t o_st, o_inv, o_mask, o_piv, o_cnt, o_snap, o_tmp, o_tph, bytes;
+};
+CanonLayout canon_layout(size_t nrows, size_t W) {
+  CanonLayout L{};
   size_t off = 0;
-  const size_t o_st = off;
+  L.o_st = off;
   off = align256(off + sizeof(CanonState));
-  const size_t o_inv = off;
+  L.o_inv = off;
   off = align256(off + nrows * sizeof(uint32_t));
-  const size_t o_mask = off;
+  L.o_mask = off;
   off = align256(off + nrows * sizeof(uint64_t));
-  const size_t o_piv = off;
+  L.o_piv = off;
   off = align256(off + 64 * sizeof(uint32_t));
-  const size_t o_cnt = off;
+  L.o_cnt = off;
   off = align256(off + 64 * sizeof(uint32_t));
-  const size_t o_snap = off;
+  L.o_snap = off;
   off = align256(off + 64 * 2 * W * sizeof(uint64_t));
-  const size_t o_tmp = off;
+  L.o_tmp = off;
   off = align256(off + nrows * 2 * W * sizeof(uint64_t));
-  const size_t o_tph = off;
+  L.o_tph = off;
   off = align256(off + nrows);
-  char *base = nullptr;
-  QCF_TRY(canon_scratch(stream, off, &base));
-  CanonState *st = (CanonState *)(base + o_st);
-  uint32_t *inv = (uint32_t *)(base + o_inv);
-  uint64_t *mask = (uint64_t *)(base + o_mask);
-  uint32_t *piv = (uint32_t *)(base + o_piv);
-  uint32_t *cnt = (uint32_t *)(base + o_cnt);
-  uint64_t *snap = (uint64_t *)(base + o_snap);
-  uint64_t *tmp = (uint64_t *)(base + o_tmp);
-  uint8_t *tph = (uint8_t *)(base + o_tph);
+  L.bytes = off;
+  return L;
+}
+
+// The elimination itself, on a checked tableau with a workspace at `base`.
+int canon_run(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nwords, size_t nrows,
+              size_t nqubits, int phases_on, uint32_t *ranks, char *base, const CanonLayout &Lo,
+              hipStream_t stream) {
+  const size_t W = nwords;
+  CanonState *st = (CanonState *)(base + Lo.o_st);
+  uint32_t *inv = (uint32_t *)(base + Lo.o_inv);
+  uint64_t *mask = (uint64_t *)(base + Lo.o_mask);
+  uint32_t *piv = (uint32_t *)(base + Lo.o_piv);
+  uint32_t *cnt = (uint32_t *)(base + Lo.o_cnt);
+  uint64_t *snap = (uint64_t *)(base + Lo.o_snap);
+  uint64_t *tmp = (uint64_t *)(base + Lo.o_tmp);
+  uint8_t *tph = (uint8_t *)(base + Lo.o_tph);
   const uint32_t n = (uint32_t)nrows;
   hipLaunchKernelGGL(canon_init, dim3((n + 255) / 256), dim3(256), 0, stream, inv, n, st);
-  DeviceState *ds;
-  QCF_TRY(device_state(&ds));
+  int dev = 0, cus = 0;
+  QCF_HIP_CHECK(hipGetDevice(&dev));
+  QCF_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const uint32_t G = W > 32 ? 64 : W > 16 ? 32 : W > 8 ? 16 : 8;
   const uint32_t rows_per_block = 256 / G;
   const uint32_t apply_grid =
       std::max<uint32_t>(1, std::min<uint32_t>((n + rows_per_block - 1) / rows_per_block,
-                                               (uint32_t)ds->cu_count * 8));
+                                               (uint32_t)cus * 8));
   const uint32_t NT = factor_threads(n);
   const uint32_t J = (n + NT - 1) / NT;
   const uint32_t snap_grid = (uint32_t)((W + 63) / 64);
@@ -555,7 +571,7 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
     }
   }
   const size_t total = nrows * 2 * W;
-  const uint32_t pgrid = (uint32_t)std::min<size_t>((total + 255) / 256, (size_t)ds->cu_count * 16);
+  const uint32_t pgrid = (uint32_t)std::min<size_t>((total + 255) / 256, (size_t)cus * 16);
   hipLaunchKernelGGL(canon_gather, dim3(pgrid), dim3(256), 0, stream, xzs, phases, ld, W, n, inv,
                      tmp, phases_on ? tph : nullptr);
   hipLaunchKernelGGL(canon_scatter_back, dim3(pgrid), dim3(256), 0, stream, xzs, phases, ld, W, n,
@@ -563,4 +579,77 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
   if (ranks) hipLaunchKernelGGL(canon_ranks, dim3(1), dim3(64), 0, stream, st, ranks);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
+}
+
+// The no-allocation entry point's workspace: a device array of fixed size,
+// one copy per GPU, whose address is looked up once per device.
+constexpr int kMaxNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxNoAllocDevices];
+
+int noalloc_scratch(char **out) {
+  int dev = 0;
+  QCF_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxNoAllocDevices)
+    return set_error(QCF_ERR_UNSUPPORTED, "canonicalize!: device %d beyond %d", dev,
+                     kMaxNoAllocDevices);
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(canon_noalloc_scratch));
+    if (e != hipSuccess || !p)
+      return set_error(QCF_ERR_HIP, "canonicalize!: workspace symbol address: %s",
+                       hipGetErrorString(e));
+    g_noalloc_ptr[dev] = p;
+  }
+  *out = (char *)g_noalloc_ptr[dev];
+  return QCF_OK;
+}
+
+}  // namespace
+}  // namespace qcf
+
+using namespace qcf;
+
+extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nwords,
+                                size_t nrows, size_t nqubits, int phases_on, uint32_t *ranks,
+                                void *stream_) {
+  clear_error();
+  hipStream_t stream = (hipStream_t)stream_;
+  if (nrows == 0) {
+    if (ranks) {
+      hipLaunchKernelGGL(canon_ranks, dim3(1), dim3(64), 0, stream, nullptr, ranks);
+      QCF_HIP_CHECK(hipGetLastError());
+    }
+    return QCF_OK;
+  }
+  QCF_TRY(canon_check(xzs, phases, ld, nwords, nrows, nqubits, phases_on));
+  const CanonLayout Lo = canon_layout(nrows, nwords);
+  char *base = nullptr;
+  QCF_TRY(canon_scratch(stream, Lo.bytes, &base));
+  return canon_run(xzs, phases, ld, nwords, nrows, nqubits, phases_on, ranks, base, Lo, stream);
+}
+
+extern "C" int qcf_canonicalize_noalloc(uint64_t *xzs, uint8_t *phases, size_t ld,
+                                        size_t nwords, size_t nrows, size_t nqubits,
+                                        int phases_on, uint32_t *ranks, void *stream_) {
+  clear_error();
+  hipStream_t stream = (hipStream_t)stream_;
+  if (nrows == 0) {
+    if (ranks) {
+      hipLaunchKernelGGL(canon_ranks, dim3(1), dim3(64), 0, stream, nullptr, ranks);
+      QCF_HIP_CHECK(hipGetLastError());
+    }
+    return QCF_OK;
+  }
+  QCF_TRY(canon_check(xzs, phases, ld, nwords, nrows, nqubits, phases_on));
+  const CanonLayout Lo = canon_layout(nrows, nwords);
+  if (Lo.bytes > kNoAllocCanonBytes)
+    return set_error(QCF_ERR_ARGUMENT,
+                     "canonicalize! without allocation: %zu rows of %zu words need %zu bytes of "
+                     "workspace, more than the %zu it holds",
+                     nrows, nwords, Lo.bytes, kNoAllocCanonBytes);
+  char *base = nullptr;
+  QCF_TRY(noalloc_scratch(&base));
+  return canon_run(xzs, phases, ld, nwords, nrows, nqubits, phases_on, ranks, base, Lo, stream);
 }

@@ -75,6 +75,7 @@ _SIGNATURES = {
     "qcf_mul_vec": (_I, [_P, _P, _SZ, _I, _I, _P, _P]),
     "qcf_mul_rows": (_I, [_P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ, _P, _I, _I, _P]),
     "qcf_canonicalize": (_I, [_P, _P, _SZ, _SZ, _SZ, _SZ, _I, _P, _P]),
+    "qcf_canonicalize_noalloc": (_I, [_P, _P, _SZ, _SZ, _SZ, _SZ, _I, _P, _P]),
     "qcf_program_build": (_I, [_P, _SZ, _P, _SZ, _U32, _U32, C.POINTER(_P)]),
     "qcf_program_free": (_I, [_P]),
     "qcf_program_get_info": (_I, [_P, C.POINTER(ProgramInfo)]),
