@@ -97,6 +97,35 @@ def bench_mul(steps, warmup, n=10**9):
     return dict(n=n, ms=ms, gbs=nbytes / (ms * 1e-3) / 1e9, bytes=nbytes)
 
 
+def time_graph(fn, per_graph, replays, warmup=1):
+    """Device time of one fn() call: per_graph calls captured in one hipGraph
+    (torch.cuda.CUDAGraph on ROCm), the graph replayed `replays` times between
+    two events on the replay stream; no Python or launch API in the timed
+    region."""
+    import torch
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        for _ in range(warmup):
+            fn()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(per_graph):
+            fn()
+    g.replay()
+    torch.cuda.synchronize()
+    cs = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(cs)
+    for _ in range(replays):
+        g.replay()
+    e1.record(cs)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / (replays * per_graph)
+
+
 def bench_mul_config(steps, warmup):
     """The rest of the mul_left! configuration: two random Paulis of 10^6
     qubits (launch-bound), and the pivot sweep (every row m != pivot:
@@ -114,18 +143,37 @@ def bench_mul_config(steps, warmup):
     p1, x1 = qcf.randoms.random_pauli_words(rng, n)
     p2, x2 = qcf.randoms.random_pauli_words(rng, n)
     l, r = qcf.PauliOperator(p1, n, x1), qcf.PauliOperator(p2, n, x2)
-    ms = time_launches(lambda: qcf.mul_left_(r, l), steps * 10, warmup)
-    out["mul_left_1mq"] = dict(n=n, ms=ms, gbs=6 * l.nwords * 8 / (ms * 1e-3) / 1e9)
+    ms_call = time_launches(lambda: qcf.mul_left_(r, l), steps * 10, warmup)
+    ms = time_graph(lambda: qcf.mul_left_(r, l), 100, max(2, steps // 2))
+    out["mul_left_1mq"] = dict(n=n, ms=ms, gbs=6 * l.nwords * 8 / (ms * 1e-3) / 1e9,
+                               ms_per_python_call=ms_call,
+                               timing="device time per product inside a captured graph of 100")
     for n in (1000, 10000):
         ph, xzs = qcf.randoms.random_clifford_tableau(np.random.default_rng(0), n)
         t = qcf.Tableau(ph, n, xzs)
         W = t.nwords
         nbytes = (n - 1) * 2 * (2 * W * 8) + 2 * W * 8 + 2 * n
-        ms = time_launches(lambda: qcf.mul_left_sweep_(t, 1), steps * 5, warmup)
         mask = torch.from_numpy((np.random.default_rng(1).integers(0, 2, n)).astype(np.uint8)).cuda()
-        ms_m = time_launches(lambda: qcf.mul_left_sweep_(t, 1, row_mask=mask), steps * 5, warmup)
+        # device time per sweep: 100 sweeps (pivots 1..100 in turn, as
+        # canonicalize! runs them) per captured graph
+        piv = [0]
+
+        def sweep():
+            piv[0] = piv[0] % 100 + 1
+            qcf.mul_left_sweep_(t, piv[0])
+
+        def sweep_masked():
+            piv[0] = piv[0] % 100 + 1
+            qcf.mul_left_sweep_(t, piv[0], row_mask=mask)
+
+        ms = time_graph(sweep, 100, max(2, steps // 2))
+        ms_m = time_graph(sweep_masked, 100, max(2, steps // 2))
+        ms_call = time_launches(lambda: qcf.mul_left_sweep_(t, 1), steps * 5, warmup)
         out[f"sweep_{n}"] = dict(n=n, rows=n, ms=ms, gbs=nbytes / (ms * 1e-3) / 1e9,
-                                 bytes=nbytes, ms_masked=ms_m)
+                                 frac_hbm=nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                 bytes=nbytes, ms_masked=ms_m,
+                                 ms_per_python_call=ms_call,
+                                 timing="device time per sweep inside a captured graph of 100")
     return out
 
 
@@ -231,6 +279,26 @@ def bench_canonicalize(steps, warmup):
     return out
 
 
+def host_cpu():
+    """CPU model, physical cores and hardware threads of the host (lscpu)."""
+    import subprocess
+    info = {"model": None, "physical_cores": None, "threads": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name")
+        cps, sockets = kv.get("Core(s) per socket"), kv.get("Socket(s)")
+        if cps and sockets:
+            info["physical_cores"] = int(cps) * int(sockets)
+    except Exception as e:  # reported, not hidden
+        info["error"] = repr(e)
+    return info
+
+
 def cpu_baseline(target_s=12.0):
     """The oracle (C restatement, OpenMP over shots like the reference's
     Threads.@threads split) on the d=5 x 25 circuit with per-shot RNG draws."""
@@ -241,7 +309,9 @@ def cpu_baseline(target_s=12.0):
     c = circuits.rotated_surface_code(5, 25, 1e-3)
     ops, args = qcf.lower_circuit(c.ops, c.nqubits)
     nq, nbits = c.nqubits + 1, c.nbits
-    cores = min(16, os.cpu_count() or 1)
+    host = host_cpu()
+    # the GPU box grants one GPU's share of its host: OMP_NUM_THREADS (16)
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     oracle.pf_run(ops, args, nq, nbits, 4000, None, True, seed=1, nthreads=cores)
     # chunks of 10^6 shots (the oracle keeps one byte per record bit) until
     # about target_s seconds of CPU work
@@ -277,7 +347,11 @@ def cpu_baseline(target_s=12.0):
     return dict(value=shots / dt, unit="shots/s", cores=cores, kind="port",
                 sample=f"{shots} shots of the d=5 x 25-round circuit (p=1e-3) in chunks of "
                        f"{chunk}, oracle restatement with OpenMP over {cores} threads, "
-                       f"{dt:.1f} s", mul_left=mul, canonicalize=canon)
+                       f"{dt:.1f} s", host_cpu=host,
+                threads_note="OpenMP threads = OMP_NUM_THREADS, the host share one GPU is "
+                             "granted on the GPU box (Julia is not installed there, so the "
+                             "reference itself is not measured)",
+                mul_left=mul, canonicalize=canon)
 
 
 def bench_mlp(steps: int, eval_shots: int):
@@ -332,6 +406,30 @@ def bench_mlp(steps: int, eval_shots: int):
                 ms_per_step=1e3 * tr["seconds"] / steps,
                 train_tflops=flops_step * steps / tr["seconds"] / 1e12,
                 evals=evals)
+
+
+def headline_ratios(extras):
+    """The two comparisons the metric names: batched over per-circuit shots/s
+    (512 circuits), and the perceptron decoder's logical error rate over
+    matching's at p = 1e-3 (with the |logit| fallback closest to 1.05x at or
+    above 10x matching's shots/s, and alone)."""
+    out = {"batched_vs_per_circuit_shots_s": None, "nn_vs_mwpm_ler_p1e-3": None,
+           "nn_alone_vs_mwpm_ler_p1e-3": None}
+    b = extras.get("batched") or {}
+    if "ratio" in b:
+        out["batched_vs_per_circuit_shots_s"] = b["ratio"]
+    m = (extras.get("mlp_decoder") or {}).get("evals", {}).get("0.001")
+    if m:
+        alone = [r for r in m if r["tau"] == 0.0]
+        if alone:
+            out["nn_alone_vs_mwpm_ler_p1e-3"] = alone[0]["ler_nn"] / alone[0]["ler_matching"]
+        ok = [r for r in m if r["combined_shots_per_s"] >= 10 * r["matching_shots_per_s"]]
+        if ok:
+            best = min(ok, key=lambda r: r["ler_combined"] / r["ler_matching"])
+            out["nn_vs_mwpm_ler_p1e-3"] = best["ler_combined"] / best["ler_matching"]
+            out["nn_fallback_tau"] = best["tau"]
+            out["nn_shots_s_vs_mwpm"] = best["combined_shots_per_s"] / best["matching_shots_per_s"]
+    return out
 
 
 def main():
@@ -433,6 +531,7 @@ def main():
         "int_valu": {"bitops_per_shot": fr["bitops_per_shot"], "bitops_s": fr["bitops_s"],
                      "lane_ops_s": fr["lane_ops_s"], "peak_lane_ops_s": INT_VALU_PEAK,
                      "frac": fr["lane_ops_s"] / INT_VALU_PEAK},
+        "ratios": headline_ratios(extras),
         "extras": extras,
         "cpu_baseline": cpu,
     }
