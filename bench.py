@@ -195,9 +195,24 @@ def bench_batched(ncirc=512, seed=0, dmax=15):
         progs.append(qcf.Program(circuits.rotated_surface_code(d, d, p).ops))
         shots.append(int(round(10 ** rng.uniform(4, 6))))
         seeds.append(int(rng.integers(0, 2**62)))
+    t0 = time.perf_counter()
     tune = batch.autotune(progs, shots)
-    b = batch.Batch(progs, shots, seeds, tune)
+    tune_s = time.perf_counter() - t0
     total = sum(shots)
+    # the library defaults (every circuit with a specialised kernel runs it on
+    # two helper streams; the persistent kernel shares the tiles of those whose
+    # frames exceed LDS), then the tune
+    b0 = batch.Batch(progs, shots, seeds, qcf._lib.Tune())
+    ms_default = time_launches(b0.launch_graph, 3, 1)
+    del b0
+    # the same without the persistent kernel sharing the tiles of the
+    # launches whose frames exceed LDS
+    t1 = qcf._lib.Tune()
+    t1.steal_planes = 0xFFFFFFFF
+    b0 = batch.Batch(progs, shots, seeds, t1)
+    ms_unshared = time_launches(b0.launch_graph, 3, 1)
+    del b0
+    b = batch.Batch(progs, shots, seeds, tune)
     ms_b = time_launches(b.launch_graph, 3, 1)
     # per-circuit launches into the same records
     import ctypes as C
@@ -212,7 +227,10 @@ def bench_batched(ncirc=512, seed=0, dmax=15):
     ms_s = time_launches(seq, 2, 1)
     return dict(ncirc=ncirc, total_shots=total, ms_batched=ms_b, ms_sequential=ms_s,
                 shots_s_batched=total / (ms_b * 1e-3), shots_s_sequential=total / (ms_s * 1e-3),
-                ratio=ms_s / ms_b, tune=[tune.waves_per_cu, tune.tile_blocks, tune.lds_cap], dmax=dmax)
+                ratio=ms_s / ms_b, ms_batched_default_tune=ms_default,
+                ms_batched_unshared=ms_unshared,
+                tune=dict(zip([f for f, _ in qcf._lib.Tune._fields_], tune.as_list())),
+                autotune_s=tune_s, dmax=dmax)
 
 
 def bench_reference_circuits(steps, warmup):

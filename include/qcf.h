@@ -287,13 +287,28 @@ int qcf_pf_sample(const qcf_program *prog, uint64_t shots, uint64_t shot_offset,
  */
 size_t qcf_plane_words(uint64_t shots);
 
-/* Launch-shape knobs chosen by the autotuner (0 = library default). */
+/* Launch knobs chosen by the autotuner (0 = library default). */
 typedef struct qcf_tune {
-  uint32_t waves_per_cu; /* resident single-wave workgroups per CU (grid size) */
-  uint32_t tile_blocks;  /* 2048-shot tiles per work item of the device queue  */
-  uint32_t lds_cap;      /* bytes of LDS a circuit's frame may take (0 = all the
-                            device allows); larger frames live in global memory,
-                            which keeps more waves resident                     */
+  uint32_t waves_per_cu;   /* persistent kernel: resident single-wave workgroups
+                              per CU (grid size)                                */
+  uint32_t tile_blocks;    /* persistent kernel: 2048-shot tiles per work item    */
+  uint32_t lds_cap;        /* persistent kernel: bytes of LDS a circuit's frame may
+                              take (0 = all the device allows); larger frames live
+                              in global memory, which keeps more waves resident  */
+  uint32_t jit_max_planes; /* circuits whose frames have more planes than this
+                              run on the persistent kernel instead of their
+                              specialised kernel (0 = every circuit that has one
+                              runs it)                                           */
+  uint32_t streams;        /* helper streams the specialised launches are spread
+                              over, 1..4 (0 = QCF_BATCH_STREAMS, default 2)      */
+  uint32_t concurrent;     /* 1: the persistent kernel starts beside the
+                              specialised launches instead of after them        */
+  uint32_t steal_planes;   /* specialised launches whose circuits' frames do not
+                              fit in LDS and have more planes than this share
+                              their tiles with the persistent kernel, which then
+                              runs beside them: both take tiles from one counter
+                              per launch (0 = every such launch, 0xFFFFFFFF =
+                              none); the records do not depend on the split     */
 } qcf_tune;
 
 /*
@@ -307,7 +322,12 @@ typedef struct qcf_tune {
  * into it (two by default; QCF_BATCH_STREAMS, 1..4).  It then enqueues
  * ONE persistent kernel for the circuits without one (frames beyond what
  * registers plus LDS hold); its workgroups pull (circuit, tile range) items
- * from a device queue.  The launch allocates and compiles nothing, so it
+ * from a device queue.  By default the specialised launches whose frames
+ * exceed LDS (they run one wave per CU) share their tiles with it: the
+ * persistent kernel then starts beside them and, once its own items are
+ * done, takes tiles from the same per-launch counters they take them from
+ * (qcf_tune.steal_planes).  The last workgroup to leave resets the
+ * counters.  The launch allocates and compiles nothing, so it
  * can be captured in a hipGraph; qcf_batch_launch_graph captures it once and
  * replays the graph.  QCF_BATCH_JIT=0 sends every circuit to the persistent
  * kernel.  Results are bit-identical to n qcf_pf_sample(..., QCF_PF_FRESH)
@@ -328,9 +348,14 @@ int qcf_pf_sample_batched(const qcf_program *const *progs, size_t n,
                           uint64_t *const *meas, const size_t *ld,
                           const qcf_tune *tune, void *stream);
 
-/* Pick qcf_tune for a set of circuits by timing candidate launch shapes on
- * the current device.  Writes only to scratch copies of the records; the
- * choice never changes results (the RNG is keyed on absolute shots). */
+/* Pick qcf_tune for a set of circuits by timing batched launches of them on
+ * the current device (on scratch records; the choice never changes results,
+ * the RNG being keyed on absolute shots).  Candidates: which frame sizes keep
+ * their specialised kernel (all of them, or all but the largest plane count,
+ * whose circuits then run on the persistent kernel), 1 or 2 helper streams,
+ * sharing the tiles of the launches whose frames exceed LDS or not, and --
+ * when the persistent kernel has work -- starting it beside the specialised
+ * launches or after them, at the default or the largest waves per CU. */
 int qcf_autotune(const qcf_program *const *progs, size_t n,
                  const uint64_t *shots, qcf_tune *out, void *stream);
 

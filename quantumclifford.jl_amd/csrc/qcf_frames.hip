@@ -297,40 +297,63 @@ __global__ __launch_bounds__(64) void pf_kernel(KParams P) {
 }
 
 // Persistent batched kernel: workgroups pull (circuit, tile range) items from
-// a device queue until it is drained.  The last workgroup to leave resets the
-// queue counters, so consecutive launches (and graph replays) need no clearing.
+// a device queue until it is drained, then take single tiles of the shared
+// specialised launches (those running beside it with a tile counter of their
+// own), largest group first.  Every workgroup of this kernel and of the
+// sharing launches adds one to queue[1] when it leaves; the last one resets
+// the counters, so consecutive launches (and graph replays) need no clearing.
 struct WorkItem {
   uint32_t circuit, blk0, nblk, pad;
 };
 
-__global__ __launch_bounds__(64) void pf_batch_kernel(const KParams *params,
-                                                      const WorkItem *items,
-                                                      uint32_t nitems,
-                                                      uint32_t *queue) {
+struct BatchArgs {
+  const KParams *params;
+  const WorkItem *items;     // own items
+  uint32_t nitems;
+  uint32_t nshared;          // shared groups
+  uint32_t *queue;           // [0] next own item, [1] workgroups done, [2 + g] next tile of group g
+  const WorkItem *shared;    // tiles of every shared group, in the group's tile order
+  const uint32_t *shared_first;  // group g owns shared[shared_first[g] .. shared_first[g + 1])
+  uint32_t total;            // workgroups that count into queue[1]
+  uint32_t nclear;           // counters the last one resets
+};
+
+__device__ __forceinline__ void batch_items(const KParams *params, const WorkItem &w,
+                                            uint32_t *lds, uint32_t lane) {
+  const KParams P = params[w.circuit];
+  if (P.scratch) {
+    Planes<kPlAtomic> F{P.scratch + (uint64_t)blockIdx.x * P.sstride};
+    for (uint32_t b = 0; b < w.nblk; ++b) run_tile<kNative, kPlAtomic>(P, w.blk0 + b, F, lane);
+  } else {
+    Planes<kPlLds> F{lds};
+    for (uint32_t b = 0; b < w.nblk; ++b) run_tile<kNative, kPlLds>(P, w.blk0 + b, F, lane);
+  }
+}
+
+__global__ __launch_bounds__(64) void pf_batch_kernel(BatchArgs A) {
   extern __shared__ uint32_t lds[];
   const uint32_t lane = threadIdx.x;
   while (true) {
     uint32_t it = 0;
-    if (lane == 0) it = atomicAdd(&queue[0], 1u);
+    if (lane == 0) it = atomicAdd(&A.queue[0], 1u);
     it = readlane(it, 0);
-    if (it >= nitems) break;
-    const WorkItem w = items[it];
-    const KParams P = params[w.circuit];
-    if (P.scratch) {
-      Planes<kPlAtomic> F{P.scratch + (uint64_t)blockIdx.x * P.sstride};
-      for (uint32_t b = 0; b < w.nblk; ++b) run_tile<kNative, kPlAtomic>(P, w.blk0 + b, F, lane);
-    } else {
-      Planes<kPlLds> F{lds};
-      for (uint32_t b = 0; b < w.nblk; ++b) run_tile<kNative, kPlLds>(P, w.blk0 + b, F, lane);
+    if (it >= A.nitems) break;
+    batch_items(A.params, A.items[it], lds, lane);
+  }
+  for (uint32_t g = 0; g < A.nshared; ++g) {
+    const uint32_t f0 = A.shared_first[g], n = A.shared_first[g + 1] - f0;
+    while (true) {
+      uint32_t t = 0;
+      if (lane == 0) t = atomicAdd(&A.queue[2 + g], 1u);
+      t = readlane(t, 0);
+      if (t >= n) break;
+      batch_items(A.params, A.shared[f0 + t], lds, lane);
     }
   }
   if (lane == 0) {
     __threadfence();
-    const uint32_t done = atomicAdd(&queue[1], 1u);
-    if (done == gridDim.x - 1) {
-      atomicExch(&queue[0], 0u);
-      atomicExch(&queue[1], 0u);
-    }
+    if (atomicAdd(&A.queue[1], 1u) == A.total - 1u)
+      for (uint32_t i = 0; i < A.nclear; ++i) atomicExch(&A.queue[i], 0u);
   }
 }
 
@@ -781,7 +804,17 @@ struct qcf_batch {
   std::vector<qcf::KParams> params;
   std::vector<uint8_t> on_jit;          // circuit k runs its specialised kernel
   std::vector<qcf::JitPlan> jit;        // one launch per kernel, costliest first
+  std::vector<double> jit_work;         // estimated cost of each launch
   std::vector<int> jit_stream;          // helper stream of each launch
+  std::vector<uint32_t> jit_planes;     // frame planes of each launch's circuits
+  std::vector<uint8_t> jit_stealable;   // the persistent kernel can run its tiles
+  std::vector<std::vector<qcf::WorkItem>> jit_tiles;  // each launch's tiles, in its order
+  std::vector<int> jit_share;           // shared group of each launch, or -1
+  uint32_t nshare = 0;                  // launches sharing tiles with the persistent kernel
+  qcf::WorkItem *d_shared = nullptr;    // their tiles, group after group
+  uint32_t *d_shared_first = nullptr;   // group g: d_shared[first[g] .. first[g + 1])
+  int nstreams = 2;                     // helper streams in use
+  bool concurrent = false;              // persistent kernel beside the specialised ones
   void *d_jit = nullptr;                // per-circuit parameters and tile offsets
   std::vector<qcf::WorkItem> items;
   qcf::KParams *d_params = nullptr;
@@ -805,6 +838,8 @@ struct qcf_batch {
     if (d_jit) (void)hipFree(d_jit);
     if (d_params) (void)hipFree(d_params);
     if (d_items) (void)hipFree(d_items);
+    if (d_shared) (void)hipFree(d_shared);
+    if (d_shared_first) (void)hipFree(d_shared_first);
     if (d_queue) (void)hipFree(d_queue);
     if (d_frames) (void)hipFree(d_frames);
   }
@@ -868,9 +903,49 @@ int upload_items(qcf_batch *B) {
   return QCF_OK;
 }
 
+// Specialised launches whose circuits the persistent kernel can run (frames
+// in its global slots) and whose frames have more than `planes` planes share
+// their tiles with it (0xFFFFFFFF: none): both kernels take tile indices
+// from one counter per launch, so the faster one takes more.  The tiles are
+// the same whichever kernel runs them (every draw is keyed on the tile), so
+// the records do not depend on the split.
+int share_tiles(qcf_batch *B, uint32_t planes) {
+  B->jit_share.assign(B->jit.size(), -1);
+  std::vector<WorkItem> shared;
+  std::vector<uint32_t> first{0u};
+  uint32_t g = 0;
+  for (size_t i = 0; i < B->jit.size(); ++i) {
+    if (!B->jit_stealable[i] || planes == 0xFFFFFFFFu || B->jit_planes[i] <= planes) continue;
+    B->jit_share[i] = (int)g++;
+    shared.insert(shared.end(), B->jit_tiles[i].begin(), B->jit_tiles[i].end());
+    first.push_back((uint32_t)shared.size());
+  }
+  B->nshare = g;
+  if (B->d_shared) QCF_HIP_CHECK(hipFree(B->d_shared));
+  if (B->d_shared_first) QCF_HIP_CHECK(hipFree(B->d_shared_first));
+  B->d_shared = nullptr;
+  B->d_shared_first = nullptr;
+  if (!g) return QCF_OK;
+  QCF_HIP_CHECK(hipMalloc(&B->d_shared, shared.size() * sizeof(WorkItem)));
+  QCF_HIP_CHECK(hipMalloc(&B->d_shared_first, first.size() * sizeof(uint32_t)));
+  QCF_HIP_CHECK(hipMemcpy(B->d_shared, shared.data(), shared.size() * sizeof(WorkItem),
+                          hipMemcpyHostToDevice));
+  QCF_HIP_CHECK(hipMemcpy(B->d_shared_first, first.data(), first.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice));
+  return QCF_OK;
+}
+
+void assign_streams(qcf_batch *B);
+int batch_streams();
+bool batch_concurrent();
+
 int set_tune(qcf_batch *B, const qcf_tune *tune) {
   DeviceState *st;
   QCF_TRY(device_state(&st));
+  B->nstreams = tune && tune->streams ? std::max(1, std::min(kBatchStreams, (int)tune->streams))
+                                      : batch_streams();
+  B->concurrent = tune && tune->concurrent ? true : batch_concurrent();
+  assign_streams(B);
   uint32_t wpc = tune ? tune->waves_per_cu : 0;
   uint32_t tb = tune ? tune->tile_blocks : 0;
   const uint32_t lds_cap = std::max<uint32_t>(1, (160u * 1024u) / std::max(1u, B->lds_bytes));
@@ -879,8 +954,11 @@ int set_tune(qcf_batch *B, const qcf_tune *tune) {
   if (tb == 0) tb = 1;
   make_items(B, tb);
   QCF_TRY(upload_items(B));
-  B->grid = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>((uint64_t)st->cu_count * wpc, B->items.size()));
+  QCF_TRY(share_tiles(B, tune ? tune->steal_planes : 0u));
+  uint64_t work = B->items.size();
+  for (size_t i = 0; i < B->jit.size(); ++i)
+    if (B->jit_share[i] >= 0) work += B->jit_tiles[i].size();
+  B->grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)st->cu_count * wpc, work));
   if (B->exec) {
     (void)hipGraphExecDestroy(B->exec);
     B->exec = nullptr;
@@ -923,6 +1001,19 @@ bool batch_concurrent() {
 // costliest first, each to the helper stream with the least work so far.
 // No allocation and no compilation happen at launch time (both are done
 // here), so the launch sequence can be captured into a graph.
+// Costliest launch first, each to the helper stream with the least work so far.
+void assign_streams(qcf_batch *B) {
+  B->jit_stream.assign(B->jit.size(), 0);
+  double load[kBatchStreams] = {};
+  for (size_t i = 0; i < B->jit.size(); ++i) {
+    int h = 0;
+    for (int j = 1; j < B->nstreams; ++j)
+      if (load[j] < load[h]) h = j;
+    load[h] += B->jit_work[i];
+    B->jit_stream[i] = h;
+  }
+}
+
 int plan_jit(qcf_batch *B) {
   std::vector<JitPlan> plans(B->jit);  // indexed by circuit
   B->jit.clear();
@@ -951,7 +1042,14 @@ int plan_jit(qcf_batch *B) {
   char *dbase = (char *)B->d_jit;
   DeviceState *st;
   QCF_TRY(device_state(&st));
-  std::vector<std::pair<double, JitPlan>> launches;
+  struct Launch {
+    double work;
+    JitPlan plan;
+    uint32_t planes;
+    bool stealable;
+    std::vector<WorkItem> tiles;
+  };
+  std::vector<Launch> launches;
   size_t pi = 0, fi = 0;
   for (const auto &m : members) {
     const size_t p0 = pi, f0 = fi;
@@ -980,21 +1078,27 @@ int plan_jit(qcf_batch *B) {
       gp.jp.xcds = nx;
     }
     gp.grid = (uint32_t)std::max<uint64_t>(1, g1);
-    launches.push_back({work / gp.grid, gp});
+    Launch L{work / gp.grid, gp, B->params[m[0]].nplanes, true, {}};
+    for (size_t k : m) {
+      L.stealable = L.stealable && B->params[k].scratch != nullptr;
+      for (uint32_t b = 0; b < B->params[k].nblocks; ++b)
+        L.tiles.push_back(WorkItem{(uint32_t)k, b, 1u, 0u});
+    }
+    if (!L.stealable) L.tiles.clear();
+    launches.push_back(std::move(L));
   }
   QCF_HIP_CHECK(hipMemcpy(B->d_jit, host.data(), host.size(), hipMemcpyHostToDevice));
   std::stable_sort(launches.begin(), launches.end(),
-                   [](const auto &a, const auto &b) { return a.first > b.first; });
-  double load[kBatchStreams] = {};
-  const int nstreams = batch_streams();
-  for (const auto &l : launches) {
-    int h = 0;
-    for (int j = 1; j < nstreams; ++j)
-      if (load[j] < load[h]) h = j;
-    load[h] += l.first;
-    B->jit.push_back(l.second);
-    B->jit_stream.push_back(h);
+                   [](const auto &a, const auto &b) { return a.work > b.work; });
+  for (auto &l : launches) {
+    B->jit.push_back(l.plan);
+    B->jit_work.push_back(l.work);
+    B->jit_planes.push_back(l.planes);
+    B->jit_stealable.push_back(l.stealable ? 1 : 0);
+    B->jit_tiles.push_back(std::move(l.tiles));
   }
+  B->jit_share.assign(B->jit.size(), -1);
+  assign_streams(B);
   for (int h = 0; h < kBatchStreams; ++h) {
     QCF_HIP_CHECK(hipStreamCreateWithFlags(&B->helpers[h], hipStreamNonBlocking));
     QCF_HIP_CHECK(hipEventCreateWithFlags(&B->join[h], hipEventDisableTiming));
@@ -1004,12 +1108,18 @@ int plan_jit(qcf_batch *B) {
 }
 
 int batch_launch(qcf_batch *B, hipStream_t s) {
-  const bool conc = batch_concurrent();
+  // shared tiles need the persistent kernel running beside the launches
+  const bool conc = B->concurrent || B->nshare;
+  uint32_t total = B->grid;  // workgroups that count into the done counter
+  for (size_t i = 0; i < B->jit.size(); ++i)
+    if (B->jit_share[i] >= 0) total += B->jit[i].grid;
   auto persistent = [&]() -> int {
-    if (B->items.empty()) return QCF_OK;
+    if (B->items.empty() && !B->nshare) return QCF_OK;
     QCF_TRY(allow_lds((const void *)pf_batch_kernel, B->lds_bytes));
-    hipLaunchKernelGGL(pf_batch_kernel, dim3(B->grid), dim3(64), B->lds_bytes, s, B->d_params,
-                       B->d_items, (uint32_t)B->items.size(), B->d_queue);
+    const BatchArgs A{B->d_params, B->d_items,        (uint32_t)B->items.size(), B->nshare,
+                      B->d_queue,  B->d_shared,       B->d_shared_first,         total,
+                      2u + B->nshare};
+    hipLaunchKernelGGL(pf_batch_kernel, dim3(B->grid), dim3(64), B->lds_bytes, s, A);
     QCF_HIP_CHECK(hipGetLastError());
     return QCF_OK;
   };
@@ -1018,7 +1128,14 @@ int batch_launch(qcf_batch *B, hipStream_t s) {
   for (int h = 0; h < kBatchStreams; ++h) QCF_HIP_CHECK(hipStreamWaitEvent(B->helpers[h], B->fork, 0));
   for (size_t i = 0; i < B->jit.size(); ++i) {
     const JitPlan &pl = B->jit[i];
-    QCF_TRY(jit_launch(pl.function, pl.jp, pl.grid, pl.lds_bytes, B->helpers[B->jit_stream[i]]));
+    JitParams jp = pl.jp;
+    if (B->jit_share[i] >= 0) {
+      jp.steal = B->d_queue + 2 + B->jit_share[i];
+      jp.sync = B->d_queue;
+      jp.sync_total = total;
+      jp.sync_words = 2u + B->nshare;
+    }
+    QCF_TRY(jit_launch(pl.function, jp, pl.grid, pl.lds_bytes, B->helpers[B->jit_stream[i]]));
   }
   if (conc) QCF_TRY(persistent());
   for (int h = 0; h < kBatchStreams; ++h) {
@@ -1068,6 +1185,7 @@ extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
     if (!use_jit || kp.nblocks == 0) continue;
     // compiled here (once per distinct kernel source), launched later
     qcf_program *P = const_cast<qcf_program *>(progs[k]);
+    if (tune && tune->jit_max_planes && P->nplanes > tune->jit_max_planes) continue;
     const bool big = (int)(P->nplanes * 256u) > lds_max;
     JitPlan pl;
     if (jit_plan(P, kp, shots[k], 1, big, 0, ld[k], &pl) == QCF_OK && !pl.global_planes) {
@@ -1077,22 +1195,23 @@ extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
       clear_error();
     }
   }
-  if (int st = plan_jit(B); st != QCF_OK) {
-    delete B;
-    return st;
-  }
+  B->nstreams = tune && tune->streams ? std::max(1, std::min(kBatchStreams, (int)tune->streams))
+                                      : batch_streams();
   // circuits whose frame does not fit in LDS keep their planes in a
-  // batch-owned global buffer: one slot of the largest such frame per wave
+  // batch-owned global buffer, one slot of the largest such frame per wave:
+  // those of the persistent kernel, and those of specialised launches whose
+  // tiles the persistent kernel may share
   const bool force_gl = force_global_frames();
   if (tune && tune->lds_cap) lds_max = std::min<int>(lds_max, (int)tune->lds_cap);
+  auto global_frame = [&](size_t k) {
+    return (int)(progs[k]->nplanes * 256u) > lds_max || (force_gl && !B->on_jit[k]);
+  };
   uint32_t gl_planes = 0;
   for (size_t k = 0; k < n; ++k) {
-    if (B->on_jit[k]) continue;
-    const uint32_t lb = progs[k]->nplanes * 256u;
-    if ((int)lb > lds_max || force_gl)
+    if (global_frame(k))
       gl_planes = std::max(gl_planes, progs[k]->nplanes);
-    else
-      B->lds_bytes = std::max(B->lds_bytes, lb);
+    else if (!B->on_jit[k])
+      B->lds_bytes = std::max(B->lds_bytes, progs[k]->nplanes * 256u);
   }
   if (gl_planes) {
     DeviceState *st;
@@ -1106,18 +1225,23 @@ extern "C" int qcf_batch_create(const qcf_program *const *progs, size_t n,
       return set_error(QCF_ERR_ALLOC, "batch frame buffer (%zu bytes)", slots * gl_planes * 256u);
     }
     for (size_t k = 0; k < n; ++k) {
-      const uint32_t lb = progs[k]->nplanes * 256u;
-      if (!B->on_jit[k] && ((int)lb > lds_max || force_gl)) {
+      if (global_frame(k)) {
         B->params[k].scratch = B->d_frames;
         B->params[k].sstride = gl_planes * 64u;
       }
     }
   }
+  if (int st = plan_jit(B); st != QCF_OK) {
+    delete B;
+    return st;
+  }
   hipError_t e = hipMalloc(&B->d_params, std::max<size_t>(1, n) * sizeof(KParams));
   if (e == hipSuccess && n)
     e = hipMemcpy(B->d_params, B->params.data(), n * sizeof(KParams), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMalloc(&B->d_queue, 2 * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMemset(B->d_queue, 0, 2 * sizeof(uint32_t));
+  // [0] next own item, [1] workgroups done, [2 + g] next tile of shared group g
+  const size_t nq = 2 + B->jit.size();
+  if (e == hipSuccess) e = hipMalloc(&B->d_queue, nq * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemset(B->d_queue, 0, nq * sizeof(uint32_t));
   if (e != hipSuccess) {
     delete B;
     return set_error(QCF_ERR_HIP, "batch setup: %s", hipGetErrorString(e));
@@ -1184,7 +1308,7 @@ extern "C" int qcf_autotune(const qcf_program *const *progs, size_t n,
                             const uint64_t *shots, qcf_tune *out, void *stream) {
   clear_error();
   if (!out) return set_error(QCF_ERR_ARGUMENT, "null output pointer");
-  *out = qcf_tune{0, 0, 0};
+  *out = qcf_tune{0, 0, 0, 0, 0, 0, 0};
   if (n == 0) return QCF_OK;
   hipStream_t s = (hipStream_t)stream;
   // scratch records so the caller's buffers are untouched
@@ -1201,40 +1325,56 @@ extern "C" int qcf_autotune(const qcf_program *const *progs, size_t n,
   if (st == QCF_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
     st = set_error(QCF_ERR_HIP, "autotune events");
   float best = 1e30f;
-  qcf_tune best_t{0, 0, 0};
-  // frames above the LDS cap go to global memory: fewer bytes of LDS per
-  // wave, more resident waves; the grid takes as many waves as LDS allows
-  const uint32_t caps[] = {0u, 72u * 1024u, 36u * 1024u, 14u * 1024u};
-  const uint32_t tbs[] = {1, 4};
-  // waves per CU: the default (16 or what LDS allows), or the most that
-  // small on-chip frames allow (waves on global frames wait on L2 latency)
-  const uint32_t wpcs[] = {0u, kMaxBatchWavesPerCU};
-  for (uint32_t cap : caps) {
-    if (st != QCF_OK) break;
+  qcf_tune best_t{0, 1, 0, 0, 0, 0, 0};
+  auto time_one = [&](qcf_batch *B, const qcf_tune &t) -> int {
+    QCF_TRY(qcf::set_tune(B, &t));
+    QCF_TRY(qcf::batch_launch(B, s));  // warm
+    if (hipEventRecord(e0, s) != hipSuccess) return set_error(QCF_ERR_HIP, "event");
+    QCF_TRY(qcf::batch_launch(B, s));
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+      return set_error(QCF_ERR_HIP, "event");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best) {
+      best = ms;
+      best_t = t;
+    }
+    return QCF_OK;
+  };
+  // Two batch layouts: every circuit that has a specialised kernel runs it,
+  // or the circuits of the largest plane count run on the persistent kernel
+  // only.  On each: 1 or 2 helper streams, the persistent kernel sharing the
+  // tiles of the launches whose frames exceed LDS or not, its grid at the
+  // default or the largest size, and, when it has circuits of its own, after
+  // or beside the specialised launches.
+  uint32_t lim = 0;
+  for (int layout = 0; layout < 2 && st == QCF_OK; ++layout) {
     qcf_batch *B = nullptr;
-    qcf_tune t0{0, 1, cap};
-    if ((st = qcf_batch_create(progs, n, shots, seeds.data(), bufs.data(), lds.data(), &t0, &B)) !=
+    const qcf_tune tb{0, 1, 0, lim, 0, 0, 0};
+    if ((st = qcf_batch_create(progs, n, shots, seeds.data(), bufs.data(), lds.data(), &tb, &B)) !=
         QCF_OK)
       break;
-    for (uint32_t k = 0; k < 4 && st == QCF_OK; ++k) {
-      const uint32_t tb = tbs[k & 1], wpc = wpcs[k >> 1];
-      qcf_tune t{wpc, tb, cap};
-      if ((st = qcf::set_tune(B, &t)) != QCF_OK) break;
-      if ((st = qcf::batch_launch(B, s)) != QCF_OK) break;  // warm
-      if (hipEventRecord(e0, s) != hipSuccess) { st = set_error(QCF_ERR_HIP, "event"); break; }
-      if ((st = qcf::batch_launch(B, s)) != QCF_OK) break;
-      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-        st = set_error(QCF_ERR_HIP, "event");
-        break;
+    uint32_t largest = 0;
+    for (size_t k = 0; k < n; ++k)
+      if (B->on_jit[k]) largest = std::max(largest, progs[k]->nplanes);
+    bool stealable = false;
+    for (size_t i = 0; i < B->jit.size(); ++i) stealable = stealable || B->jit_stealable[i];
+    const bool own = !B->items.empty();
+    for (uint32_t streams : {1u, 2u})
+      for (uint32_t steal : {0u, 0xFFFFFFFFu}) {
+        if (steal == 0u && !stealable) continue;
+        const bool persistent = own || steal == 0u;
+        for (uint32_t conc : {0u, 1u}) {
+          if (conc && (!own || B->jit.empty() || steal == 0u)) continue;
+          for (uint32_t wpc : {0u, kMaxBatchWavesPerCU}) {
+            if (wpc && !persistent) continue;
+            if (st == QCF_OK) st = time_one(B, qcf_tune{wpc, 1, 0, lim, streams, conc, steal});
+          }
+        }
       }
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best) {
-        best = ms;
-        best_t = t;
-      }
-    }
     qcf_batch_free(B);
+    if (largest <= 1) break;
+    lim = largest - 1u;
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

@@ -52,6 +52,9 @@ struct KJ {
   const uint32_t *first;
   uint32_t ncirc, ntiles;
   uint32_t tile0;     // absolute index of tile 0: keys the RNG
+  uint32_t *steal;    // shared tiles: the next tile index (null: dealt statically)
+  uint32_t *sync;     // shared tiles: [1] workgroups done, cleared by the last
+  uint32_t sync_total, sync_words;
 };
 struct FlipLDS {
   uint32_t *L;
@@ -925,8 +928,16 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   const uint32_t nx = P0.xcds ? P0.xcds : 1u;
   const uint32_t per = gridDim.x / nx, xi = blockIdx.x % nx, wi = blockIdx.x / nx;
   const uint32_t span = (ntiles + nx - 1u) / nx;
-  for (uint32_t t = wi; t < span; t += per) {
-    const uint32_t tile = xi * span + t;
+  for (uint32_t t = wi;; t += per) {
+    uint32_t tile;
+    if (P0.steal) {  // shared with the persistent batch kernel: take the next tile
+      uint32_t v = 0u;
+      if (l == 0u) v = atomicAdd(P0.steal, 1u);
+      tile = __builtin_amdgcn_readfirstlane(v);
+    } else {
+      if (t >= span) break;
+      tile = xi * span + t;
+    }
     if (tile >= ntiles) break;
     // the circuit of this tile in a multi-circuit launch: the last i with
     // first[i] <= tile (uniform, so scalar loads, which never wait behind
@@ -997,7 +1008,15 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
         << "    const uint32_t lm = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1u ? P.lastmask : "
            "0xFFFFFFFFu);\n"
         << part.str() << "    }\n";
-  src << "  }\n}\n";
+  src << R"QCFJIT(  }
+  if (P0.sync && l == 0u) {  // the last workgroup of the sharing launches clears the counters
+    __threadfence();
+    if (atomicAdd(&P0.sync[1], 1u) == P0.sync_total - 1u)
+      for (uint32_t i = 0; i < P0.sync_words; ++i)
+        __hip_atomic_store(&P0.sync[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+)QCFJIT";
   *out = src.str();
   if (lds_bytes)
     *lds_bytes = glob    ? 0u

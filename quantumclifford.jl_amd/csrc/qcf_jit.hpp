@@ -26,6 +26,13 @@ struct JitParams {
   const uint32_t *first;
   uint32_t ncirc, ntiles;
   uint32_t tile0;     // absolute index of tile 0: keys the RNG
+  // shared tiles (steal non-null): workgroups take tile indices from *steal,
+  // which the persistent batch kernel draws from too; every workgroup of the
+  // launches that share counters adds one to sync[1] when it leaves, and the
+  // last of sync_total clears sync[0 .. sync_words)
+  uint32_t *steal;
+  uint32_t *sync;
+  uint32_t sync_total, sync_words;
 };
 
 // QCF_NO_JIT=1 keeps every launch on the interpreter.

@@ -57,7 +57,12 @@ class ProgramInfo(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("waves_per_cu", C.c_uint32), ("tile_blocks", C.c_uint32),
-                ("lds_cap", C.c_uint32)]
+                ("lds_cap", C.c_uint32), ("jit_max_planes", C.c_uint32),
+                ("streams", C.c_uint32), ("concurrent", C.c_uint32),
+                ("steal_planes", C.c_uint32)]
+
+    def as_list(self):
+        return [getattr(self, f) for f, _ in self._fields_]
 
 
 _P = C.c_void_p

@@ -29,8 +29,7 @@ def test_512_circuit_batch_equals_single_launches(gpu):
     b = batch.Batch(progs, shots, seeds, tune)
     b.launch_graph()
     torch.cuda.synchronize()
-    print(f"batched launch done (tune {tune.waves_per_cu}, {tune.tile_blocks}, {tune.lds_cap})",
-          flush=True)
+    print(f"batched launch done (tune {tune.as_list()})", flush=True)
     st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     bad = []
     for k, (p, s, sd) in enumerate(zip(progs, shots, seeds)):
@@ -41,6 +40,27 @@ def test_512_circuit_batch_equals_single_launches(gpu):
         n = p.nbits * ld
         if not torch.equal(single[:n], b.records[k][:n]):
             bad.append((k, p.nq, s))
+    assert not bad, bad[:10]
+    # the other launch shapes the tuner can choose give the same bits: the
+    # persistent kernel sharing the tiles of the launches whose frames exceed
+    # LDS or not, and every circuit of the largest frames on the persistent
+    # kernel, started beside the specialised launches on one helper stream
+    big = max(p.info.nplanes for p in progs)
+    for fields in ((0, 1, 0, 0, 2, 0, 0), (0, 1, 0, 0, 2, 0, 0xFFFFFFFF),
+                   (32, 1, 0, big - 1, 1, 1, 0xFFFFFFFF)):
+        alt = qcf._lib.Tune(*fields)
+        if alt.as_list() == tune.as_list():
+            continue
+        b2 = batch.Batch(progs, shots, seeds, alt)
+        b2.launch_graph()
+        torch.cuda.synchronize()
+        print(f"launch shape {alt.as_list()} done", flush=True)
+        for k, p in enumerate(progs):
+            n = p.nbits * qcf.frames.plane_words(shots[k])
+            if not torch.equal(b.records[k][:n], b2.records[k][:n]):
+                bad.append((fields, k, p.nq, shots[k]))
+        del b2
+        torch.cuda.empty_cache()
     assert not bad, bad[:10]
     del b
     torch.cuda.empty_cache()

@@ -467,6 +467,37 @@ def test_batched_with_large_circuits(gpu, monkeypatch, batch_jit):
         assert np.array_equal(o, single), k
 
 
+@pytest.mark.parametrize("steal", [0, 0xFFFFFFFF])
+def test_shared_tiles_over_repeated_launches(gpu, steal):
+    """Tiles of the specialised launches whose frames exceed LDS (d = 13, 15)
+    taken by both those launches and the persistent kernel from one counter
+    per launch (steal_planes 0), or by the launches alone (0xFFFFFFFF), equal
+    single launches; the counters reset themselves, so every further launch
+    and graph replay (records refilled with ones in between) writes them all
+    again."""
+    rng = np.random.default_rng(31)
+    progs, shots, seeds = [], [], []
+    for d in (15, 13, 5, 13, 15, 3):
+        c = circuits.rotated_surface_code(d, d, float(10 ** rng.uniform(-4, -2)))
+        progs.append(qcf.Program(c.ops))
+        shots.append(int(10 ** rng.uniform(4, 4.7)))
+        seeds.append(int(rng.integers(0, 2**62)))
+    from qcf import batch
+    singles = [qcf.pftrajectories(p, s, seed=sd, keep_frame=False).record
+               for p, s, sd in zip(progs, shots, seeds)]
+    t = qcf._lib.Tune()
+    t.steal_planes = steal
+    b = batch.Batch(progs, shots, seeds, t)
+    for run, go in enumerate((b.launch, b.launch, b.launch_graph, b.launch_graph, b.launch)):
+        for r in b.records:
+            r.fill_(-1)
+        go()
+        torch.cuda.synchronize()
+        for k, (p, s) in enumerate(zip(progs, shots)):
+            n = p.nbits * qcf.frames.plane_words(s)
+            assert torch.equal(b.records[k][:n], singles[k][:n]), (run, k, p.nq, s)
+
+
 def test_wave_scan_is_exact(gpu):
     """The DPP prefix sum that places noise events equals numpy's cumsum."""
     rng = np.random.default_rng(1)
