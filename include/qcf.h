@@ -307,8 +307,9 @@ typedef struct qcf_tune {
                               fit in LDS and have more planes than this share
                               their tiles with the persistent kernel, which then
                               runs beside them: both take tiles from one counter
-                              per launch (0 = every such launch, 0xFFFFFFFF =
-                              none); the records do not depend on the split     */
+                              per launch (0 = those with more than 1024 planes,
+                              0xFFFFFFFF = none); the records do not depend on
+                              the split                                         */
 } qcf_tune;
 
 /*
@@ -323,7 +324,8 @@ typedef struct qcf_tune {
  * ONE persistent kernel for the circuits without one (frames beyond what
  * registers plus LDS hold); its workgroups pull (circuit, tile range) items
  * from a device queue.  By default the specialised launches whose frames
- * exceed LDS (they run one wave per CU) share their tiles with it: the
+ * exceed LDS and have more than 1024 planes (their part-tile kernels hold
+ * 32 shot words per CU) share their tiles with it: the
  * persistent kernel then starts beside them and, once its own items are
  * done, takes tiles from the same per-launch counters they take them from
  * (qcf_tune.steal_planes).  The last workgroup to leave resets the
@@ -350,12 +352,12 @@ int qcf_pf_sample_batched(const qcf_program *const *progs, size_t n,
 
 /* Pick qcf_tune for a set of circuits by timing batched launches of them on
  * the current device (on scratch records; the choice never changes results,
- * the RNG being keyed on absolute shots).  Candidates: which frame sizes keep
- * their specialised kernel (all of them, or all but the largest plane count,
- * whose circuits then run on the persistent kernel), 1 or 2 helper streams,
- * sharing the tiles of the launches whose frames exceed LDS or not, and --
- * when the persistent kernel has work -- starting it beside the specialised
- * launches or after them, at the default or the largest waves per CU. */
+ * the RNG being keyed on absolute shots).  Candidates: 1 or 2 helper
+ * streams; the persistent kernel sharing the tiles of no specialised launch,
+ * or of those of the largest one, two or three frame sizes among the ones it
+ * can share, at 8 or 16 waves per CU; and, when it has circuits of its own
+ * and shares nothing, starting it beside the specialised launches or after
+ * them. */
 int qcf_autotune(const qcf_program *const *progs, size_t n,
                  const uint64_t *shots, qcf_tune *out, void *stream);
 

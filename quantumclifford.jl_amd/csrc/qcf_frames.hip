@@ -905,11 +905,23 @@ int upload_items(qcf_batch *B) {
 
 // Specialised launches whose circuits the persistent kernel can run (frames
 // in its global slots) and whose frames have more than `planes` planes share
-// their tiles with it (0xFFFFFFFF: none): both kernels take tile indices
-// from one counter per launch, so the faster one takes more.  The tiles are
-// the same whichever kernel runs them (every draw is keyed on the tile), so
-// the records do not depend on the split.
+// their tiles with it (0: more than kStealPlanes; 0xFFFFFFFF: none): both
+// kernels take tile indices from one counter per launch.  The tiles are the
+// same whichever kernel runs them (every draw is keyed on the tile), so the
+// records do not depend on the split.
+//
+// It pays only where the specialised kernel is slow per tile: a persistent
+// wave that takes a tile holds it for far longer (it interprets the program
+// on global-memory planes, beside thousands of others), so the launch ends
+// up to one such tile after the specialised kernel would have.  On the
+// 512-circuit memory batch (d = 3..15, 1.6e7 to 2e7 shots per distance),
+// sharing took d = 15 (1237 planes) from 108 to 55 ms at 8 persistent waves
+// per CU, but d = 13 (929 planes) from 46 to 48-53 ms and d = 11 (665) from
+// 14 to 26-30 ms.
+constexpr uint32_t kStealPlanes = 1024;
+constexpr uint32_t kStealWavesPerCU = 8;  // persistent grid when it shares tiles
 int share_tiles(qcf_batch *B, uint32_t planes) {
+  if (planes == 0) planes = kStealPlanes;
   B->jit_share.assign(B->jit.size(), -1);
   std::vector<WorkItem> shared;
   std::vector<uint32_t> first{0u};
@@ -949,12 +961,12 @@ int set_tune(qcf_batch *B, const qcf_tune *tune) {
   uint32_t wpc = tune ? tune->waves_per_cu : 0;
   uint32_t tb = tune ? tune->tile_blocks : 0;
   const uint32_t lds_cap = std::max<uint32_t>(1, (160u * 1024u) / std::max(1u, B->lds_bytes));
-  if (wpc == 0) wpc = std::min<uint32_t>(16, lds_cap);
+  QCF_TRY(share_tiles(B, tune ? tune->steal_planes : 0u));
+  if (wpc == 0) wpc = std::min<uint32_t>(B->nshare ? kStealWavesPerCU : 16u, lds_cap);
   wpc = std::max<uint32_t>(1, std::min(std::min(wpc, lds_cap), kMaxBatchWavesPerCU));
   if (tb == 0) tb = 1;
   make_items(B, tb);
   QCF_TRY(upload_items(B));
-  QCF_TRY(share_tiles(B, tune ? tune->steal_planes : 0u));
   uint64_t work = B->items.size();
   for (size_t i = 0; i < B->jit.size(); ++i)
     if (B->jit_share[i] >= 0) work += B->jit_tiles[i].size();
@@ -1341,41 +1353,39 @@ extern "C" int qcf_autotune(const qcf_program *const *progs, size_t n,
     }
     return QCF_OK;
   };
-  // Two batch layouts: every circuit that has a specialised kernel runs it,
-  // or the circuits of the largest plane count run on the persistent kernel
-  // only.  On each: 1 or 2 helper streams, the persistent kernel sharing the
-  // tiles of the launches whose frames exceed LDS or not, its grid at the
-  // default or the largest size, and, when it has circuits of its own, after
-  // or beside the specialised launches.
-  uint32_t lim = 0;
-  for (int layout = 0; layout < 2 && st == QCF_OK; ++layout) {
-    qcf_batch *B = nullptr;
-    const qcf_tune tb{0, 1, 0, lim, 0, 0, 0};
-    if ((st = qcf_batch_create(progs, n, shots, seeds.data(), bufs.data(), lds.data(), &tb, &B)) !=
-        QCF_OK)
-      break;
-    uint32_t largest = 0;
-    for (size_t k = 0; k < n; ++k)
-      if (B->on_jit[k]) largest = std::max(largest, progs[k]->nplanes);
-    bool stealable = false;
-    for (size_t i = 0; i < B->jit.size(); ++i) stealable = stealable || B->jit_stealable[i];
+  // Every circuit that has a specialised kernel runs it, on 1 or 2 helper
+  // streams; the persistent kernel shares the tiles of no launch or of those
+  // whose frames have more planes than one of the three largest plane counts
+  // among the launches it can share (the largest only, the two largest, the
+  // three largest), at 8 or 16 waves per CU; when it has circuits of its own
+  // and shares nothing, it starts after the launches or beside them.
+  qcf_batch *B = nullptr;
+  const qcf_tune tb{0, 1, 0, 0, 0, 0, 0xFFFFFFFFu};
+  if (st == QCF_OK)
+    st = qcf_batch_create(progs, n, shots, seeds.data(), bufs.data(), lds.data(), &tb, &B);
+  if (st == QCF_OK) {
+    std::vector<uint32_t> sizes;
+    for (size_t i = 0; i < B->jit.size(); ++i)
+      if (B->jit_stealable[i]) sizes.push_back(B->jit_planes[i]);
+    std::sort(sizes.begin(), sizes.end());
+    sizes.erase(std::unique(sizes.begin(), sizes.end()), sizes.end());
+    std::vector<uint32_t> steals{0xFFFFFFFFu};
+    for (size_t j = 0; j < 3 && j < sizes.size(); ++j)
+      steals.push_back(sizes[sizes.size() - 1 - j] - 1u);
     const bool own = !B->items.empty();
     for (uint32_t streams : {1u, 2u})
-      for (uint32_t steal : {0u, 0xFFFFFFFFu}) {
-        if (steal == 0u && !stealable) continue;
-        const bool persistent = own || steal == 0u;
+      for (uint32_t steal : steals) {
+        const bool shares = steal != 0xFFFFFFFFu;
         for (uint32_t conc : {0u, 1u}) {
-          if (conc && (!own || B->jit.empty() || steal == 0u)) continue;
-          for (uint32_t wpc : {0u, kMaxBatchWavesPerCU}) {
-            if (wpc && !persistent) continue;
-            if (st == QCF_OK) st = time_one(B, qcf_tune{wpc, 1, 0, lim, streams, conc, steal});
+          if (conc && (!own || B->jit.empty() || shares)) continue;
+          for (uint32_t wpc : {8u, 16u}) {
+            if (!own && !shares && wpc != 8u) continue;  // no persistent kernel
+            if (st == QCF_OK) st = time_one(B, qcf_tune{wpc, 1, 0, 0, streams, conc, steal});
           }
         }
       }
-    qcf_batch_free(B);
-    if (largest <= 1) break;
-    lim = largest - 1u;
   }
+  if (B) qcf_batch_free(B);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   for (auto *p : bufs)

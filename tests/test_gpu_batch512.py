@@ -41,12 +41,12 @@ def test_512_circuit_batch_equals_single_launches(gpu):
         if not torch.equal(single[:n], b.records[k][:n]):
             bad.append((k, p.nq, s))
     assert not bad, bad[:10]
-    # the other launch shapes the tuner can choose give the same bits: the
-    # persistent kernel sharing the tiles of the launches whose frames exceed
-    # LDS or not, and every circuit of the largest frames on the persistent
+    # other launch shapes give the same bits: the persistent kernel sharing
+    # the tiles of the launches of more than 1024 planes, of all whose frames
+    # exceed LDS, or of none, and every circuit of the largest frames on the persistent
     # kernel, started beside the specialised launches on one helper stream
     big = max(p.info.nplanes for p in progs)
-    for fields in ((0, 1, 0, 0, 2, 0, 0), (0, 1, 0, 0, 2, 0, 0xFFFFFFFF),
+    for fields in ((0, 1, 0, 0, 2, 0, 0), (0, 1, 0, 0, 2, 0, 1), (0, 1, 0, 0, 2, 0, 0xFFFFFFFF),
                    (32, 1, 0, big - 1, 1, 1, 0xFFFFFFFF)):
         alt = qcf._lib.Tune(*fields)
         if alt.as_list() == tune.as_list():

@@ -467,14 +467,14 @@ def test_batched_with_large_circuits(gpu, monkeypatch, batch_jit):
         assert np.array_equal(o, single), k
 
 
-@pytest.mark.parametrize("steal", [0, 0xFFFFFFFF])
+@pytest.mark.parametrize("steal", [0, 1, 0xFFFFFFFF])
 def test_shared_tiles_over_repeated_launches(gpu, steal):
-    """Tiles of the specialised launches whose frames exceed LDS (d = 13, 15)
-    taken by both those launches and the persistent kernel from one counter
-    per launch (steal_planes 0), or by the launches alone (0xFFFFFFFF), equal
-    single launches; the counters reset themselves, so every further launch
-    and graph replay (records refilled with ones in between) writes them all
-    again."""
+    """Tiles of the specialised launches whose frames exceed LDS taken by both
+    those launches and the persistent kernel from one counter per launch (d =
+    15 with steal_planes 0, d = 13 and 15 with 1), or by the launches alone
+    (0xFFFFFFFF), equal single launches; the counters reset themselves, so
+    every further launch and graph replay (records refilled with ones in
+    between) writes them all again."""
     rng = np.random.default_rng(31)
     progs, shots, seeds = [], [], []
     for d in (15, 13, 5, 13, 15, 3):
