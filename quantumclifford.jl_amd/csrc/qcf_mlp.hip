@@ -35,6 +35,7 @@ namespace qcf {
 namespace {
 
 constexpr int kTile = 128;  // output tile edge
+constexpr int kTS = kTile + 8;  // LDS row stride of the epilogue's transposed tile
 constexpr int kBK = 32;     // K per LDS stage
 constexpr int kMaxSplitK = 16;
 constexpr bool kWideK = true;  // 64-wide K stages when K allows
@@ -53,6 +54,7 @@ struct GemmArgs {
   bf16 *Cb;            // bf16 output
   const float *bias;   // per column
   const float *pre_in; // kEpiDGelu
+  bf16 *CbT;           // optional transposed copy of Cb: CbT[n * M + m]
 };
 
 __device__ __forceinline__ float gelu(float x) {
@@ -123,8 +125,13 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
   // STAGES == 1: one LDS buffer (half the LDS, twice the resident
   // workgroups), two barriers per stage; both prefetch the next stage into
   // registers while the matrix cores work on the current one
-  __shared__ __attribute__((aligned(16))) bf16 As[STAGES][kTile][BK + 8];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[STAGES][kTile][BK + 8];
+  // operand stages As, Bs; after the K loop the same LDS holds the output
+  // tile's transpose for the optional batch-minor copy (rows padded to kTS)
+  constexpr int kStage = kTile * (BK + 8);
+  constexpr int kLds = 2 * STAGES * kStage > kTile * kTS ? 2 * STAGES * kStage : kTile * kTS;
+  __shared__ __attribute__((aligned(16))) bf16 lds_raw[kLds];
+  bf16(*As)[kTile][BK + 8] = reinterpret_cast<bf16(*)[kTile][BK + 8]>(lds_raw);
+  bf16(*Bs)[kTile][BK + 8] = reinterpret_cast<bf16(*)[kTile][BK + 8]>(lds_raw + STAGES * kStage);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
@@ -188,23 +195,49 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
       const int n = n0 + wn * 64 + j * 32 + (lane & 31);
       const float bn = (EPI == kEpiBiasGelu || EPI == kEpiBiasGeluInfer) ? g.bias[n] : 0.0f;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const size_t o = (size_t)m * g.ldc + n;
-        const float v = acc[i][j][r];
-        if (EPI == kEpiStoreF32) {
-          g.Cf[o + (size_t)blockIdx.z * g.M * g.ldc] = v;
-        } else if (EPI == kEpiBiasGelu) {
-          const float p = v + bn;
-          g.Cf[o] = p;
-          g.Cb[o] = (bf16)gelu(p);
-        } else if (EPI == kEpiBiasGeluInfer) {
-          g.Cb[o] = (bf16)gelu(v + bn);
-        } else {
-          g.Cb[o] = (bf16)(v * dgelu(g.pre_in[o]));
+      for (int q = 0; q < 4; ++q) {
+        // r = 4q .. 4q+3 are four consecutive rows m
+        const int mq = m0 + wm * 64 + i * 32 + 8 * q + 4 * (lane >> 5);
+        uint16_t h[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int r = 4 * q + u, m = mq + u;
+          const size_t o = (size_t)m * g.ldc + n;
+          const float v = acc[i][j][r];
+          bf16 b = (bf16)0.0f;
+          if (EPI == kEpiStoreF32) {
+            g.Cf[o + (size_t)blockIdx.z * g.M * g.ldc] = v;
+          } else if (EPI == kEpiBiasGelu) {
+            const float p = v + bn;
+            g.Cf[o] = p;
+            b = (bf16)gelu(p);
+          } else if (EPI == kEpiBiasGeluInfer) {
+            b = (bf16)gelu(v + bn);
+          } else {
+            b = (bf16)(v * dgelu(g.pre_in[o]));
+          }
+          if (EPI != kEpiStoreF32) g.Cb[o] = b;
+          h[u] = __builtin_bit_cast(uint16_t, b);
+        }
+        if (EPI != kEpiStoreF32 && g.CbT) {
+          // the tile's transpose goes through LDS: T[n - n0][m - m0]
+          uint16_t *T = (uint16_t *)lds_raw;
+          *(uint2 *)(T + (size_t)(n - n0) * kTS + (mq - m0)) =
+              uint2{h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16)};
         }
       }
     }
+  if (EPI != kEpiStoreF32 && g.CbT) {
+    // 128 rows n of 128 consecutive m: 16 bytes per thread, 256 per row
+    __syncthreads();
+    const uint16_t *T = (const uint16_t *)lds_raw;
+#pragma unroll
+    for (int h = 0; h < kTile * kTile / 8 / 256; ++h) {
+      const int c = t + h * 256, row = c >> 4, col = (c & 15) * 8;
+      *(uint4 *)(g.CbT + (size_t)(n0 + row) * g.M + m0 + col) =
+          *(const uint4 *)(T + (size_t)row * kTS + col);
+    }
+  }
 }
 
 // out[i] = sum_z part[z][i] in a fixed order (split-K reduction).
@@ -261,7 +294,8 @@ int launch_gemm(const GemmArgs &g, hipStream_t s, int splitk = 1, float *part = 
 // Syndromes of `rows` shots starting at shot `s0` (a multiple of 64) of a
 // packed record: x[m][j] = detector j of shot m (0 past the detectors and past
 // the shots), optionally the transpose xT[j][m], and y[m] = observable.  One
-// workgroup per 64 shots: each detector word is read once.
+// workgroup per 64 shots and gridDim.y-th of the columns: each detector word
+// is read once.
 __global__ void gather_inputs(const uint64_t *rec, size_t ld, uint64_t s0, uint64_t nshots,
                               const uint32_t *det_cols, uint32_t ndet, uint32_t kin,
                               uint32_t obs_col, float *y, uint16_t *x, uint16_t *xT,
@@ -271,7 +305,7 @@ __global__ void gather_inputs(const uint64_t *rec, size_t ld, uint64_t s0, uint6
   const uint64_t left = nshots > first ? nshots - first : 0;
   const uint64_t valid = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
   constexpr uint16_t kOne = 0x3F80;  // bf16 1.0
-  for (uint32_t j = threadIdx.x; j < kin; j += blockDim.x) {
+  for (uint32_t j = blockIdx.y * blockDim.x + threadIdx.x; j < kin; j += gridDim.y * blockDim.x) {
     uint64_t w = 0;
     if (j < ndet && valid) w = rec[(size_t)det_cols[j] * ld + wi] & valid;
     for (uint32_t b = 0; b < 64; ++b) x[(size_t)(m0 + b) * kin + j] = ((w >> b) & 1u) ? kOne : 0;
@@ -287,7 +321,7 @@ __global__ void gather_inputs(const uint64_t *rec, size_t ld, uint64_t s0, uint6
       }
     }
   }
-  if (y && threadIdx.x < 64) {
+  if (y && blockIdx.y == 0 && threadIdx.x < 64) {
     const uint64_t o = valid ? rec[(size_t)obs_col * ld + wi] & valid : 0;
     y[m0 + threadIdx.x] = (float)((o >> threadIdx.x) & 1u);
   }
@@ -309,76 +343,150 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
-// Output layer h3 -> 1: z[m] = a3[m] . w4 + b4, one wave per row.  With
-// labels: loss += mean(softplus(z) - y z), dz = (sigmoid(z) - y) / rows.
+// Output layer h3 -> 1: z[m] = a3[m] . w4 + b4, one wave per row, each wave
+// kHeadRows rows.  With labels: dz = (sigmoid(z) - y) / rows, and the block's
+// share of mean(softplus(z) - y z) goes to lpart[block] (summed by
+// head_sums in a fixed order: one atomic per row on one address serialises).
+constexpr uint32_t kHeadRows = 8;
 __global__ void head_forward(const bf16 *a3, const bf16 *w4, const float *b4, uint32_t h3,
-                             uint32_t rows, float *z, const float *y, float *dz, float *loss,
+                             uint32_t rows, float *z, const float *y, float *dz, float *lpart,
                              float inv_rows) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t m = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (m >= rows) return;
-  float s = 0.0f;
-  for (uint32_t k = lane; k < h3; k += 64) s += (float)a3[(size_t)m * h3 + k] * (float)w4[k];
-  s = wave_sum(s) + b4[0];
-  if (lane == 0) {
-    z[m] = s;
-    if (y) {
-      const float yy = y[m];
-      const float sp = s > 0 ? s + log1pf(__expf(-s)) : log1pf(__expf(s));
-      const float sg = 1.0f / (1.0f + __expf(-s));
-      dz[m] = (sg - yy) * inv_rows;
-      if (loss) atomicAdd(loss, (sp - yy * s) * inv_rows);
+  __shared__ float sh[4];
+  const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  float lsum = 0.0f;
+  // 8 consecutive k per lane and step (16-byte loads); h3 is a multiple of 128
+  for (uint32_t i = 0; i < kHeadRows; ++i) {
+    const uint32_t m = (blockIdx.x * 4 + w) * kHeadRows + i;
+    if (m >= rows) break;
+    float s = 0.0f;
+    for (uint32_t k = lane * 8; k < h3; k += 512) {
+      const uint4 a = *(const uint4 *)(a3 + (size_t)m * h3 + k);
+      const uint4 b = *(const uint4 *)(w4 + k);
+      const uint32_t ua[4] = {a.x, a.y, a.z, a.w}, ub[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+      for (int h = 0; h < 4; ++h)
+        s += __builtin_bit_cast(float, ua[h] << 16) * __builtin_bit_cast(float, ub[h] << 16) +
+             __builtin_bit_cast(float, ua[h] & 0xFFFF0000u) *
+                 __builtin_bit_cast(float, ub[h] & 0xFFFF0000u);
     }
+    s = wave_sum(s) + b4[0];
+    if (lane == 0) {
+      z[m] = s;
+      if (y) {
+        const float yy = y[m];
+        const float sp = s > 0 ? s + log1pf(__expf(-s)) : log1pf(__expf(s));
+        const float sg = 1.0f / (1.0f + __expf(-s));
+        dz[m] = (sg - yy) * inv_rows;
+        lsum += (sp - yy * s) * inv_rows;
+      }
+    }
+  }
+  if (!lpart) return;
+  if (lane == 0) sh[w] = lsum;
+  __syncthreads();
+  if (threadIdx.x == 0) lpart[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+// d3[m][k] = dz[m] * w4[k] * gelu'(pre3[m][k]), written row-major and as the
+// batch-minor copy d3T[k][m], one 64 x 64 tile per workgroup through LDS.
+__global__ void head_backward(const float *dz, const bf16 *w4, const float *pre3, uint32_t h3,
+                              uint32_t rows, bf16 *d3, bf16 *d3T) {
+  __shared__ bf16 t[64][66];
+  const uint32_t r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const uint32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const float wk = (float)w4[c0 + tx];
+  for (uint32_t i = ty; i < 64; i += 4) {
+    const size_t o = (size_t)(r0 + i) * h3 + c0 + tx;
+    const bf16 v = (bf16)(dz[r0 + i] * wk * dgelu(pre3[o]));
+    d3[o] = v;
+    t[i][tx] = v;
+  }
+  __syncthreads();
+  for (uint32_t i = ty; i < 64; i += 4) d3T[(size_t)(c0 + i) * rows + r0 + tx] = t[tx][i];
+}
+
+// grad b4 = sum of dz over the batch and loss += sum of the head's block
+// partials, both in a fixed order.
+__global__ void head_sums(const float *dz, uint32_t n, const float *lpart, uint32_t nl,
+                          float *gb4, float *loss) {
+  __shared__ float sh[2][256];
+  float s = 0.0f, l = 0.0f;
+  for (uint32_t i = threadIdx.x; i < n; i += 256) s += dz[i];
+  if (loss)
+    for (uint32_t i = threadIdx.x; i < nl; i += 256) l += lpart[i];
+  sh[0][threadIdx.x] = s;
+  sh[1][threadIdx.x] = l;
+  __syncthreads();
+  for (uint32_t o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    gb4[0] = sh[0][0];
+    if (loss) loss[0] += sh[1][0];
   }
 }
 
-// d3[m][k] = dz[m] * w4[k] * gelu'(pre3[m][k])
-__global__ void head_backward(const float *dz, const bf16 *w4, const float *pre3, uint32_t h3,
-                              uint32_t rows, bf16 *d3) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)rows * h3) return;
-  const uint32_t m = (uint32_t)(i / h3), k = (uint32_t)(i % h3);
-  d3[i] = (bf16)(dz[m] * (float)w4[k] * dgelu(pre3[i]));
+// out[r] = sum of row r of a rows x cols bf16 matrix (cols a multiple of
+// 8), one workgroup per row, fixed order: the bias gradients as row sums
+// of the batch-minor gradient copies.
+__global__ void row_sums(const bf16 *T, uint32_t cols, float *out) {
+  __shared__ float sh[4];
+  const uint4 *row = (const uint4 *)(T + (size_t)blockIdx.x * cols);
+  float s = 0.0f;
+  for (uint32_t c = threadIdx.x; c < cols / 8; c += 256) {
+    const uint4 v = row[c];
+    const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+      s += __builtin_bit_cast(float, u[h] << 16) + __builtin_bit_cast(float, u[h] & 0xFFFF0000u);
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // Column sums out[n] = sum_m D[m][n] * (scale ? scale[m] : 1), in two fixed
-// passes (partials per 128-row chunk, then their sum) so the result does not
-// depend on scheduling.
+// passes (partials per kColChunk-row chunk, then their sum) so the result
+// does not depend on scheduling.
+constexpr uint32_t kColChunk = 256;
 __global__ void colsum_partial(const bf16 *D, const float *scale, uint32_t rows, uint32_t cols,
                                float *part) {
   const uint32_t n = blockIdx.x * 64 + (threadIdx.x & 63);
-  const uint32_t r0 = blockIdx.y * 128;
+  const uint32_t r0 = blockIdx.y * kColChunk;
   __shared__ float sh[4][64];
   float s = 0.0f;
-  if (n < cols)
-    for (uint32_t m = r0 + (threadIdx.x >> 6); m < min(rows, r0 + 128); m += 4)
+  if (n < cols) {
+    const uint32_t r1 = min(rows, r0 + kColChunk);
+#pragma unroll 8
+    for (uint32_t m = r0 + (threadIdx.x >> 6); m < r1; m += 4)
       s += (float)D[(size_t)m * cols + n] * (scale ? scale[m] : 1.0f);
+  }
   sh[threadIdx.x >> 6][threadIdx.x & 63] = s;
   __syncthreads();
   if (threadIdx.x < 64 && n < cols)
-    part[(size_t)blockIdx.y * cols + n] = sh[0][threadIdx.x] + sh[1][threadIdx.x] +
-                                          sh[2][threadIdx.x] + sh[3][threadIdx.x];
+    part[(size_t)blockIdx.y * cols + n] = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) +
+                                          (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
 }
 
+// out[n] = sum over chunks of part[c][n]: 64 columns per workgroup, the four
+// waves take every fourth chunk, then a fixed-order sum of the four.
 __global__ void colsum_final(const float *part, uint32_t nchunks, uint32_t cols, float *out) {
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= cols) return;
+  __shared__ float sh[4][64];
+  const uint32_t n = blockIdx.x * 64 + (threadIdx.x & 63), w = threadIdx.x >> 6;
   float s = 0.0f;
-  for (uint32_t c = 0; c < nchunks; ++c) s += part[(size_t)c * cols + n];
-  out[n] = s;
-}
-
-__global__ void vec_sum(const float *v, uint32_t n, float *out) {
-  __shared__ float sh[256];
-  float s = 0.0f;
-  for (uint32_t i = threadIdx.x; i < n; i += 256) s += v[i];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (uint32_t o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
+  if (n < cols) {
+#pragma unroll 8
+    for (uint32_t c = w; c < nchunks; c += 4) s += part[(size_t)c * cols + n];
   }
-  if (threadIdx.x == 0) out[0] = sh[0];
+  sh[w][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (threadIdx.x < 64 && n < cols)
+    out[n] = (sh[0][threadIdx.x] + sh[1][threadIdx.x]) + (sh[2][threadIdx.x] + sh[3][threadIdx.x]);
 }
 
 // Adam (Kingma & Ba 2015) on the fp32 master weights; refreshes the bf16 copy.
@@ -423,6 +531,7 @@ struct qcf_mlp {
   bf16 *WT = nullptr;  // transposed weights for the input gradients
   float *pre1 = nullptr, *pre2 = nullptr, *pre3 = nullptr;
   float *y = nullptr, *z = nullptr, *dz = nullptr, *part = nullptr, *lossbuf = nullptr;
+  float *lpart = nullptr;  // per-workgroup loss partials of the head
   float *kpart = nullptr;  // split-K partial sums of the weight gradients
   std::vector<void *> allocs;
   ~qcf_mlp() {
@@ -451,24 +560,26 @@ int forward(qcf_mlp *M, uint32_t rows, bool train, hipStream_t s) {
   GemmArgs g{};
   const bf16 *W1 = M->wb + M->off[0], *W2 = M->wb + M->off[2], *W3 = M->wb + M->off[4];
   const float *b1 = M->w + M->off[1], *b2 = M->w + M->off[3], *b3 = M->w + M->off[5];
+  // training also keeps the batch-minor copies of a1 and a2 that the weight
+  // gradients contract (written by the epilogue)
   auto layer = [&](const bf16 *X, int kin, const bf16 *W, const float *b, int nout, float *pre,
-                   bf16 *act) {
-    g = GemmArgs{X, W, (int)rows, nout, kin, kin, kin, nout, pre, act, b, nullptr};
+                   bf16 *act, bf16 *actT) {
+    g = GemmArgs{X, W, (int)rows, nout, kin, kin, kin, nout, pre, act, b, nullptr, actT};
     return train ? launch_gemm<true, true, kEpiBiasGelu>(g, s)
                  : launch_gemm<true, true, kEpiBiasGeluInfer>(g, s);
   };
-  QCF_TRY(layer(M->x, M->kin, W1, b1, M->h1, M->pre1, M->a1));
-  QCF_TRY(layer(M->a1, M->h1, W2, b2, M->h2, M->pre2, M->a2));
-  QCF_TRY(layer(M->a2, M->h2, W3, b3, M->h3, M->pre3, M->a3));
+  QCF_TRY(layer(M->x, M->kin, W1, b1, M->h1, M->pre1, M->a1, train ? M->a1T : nullptr));
+  QCF_TRY(layer(M->a1, M->h1, W2, b2, M->h2, M->pre2, M->a2, train ? M->a2T : nullptr));
+  QCF_TRY(layer(M->a2, M->h2, W3, b3, M->h3, M->pre3, M->a3, nullptr));
   return QCF_OK;
 }
 
 int colsum(qcf_mlp *M, const bf16 *D, const float *scale, uint32_t rows, uint32_t cols,
            float *out, hipStream_t s) {
-  const uint32_t nch = (rows + 127) / 128;
+  const uint32_t nch = (rows + kColChunk - 1) / kColChunk;
   hipLaunchKernelGGL(colsum_partial, dim3((cols + 63) / 64, nch), dim3(256), 0, s, D, scale,
                      rows, cols, M->part);
-  hipLaunchKernelGGL(colsum_final, dim3((cols + 255) / 256), dim3(256), 0, s, M->part, nch, cols,
+  hipLaunchKernelGGL(colsum_final, dim3((cols + 63) / 64), dim3(256), 0, s, M->part, nch, cols,
                      out);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
@@ -483,42 +594,45 @@ int transpose(const bf16 *src, bf16 *dst, uint32_t rows, uint32_t cols, hipStrea
 
 // Forward + backward of one batch already gathered into x / xT / y;
 // gradients in g.  Weight gradients sum over the batch, so they contract two
-// batch-minor operands (transposed copies of the activations and of the
-// back-propagated pre-activation gradients) with K-contiguous reads.
+// batch-minor operands (copies of the activations and of the back-propagated
+// pre-activation gradients, written by the kernels that produce them) with
+// K-contiguous reads; the bias gradients are row sums of the latter.
 int fwd_bwd(qcf_mlp *M, float *loss, hipStream_t s) {
   const uint32_t B = M->batch;
   QCF_TRY(forward(M, B, true, s));
   const bf16 *w4 = M->wb + M->off[6];
-  hipLaunchKernelGGL(head_forward, dim3(B / 4), dim3(256), 0, s, M->a3, w4, M->w + M->off[7],
-                     M->h3, B, M->z, M->y, M->dz, loss, 1.0f / (float)B);
-  hipLaunchKernelGGL(head_backward, dim3((unsigned)(((size_t)B * M->h3 + 255) / 256)), dim3(256),
-                     0, s, M->dz, w4, M->pre3, M->h3, B, M->d3);
+  const uint32_t nhead = B / (4 * kHeadRows);
+  hipLaunchKernelGGL(head_forward, dim3(nhead), dim3(256), 0, s, M->a3, w4, M->w + M->off[7],
+                     M->h3, B, M->z, M->y, M->dz, M->lpart, 1.0f / (float)B);
+  hipLaunchKernelGGL(head_backward, dim3(M->h3 / 64, B / 64), dim3(256), 0, s, M->dz, w4,
+                     M->pre3, M->h3, B, M->d3, M->dT);
   QCF_HIP_CHECK(hipGetLastError());
-  QCF_TRY(transpose(M->a1, M->a1T, B, M->h1, s));
-  QCF_TRY(transpose(M->a2, M->a2T, B, M->h2, s));
   float *G = M->g;
-  // output layer: grad w4 = a3^T dz, grad b4 = sum dz
+  // output layer: grad w4 = a3^T dz, grad b4 = sum dz (and the loss)
   QCF_TRY(colsum(M, M->a3, M->dz, B, M->h3, G + M->off[6], s));
-  hipLaunchKernelGGL(vec_sum, dim3(1), dim3(256), 0, s, M->dz, B, G + M->off[7]);
+  hipLaunchKernelGGL(head_sums, dim3(1), dim3(256), 0, s, M->dz, B, M->lpart, nhead,
+                     G + M->off[7], loss);
   const bf16 *W2 = M->wb + M->off[2], *W3 = M->wb + M->off[4];
-  // one layer back: grad W = dT aT^T, grad b = colsum d, and (unless first)
-  // d_prev = (d W) * gelu'(pre_prev)
+  // one layer back, with dT = d^T already in M->dT: grad W = dT aT^T, grad b
+  // = row sums of dT, and (unless first) d_prev = (d W) * gelu'(pre_prev),
+  // whose epilogue also writes d_prev^T into M->dT (every reader of the old
+  // contents is earlier on the stream)
   auto back = [&](const bf16 *d, uint32_t hout, const bf16 *aT, uint32_t hin, float *gW,
                   float *gb, const bf16 *W, const float *pre_prev, bf16 *d_prev) -> int {
-    QCF_TRY(transpose(d, M->dT, B, hout, s));
     GemmArgs g{M->dT, aT, (int)hout, (int)hin, (int)B, (int)B, (int)B, (int)hin,
-               gW, nullptr, nullptr, nullptr};
+               gW, nullptr, nullptr, nullptr, nullptr};
     // few output tiles, long K: split K so that ~512 workgroups run
     const int tiles = (int)((hout / kTile) * (hin / kTile));
     int sk = 1;
     while (sk * tiles < 512 && (int)B % (2 * sk * 256) == 0 && 2 * sk <= kMaxSplitK) sk *= 2;
     QCF_TRY((launch_gemm<true, true, kEpiStoreF32>(g, s, sk, M->kpart)));
-    QCF_TRY(colsum(M, d, nullptr, B, hout, gb, s));
+    hipLaunchKernelGGL(row_sums, dim3(hout), dim3(256), 0, s, M->dT, B, gb);
+    QCF_HIP_CHECK(hipGetLastError());
     if (d_prev) {
       // W^T (hin x hout) so both operands are read K-contiguous
       QCF_TRY(transpose(W, M->WT, hout, hin, s));
       g = GemmArgs{d, M->WT, (int)B, (int)hin, (int)hout, (int)hout, (int)hout, (int)hin,
-                   nullptr, d_prev, nullptr, pre_prev};
+                   nullptr, d_prev, nullptr, pre_prev, M->dT};
       QCF_TRY((launch_gemm<true, true, kEpiDGelu>(g, s)));
     }
     return QCF_OK;
@@ -534,7 +648,9 @@ int gather(qcf_mlp *M, const uint64_t *rec, size_t ld, uint64_t s0, uint64_t nsh
            uint32_t rows, bool train, hipStream_t s) {
   if (s0 % 64) return set_error(QCF_ERR_ARGUMENT, "first shot %llu is not a multiple of 64",
                                 (unsigned long long)s0);
-  hipLaunchKernelGGL(gather_inputs, dim3(rows / 64), dim3(256), 0, s, rec, ld, s0, nshots,
+  // one workgroup per 64 shots and 128 input columns
+  hipLaunchKernelGGL(gather_inputs, dim3(rows / 64, (M->kin + 127) / 128), dim3(128), 0, s, rec,
+                     ld, s0, nshots,
                      M->det_cols, M->ndet, M->kin, M->obs_col, train ? M->y : nullptr,
                      (uint16_t *)M->x, train ? (uint16_t *)M->xT : nullptr, rows);
   QCF_HIP_CHECK(hipGetLastError());
@@ -605,6 +721,7 @@ extern "C" int qcf_mlp_create(const uint32_t *det_bits, uint32_t ndet, uint32_t 
   A(&M->z, R);
   A(&M->dz, R);
   A(&M->lossbuf, 1);
+  A(&M->lpart, (size_t)batch / (4 * kHeadRows));
   A(&M->kpart, (size_t)kMaxSplitK * std::max((size_t)h1 * M->kin, std::max((size_t)h2 * h1, (size_t)h3 * h2)));
   A(&M->part, (size_t)((batch + 127) / 128) * std::max(std::max(h1, h2), std::max(h3, M->kin)));
   if (st == QCF_OK) {
@@ -707,7 +824,7 @@ extern "C" int qcf_mlp_infer(qcf_mlp *m, const uint64_t *rec, size_t ld, uint64_
     const uint32_t rows128 = (rows + 127) / 128 * 128;
     QCF_TRY(gather(m, rec, ld, s0, shots, rows128, false, s));
     QCF_TRY(forward(m, rows128, false, s));
-    hipLaunchKernelGGL(head_forward, dim3(rows128 / 4), dim3(256), 0, s, m->a3,
+    hipLaunchKernelGGL(head_forward, dim3(rows128 / (4 * kHeadRows)), dim3(256), 0, s, m->a3,
                        m->wb + m->off[6], m->w + m->off[7], m->h3, rows, logits + s0,
                        (const float *)nullptr, (float *)nullptr, (float *)nullptr, 0.0f);
     QCF_HIP_CHECK(hipGetLastError());
@@ -718,7 +835,8 @@ extern "C" int qcf_mlp_infer(qcf_mlp *m, const uint64_t *rec, size_t ld, uint64_
 extern "C" int qcf_debug_gemm_bf16(int layout, int M, int N, int K, const void *A, const void *B,
                                    float *C, void *stream) {
   clear_error();
-  GemmArgs g{(const bf16 *)A, (const bf16 *)B, M, N, K, 0, 0, N, C, nullptr, nullptr, nullptr};
+  GemmArgs g{(const bf16 *)A, (const bf16 *)B, M, N, K, 0, 0, N, C, nullptr, nullptr, nullptr,
+             nullptr};
   hipStream_t s = (hipStream_t)stream;
   switch (layout) {
     case 0:  // A[M][K], B[N][K]
