@@ -6,8 +6,10 @@ HIP decoder with it; the product never imports it.
 The network and its numerics restate the decoder kernels: ndet -> h1 -> h2 ->
 h3 -> 1, GELU (erf form) between layers, binary cross-entropy on the logit.
 With `emulate=True` the values the device keeps in BF16 are rounded to BF16
-at the same points (weights, activations after GELU, back-propagated
-pre-activation gradients); sums are float64 here and float32 on the device.
+at the same points (weights, activations after GELU, the GELU derivative at
+each pre-activation that the forward pass keeps for the backward one,
+back-propagated pre-activation gradients); sums are float64 here and float32
+on the device.
 With `emulate=False` everything stays float64, which is what the finite-
 difference check of the gradient formulas uses.
 
@@ -81,15 +83,16 @@ def loss_and_grads(params, X, y, dims, emulate=True):
     sp = np.where(z > 0, z + np.log1p(np.exp(-np.abs(z))), np.log1p(np.exp(z)))
     loss = float(np.mean(sp - y * z))
     dz = (1.0 / (1.0 + np.exp(-z)) - y) / B
-    d3 = r(dz[:, None] * w4[None, :] * dgelu(pre3)).astype(np.float64)
+    dg1, dg2, dg3 = (r(dgelu(p)).astype(np.float64) for p in (pre1, pre2, pre3))
+    d3 = r(dz[:, None] * w4[None, :] * dg3).astype(np.float64)
     gw4 = a3.T @ dz
     gb4 = np.array([dz.sum()])
     gW3 = d3.T @ a2
     gb3 = d3.sum(0)
-    d2 = r((d3 @ W3) * dgelu(pre2)).astype(np.float64)
+    d2 = r((d3 @ W3) * dg2).astype(np.float64)
     gW2 = d2.T @ a1
     gb2 = d2.sum(0)
-    d1 = r((d2 @ W2) * dgelu(pre1)).astype(np.float64)
+    d1 = r((d2 @ W2) * dg1).astype(np.float64)
     gW1 = d1.T @ X
     gb1 = d1.sum(0)
     g = np.concatenate([a.reshape(-1) for a in (gW1, gb1, gW2, gb2, gW3, gb3, gw4, gb4)])

@@ -10,12 +10,14 @@
 //
 // Every dense contraction runs in one hand-written GEMM kernel on the gfx950
 // matrix cores (v_mfma_f32_32x32x16_bf16): 128x128 output tiles per
-// 256-thread workgroup (four waves, 64x64 each as 2x2 MFMA tiles), K in steps
-// of 32 through double-buffered LDS.  Operands are read either K-contiguous
-// or transposed (M- or N-contiguous), which covers the forward pass
-// (X W^T), the input gradient (dY W) and the weight gradient (dY^T X)
-// without transposing anything in memory.  Epilogues fuse bias + GELU
-// (forward) and the GELU derivative (backward).
+// 256-thread workgroup (four waves, 64x64 each as 2x2 MFMA tiles), K in
+// stages of 64 through one LDS buffer with the next stage prefetched into
+// registers, tiles renumbered so that each XCD works on one band of rows.
+// Operands are read K-contiguous (or, in the debug entry, transposed).
+// Epilogues fuse bias + GELU, keeping the GELU derivative for the backward
+// pass (forward), the multiplication by that derivative (backward), and
+// optionally write a transposed copy of their output through LDS: the
+// batch-minor operands of the weight gradients.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -35,26 +37,27 @@ namespace qcf {
 namespace {
 
 constexpr int kTile = 128;  // output tile edge
-constexpr int kTS = kTile + 8;  // LDS row stride of the epilogue's transposed tile
+constexpr int kXcds = 8;    // accelerator dies of an MI355X
 constexpr int kBK = 32;     // K per LDS stage
 constexpr int kMaxSplitK = 16;
 constexpr bool kWideK = true;  // 64-wide K stages when K allows
 
 enum Epi : int {
   kEpiStoreF32 = 0,   // C (fp32) = acc
-  kEpiBiasGelu = 1,   // pre = acc + bias (fp32); act = gelu(pre) (bf16)
-  kEpiDGelu = 2,      // out = acc * gelu'(pre_in) (bf16)
-  kEpiBiasGeluInfer = 3,  // act = gelu(acc + bias) (bf16), no pre-activation kept
+  kEpiBiasGelu = 1,   // p = acc + bias: act = gelu(p), dg = gelu'(p) (both bf16)
+  kEpiDGelu = 2,      // out = acc * dg_in (bf16)
+  kEpiBiasGeluInfer = 3,  // act = gelu(acc + bias) (bf16), no derivative kept
 };
 
 struct GemmArgs {
   const bf16 *A, *B;
   int M, N, K, lda, ldb, ldc;
-  float *Cf;           // kEpiStoreF32 output / kEpiBiasGelu pre-activation
+  float *Cf;           // kEpiStoreF32 output
   bf16 *Cb;            // bf16 output
   const float *bias;   // per column
-  const float *pre_in; // kEpiDGelu
+  const bf16 *dg_in;   // kEpiDGelu: gelu' at the layer's pre-activations
   bf16 *CbT;           // optional transposed copy of Cb: CbT[n * M + m]
+  bf16 *Cg;            // kEpiBiasGelu: gelu'(p), for the backward pass
 };
 
 __device__ __forceinline__ float gelu(float x) {
@@ -66,27 +69,27 @@ __device__ __forceinline__ float dgelu(float x) {
   return cdf + x * pdf;
 }
 
-// One operand tile (128 rows of the output dimension x BK of K) moves from
+// One operand tile (ROWS rows of the output dimension x BK of K) moves from
 // global memory to LDS as [row][k].  K-contiguous operands move 16 bytes per
 // thread-chunk; transposed ones are read along the output dimension and
-// scattered into the [row][k] layout.  BK/16 chunks per thread.
-// BK/16 chunks of 16 bytes per thread, held as one vector of 32-bit words
-// (a plain array of uint4 ends up in scratch memory)
-template <int BK>
-using TileRegs = uint32_t __attribute__((ext_vector_type(BK / 4)));
+// scattered into the [row][k] layout.  ROWS * BK / 2048 chunks of 16 bytes
+// per thread, held as one vector of 32-bit words (a plain array of uint4
+// ends up in scratch memory).
+template <int ROWS, int BK>
+using TileRegs = uint32_t __attribute__((ext_vector_type(ROWS * BK / 512)));
 
-template <bool KMAJOR, int BK>
+template <bool KMAJOR, int ROWS, int BK>
 __device__ __forceinline__ void load_tile(const bf16 *G, int ld, int r0, int k0, int t,
-                                          TileRegs<BK> &regs) {
+                                          TileRegs<ROWS, BK> &regs) {
 #pragma unroll
-  for (int h = 0; h < BK / 16; ++h) {
+  for (int h = 0; h < ROWS * BK / 2048; ++h) {
     const int c = t + h * 256;
     uint4 v;
     if constexpr (KMAJOR) {
       const int row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
       v = *(const uint4 *)(G + (size_t)(r0 + row) * ld + k0 + kc);
     } else {
-      const int k = c >> 4, rc = (c & 15) * 8;
+      const int k = c / (ROWS / 8), rc = (c % (ROWS / 8)) * 8;
       v = *(const uint4 *)(G + (size_t)(k0 + k) * ld + r0 + rc);
     }
     regs[4 * h + 0] = v.x;
@@ -96,10 +99,11 @@ __device__ __forceinline__ void load_tile(const bf16 *G, int ld, int r0, int k0,
   }
 }
 
-template <bool KMAJOR, int BK>
-__device__ __forceinline__ void store_tile(bf16 (*S)[BK + 8], int t, const TileRegs<BK> &regs) {
+template <bool KMAJOR, int ROWS, int BK>
+__device__ __forceinline__ void store_tile(bf16 (*S)[BK + 8], int t,
+                                           const TileRegs<ROWS, BK> &regs) {
 #pragma unroll
-  for (int h = 0; h < BK / 16; ++h) {
+  for (int h = 0; h < ROWS * BK / 2048; ++h) {
     const int c = t + h * 256;
     if constexpr (KMAJOR) {
       const int row = c / (BK / 8), kc = (c % (BK / 8)) * 8;
@@ -107,7 +111,7 @@ __device__ __forceinline__ void store_tile(bf16 (*S)[BK + 8], int t, const TileR
           uint4{regs[4 * h], regs[4 * h + 1], regs[4 * h + 2], regs[4 * h + 3]};
     } else {
       // the 8 values go to 8 rows
-      const int k = c >> 4, rc = (c & 15) * 8;
+      const int k = c / (ROWS / 8), rc = (c % (ROWS / 8)) * 8;
 #pragma unroll
       for (int j = 0; j < 8; ++j)
         S[rc + j][k] =
@@ -117,87 +121,100 @@ __device__ __forceinline__ void store_tile(bf16 (*S)[BK + 8], int t, const TileR
 }
 
 // C[m][n] = sum_k A(m,k) B(k,n) with A(m,k) = A[m*lda+k] (A_KMAJOR) or
-// A[k*lda+m], and B(k,n) = B[n*ldb+k] (B_KMAJOR) or B[k*ldb+n].
-// Requires M, N multiples of 128 and K a multiple of 32 (checked on the host).
-template <bool A_KMAJOR, bool B_KMAJOR, int EPI, int BK, int STAGES>
+// A[k*lda+m], and B(k,n) = B[n*ldb+k] (B_KMAJOR) or B[k*ldb+n].  A TM x TN
+// output tile per 256-thread workgroup: four waves in a 2 x 2 grid, each
+// (TM/2) x (TN/2) as 32 x 32 MFMA tiles.  Requires M, N multiples of TM, TN
+// and K a multiple of BK (checked on the host).
+template <bool A_KMAJOR, bool B_KMAJOR, int EPI, int BK, int STAGES, int TM, int TN>
 __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
   // STAGES == 2: double-buffered LDS, one barrier per K stage;
   // STAGES == 1: one LDS buffer (half the LDS, twice the resident
   // workgroups), two barriers per stage; both prefetch the next stage into
   // registers while the matrix cores work on the current one
+  constexpr int FM = TM / 64, FN = TN / 64;  // 32 x 32 tiles per wave
   // operand stages As, Bs; after the K loop the same LDS holds the output
-  // tile's transpose for the optional batch-minor copy (rows padded to kTS)
-  constexpr int kStage = kTile * (BK + 8);
-  constexpr int kLds = 2 * STAGES * kStage > kTile * kTS ? 2 * STAGES * kStage : kTile * kTS;
+  // tile's transpose for the optional batch-minor copy (rows padded)
+  constexpr int kTSp = TM + 8;
+  constexpr int kA = TM * (BK + 8), kB = TN * (BK + 8);
+  constexpr int kLds = STAGES * (kA + kB) > TN * kTSp ? STAGES * (kA + kB) : TN * kTSp;
   __shared__ __attribute__((aligned(16))) bf16 lds_raw[kLds];
-  bf16(*As)[kTile][BK + 8] = reinterpret_cast<bf16(*)[kTile][BK + 8]>(lds_raw);
-  bf16(*Bs)[kTile][BK + 8] = reinterpret_cast<bf16(*)[kTile][BK + 8]>(lds_raw + STAGES * kStage);
+  bf16(*As)[TM][BK + 8] = reinterpret_cast<bf16(*)[TM][BK + 8]>(lds_raw);
+  bf16(*Bs)[TN][BK + 8] = reinterpret_cast<bf16(*)[TN][BK + 8]>(lds_raw + STAGES * kA);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wm = w >> 1, wn = w & 1;
-  const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+  // Workgroup i runs on XCD i mod 8, each XCD with its own L2.  Tiles are
+  // renumbered so that XCD x takes the x-th contiguous run of row-major tile
+  // indices (whole rows of output tiles where the count allows): its L2 then
+  // holds one band of A and the B tiles, instead of every XCD streaming all
+  // of A.
+  const int ntn = (int)gridDim.x, nt = (int)(gridDim.x * gridDim.y);
+  int lin = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+  if (nt % kXcds == 0) lin = (lin % kXcds) * (nt / kXcds) + lin / kXcds;
+  const int m0 = (lin / ntn) * TM, n0 = (lin % ntn) * TN;
   // split K: workgroup z covers K range [z*K/Z, (z+1)*K/Z) and, for the fp32
   // store, writes its partial sum to slice z of the output
   const int kz = g.K / (int)gridDim.z, kbase = (int)blockIdx.z * kz;
-  f32x16 acc[2][2];
+  f32x16 acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-  TileRegs<BK> ra, rb;
-  load_tile<A_KMAJOR, BK>(g.A, g.lda, m0, kbase, t, ra);
-  load_tile<B_KMAJOR, BK>(g.B, g.ldb, n0, kbase, t, rb);
+  TileRegs<TM, BK> ra;
+  TileRegs<TN, BK> rb;
+  load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, kbase, t, ra);
+  load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, kbase, t, rb);
   if (STAGES == 2) {
-    store_tile<A_KMAJOR, BK>(As[0], t, ra);
-    store_tile<B_KMAJOR, BK>(Bs[0], t, rb);
+    store_tile<A_KMAJOR, TM, BK>(As[0], t, ra);
+    store_tile<B_KMAJOR, TN, BK>(Bs[0], t, rb);
     __syncthreads();
   }
   const int KT = kz / BK;
   for (int kt = 0; kt < KT; ++kt) {
     const int cur = STAGES == 2 ? (kt & 1) : 0;
     if (STAGES == 1) {
-      store_tile<A_KMAJOR, BK>(As[0], t, ra);
-      store_tile<B_KMAJOR, BK>(Bs[0], t, rb);
+      store_tile<A_KMAJOR, TM, BK>(As[0], t, ra);
+      store_tile<B_KMAJOR, TN, BK>(Bs[0], t, rb);
       __syncthreads();
     }
     if (kt + 1 < KT) {
-      load_tile<A_KMAJOR, BK>(g.A, g.lda, m0, kbase + (kt + 1) * BK, t, ra);
-      load_tile<B_KMAJOR, BK>(g.B, g.ldb, n0, kbase + (kt + 1) * BK, t, rb);
+      load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, kbase + (kt + 1) * BK, t, ra);
+      load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, kbase + (kt + 1) * BK, t, rb);
     }
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 16) {
-      bf16x8 a[2], b[2];
+      bf16x8 a[FM], b[FN];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-        a[i] = *(const bf16x8 *)&As[cur][wm * 64 + i * 32 + (lane & 31)][kk + 8 * (lane >> 5)];
+      for (int i = 0; i < FM; ++i)
+        a[i] = *(const bf16x8 *)&As[cur][wm * (TM / 2) + i * 32 + (lane & 31)][kk + 8 * (lane >> 5)];
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-        b[j] = *(const bf16x8 *)&Bs[cur][wn * 64 + j * 32 + (lane & 31)][kk + 8 * (lane >> 5)];
+      for (int j = 0; j < FN; ++j)
+        b[j] = *(const bf16x8 *)&Bs[cur][wn * (TN / 2) + j * 32 + (lane & 31)][kk + 8 * (lane >> 5)];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < FN; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     if (STAGES == 2 && kt + 1 < KT) {
-      store_tile<A_KMAJOR, BK>(As[cur ^ 1], t, ra);
-      store_tile<B_KMAJOR, BK>(Bs[cur ^ 1], t, rb);
+      store_tile<A_KMAJOR, TM, BK>(As[cur ^ 1], t, ra);
+      store_tile<B_KMAJOR, TN, BK>(Bs[cur ^ 1], t, rb);
     }
     __syncthreads();
   }
   // epilogue: lane holds column (lane & 31) of each 32x32 tile, rows
   // 8*(r/4) + 4*(lane/32) + r%4
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int n = n0 + wn * 64 + j * 32 + (lane & 31);
+    for (int j = 0; j < FN; ++j) {
+      const int n = n0 + wn * (TN / 2) + j * 32 + (lane & 31);
       const float bn = (EPI == kEpiBiasGelu || EPI == kEpiBiasGeluInfer) ? g.bias[n] : 0.0f;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         // r = 4q .. 4q+3 are four consecutive rows m
-        const int mq = m0 + wm * 64 + i * 32 + 8 * q + 4 * (lane >> 5);
+        const int mq = m0 + wm * (TM / 2) + i * 32 + 8 * q + 4 * (lane >> 5);
         uint16_t h[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -209,12 +226,12 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
             g.Cf[o + (size_t)blockIdx.z * g.M * g.ldc] = v;
           } else if (EPI == kEpiBiasGelu) {
             const float p = v + bn;
-            g.Cf[o] = p;
             b = (bf16)gelu(p);
+            g.Cg[o] = (bf16)dgelu(p);
           } else if (EPI == kEpiBiasGeluInfer) {
             b = (bf16)gelu(v + bn);
           } else {
-            b = (bf16)(v * dgelu(g.pre_in[o]));
+            b = (bf16)(v * (float)g.dg_in[o]);
           }
           if (EPI != kEpiStoreF32) g.Cb[o] = b;
           h[u] = __builtin_bit_cast(uint16_t, b);
@@ -222,20 +239,20 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
         if (EPI != kEpiStoreF32 && g.CbT) {
           // the tile's transpose goes through LDS: T[n - n0][m - m0]
           uint16_t *T = (uint16_t *)lds_raw;
-          *(uint2 *)(T + (size_t)(n - n0) * kTS + (mq - m0)) =
+          *(uint2 *)(T + (size_t)(n - n0) * kTSp + (mq - m0)) =
               uint2{h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16)};
         }
       }
     }
   if (EPI != kEpiStoreF32 && g.CbT) {
-    // 128 rows n of 128 consecutive m: 16 bytes per thread, 256 per row
+    // TN rows n of TM consecutive m: 16 bytes per thread
     __syncthreads();
     const uint16_t *T = (const uint16_t *)lds_raw;
 #pragma unroll
-    for (int h = 0; h < kTile * kTile / 8 / 256; ++h) {
-      const int c = t + h * 256, row = c >> 4, col = (c & 15) * 8;
+    for (int h = 0; h < TM * TN / 8 / 256; ++h) {
+      const int c = t + h * 256, row = c / (TM / 8), col = (c % (TM / 8)) * 8;
       *(uint4 *)(g.CbT + (size_t)(n0 + row) * g.M + m0 + col) =
-          *(const uint4 *)(T + (size_t)row * kTS + col);
+          *(const uint4 *)(T + (size_t)row * kTSp + col);
     }
   }
 }
@@ -259,6 +276,19 @@ int gemm_stages() {
   return v;
 }
 
+template <bool AKM, bool BKM, int EPI, int TM, int TN>
+void launch_tiled(const GemmArgs &gp, bool wide, bool one, int splitk, hipStream_t s) {
+  const dim3 grid(gp.N / TN, gp.M / TM, splitk);
+  if (wide && one)
+    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 64, 1, TM, TN>), grid, dim3(256), 0, s, gp);
+  else if (wide)
+    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 64, 2, TM, TN>), grid, dim3(256), 0, s, gp);
+  else if (one)
+    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 32, 1, TM, TN>), grid, dim3(256), 0, s, gp);
+  else
+    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 32, 2, TM, TN>), grid, dim3(256), 0, s, gp);
+}
+
 // `splitk` > 1 (fp32 store only) runs that many K slices into `part`
 // (splitk * M * ldc floats) and sums them into C.
 template <bool AKM, bool BKM, int EPI>
@@ -272,16 +302,11 @@ int launch_gemm(const GemmArgs &g, hipStream_t s, int splitk = 1, float *part = 
   const bool wide = kWideK && (g.K % (splitk * 64)) == 0;
   GemmArgs gp = g;
   if (splitk > 1) gp.Cf = part;
-  const dim3 grid(g.N / kTile, g.M / kTile, splitk);
   const bool one = gemm_stages() == 1;
-  if (wide && one)
-    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 64, 1>), grid, dim3(256), 0, s, gp);
-  else if (wide)
-    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 64, 2>), grid, dim3(256), 0, s, gp);
-  else if (one)
-    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 32, 1>), grid, dim3(256), 0, s, gp);
-  else
-    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 32, 2>), grid, dim3(256), 0, s, gp);
+  // 128 x 128 tiles: 128 x 256, 256 x 128, 128 x 64 and 64 x 128 measured
+  // slower or no faster on the decoder's shapes (M = 8192, N = K = 1024:
+  // 577, 539, 574, 547 against 606 TFLOP/s), as did 128-wide K stages (547)
+  launch_tiled<AKM, BKM, EPI, kTile, kTile>(gp, wide, one, splitk, s);
   if (splitk > 1) {
     const size_t n = (size_t)g.M * g.ldc;
     hipLaunchKernelGGL(sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part,
@@ -294,33 +319,46 @@ int launch_gemm(const GemmArgs &g, hipStream_t s, int splitk = 1, float *part = 
 // Syndromes of `rows` shots starting at shot `s0` (a multiple of 64) of a
 // packed record: x[m][j] = detector j of shot m (0 past the detectors and past
 // the shots), optionally the transpose xT[j][m], and y[m] = observable.  One
-// workgroup per 64 shots and gridDim.y-th of the columns: each detector word
-// is read once.
+// workgroup per 64 shots and 128 columns: the 128 detector words go to LDS
+// once, then every thread writes 16-byte runs of x (8 columns of one shot)
+// and of xT (8 shots of one column).
 __global__ void gather_inputs(const uint64_t *rec, size_t ld, uint64_t s0, uint64_t nshots,
                               const uint32_t *det_cols, uint32_t ndet, uint32_t kin,
                               uint32_t obs_col, float *y, uint16_t *x, uint16_t *xT,
                               uint32_t rows) {
-  const uint32_t m0 = blockIdx.x * 64;
+  __shared__ uint64_t wd[128];
+  const uint32_t m0 = blockIdx.x * 64, j0 = blockIdx.y * 128;
   const uint64_t first = s0 + m0, wi = first >> 6;
   const uint64_t left = nshots > first ? nshots - first : 0;
   const uint64_t valid = left >= 64 ? ~0ull : ((1ull << left) - 1ull);
-  constexpr uint16_t kOne = 0x3F80;  // bf16 1.0
-  for (uint32_t j = blockIdx.y * blockDim.x + threadIdx.x; j < kin; j += gridDim.y * blockDim.x) {
+  constexpr uint32_t kOne = 0x3F80;  // bf16 1.0
+  if (threadIdx.x < 128) {
+    const uint32_t j = j0 + threadIdx.x;
     uint64_t w = 0;
     if (j < ndet && valid) w = rec[(size_t)det_cols[j] * ld + wi] & valid;
-    for (uint32_t b = 0; b < 64; ++b) x[(size_t)(m0 + b) * kin + j] = ((w >> b) & 1u) ? kOne : 0;
-    if (xT) {
-      uint4 *dst = (uint4 *)(xT + (size_t)j * rows + m0);
-      for (uint32_t q = 0; q < 8; ++q) {
-        uint32_t v[4];
-        for (uint32_t h = 0; h < 4; ++h) {
-          const uint32_t b = q * 8 + h * 2;
-          v[h] = (((w >> b) & 1u) ? kOne : 0u) | ((((w >> (b + 1)) & 1u) ? kOne : 0u) << 16);
-        }
-        dst[q] = uint4{v[0], v[1], v[2], v[3]};
-      }
-    }
+    wd[threadIdx.x] = w;
   }
+  __syncthreads();
+  const uint32_t ncol = min(128u, kin - j0);  // kin is a multiple of 128
+  for (uint32_t c = threadIdx.x; c < 64 * (ncol / 8); c += blockDim.x) {
+    const uint32_t b = c / (ncol / 8), jc = (c % (ncol / 8)) * 8;
+    uint32_t v[4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+      v[h] = (((wd[jc + 2 * h] >> b) & 1u) ? kOne : 0u) |
+             ((((wd[jc + 2 * h + 1] >> b) & 1u) ? kOne : 0u) << 16);
+    *(uint4 *)(x + (size_t)(m0 + b) * kin + j0 + jc) = uint4{v[0], v[1], v[2], v[3]};
+  }
+  if (xT)
+    for (uint32_t c = threadIdx.x; c < ncol * 8; c += blockDim.x) {
+      const uint32_t j = c >> 3, b0 = (c & 7) * 8;
+      const uint64_t w = wd[j] >> b0;
+      uint32_t v[4];
+#pragma unroll
+      for (int h = 0; h < 4; ++h)
+        v[h] = (((w >> (2 * h)) & 1u) ? kOne : 0u) | ((((w >> (2 * h + 1)) & 1u) ? kOne : 0u) << 16);
+      *(uint4 *)(xT + (size_t)(j0 + j) * rows + m0 + b0) = uint4{v[0], v[1], v[2], v[3]};
+    }
   if (y && blockIdx.y == 0 && threadIdx.x < 64) {
     const uint64_t o = valid ? rec[(size_t)obs_col * ld + wi] & valid : 0;
     y[m0 + threadIdx.x] = (float)((o >> threadIdx.x) & 1u);
@@ -347,7 +385,7 @@ __device__ __forceinline__ float wave_sum(float v) {
 // kHeadRows rows.  With labels: dz = (sigmoid(z) - y) / rows, and the block's
 // share of mean(softplus(z) - y z) goes to lpart[block] (summed by
 // head_sums in a fixed order: one atomic per row on one address serialises).
-constexpr uint32_t kHeadRows = 8;
+constexpr uint32_t kHeadRows = 2;
 __global__ void head_forward(const bf16 *a3, const bf16 *w4, const float *b4, uint32_t h3,
                              uint32_t rows, float *z, const float *y, float *dz, float *lpart,
                              float inv_rows) {
@@ -387,9 +425,10 @@ __global__ void head_forward(const bf16 *a3, const bf16 *w4, const float *b4, ui
   if (threadIdx.x == 0) lpart[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
-// d3[m][k] = dz[m] * w4[k] * gelu'(pre3[m][k]), written row-major and as the
-// batch-minor copy d3T[k][m], one 64 x 64 tile per workgroup through LDS.
-__global__ void head_backward(const float *dz, const bf16 *w4, const float *pre3, uint32_t h3,
+// d3[m][k] = dz[m] * w4[k] * gelu'(pre3[m][k]) (dg3 holds the derivative),
+// written row-major and as the batch-minor copy d3T[k][m], one 64 x 64 tile
+// per workgroup through LDS.
+__global__ void head_backward(const float *dz, const bf16 *w4, const bf16 *dg3, uint32_t h3,
                               uint32_t rows, bf16 *d3, bf16 *d3T) {
   __shared__ bf16 t[64][66];
   const uint32_t r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
@@ -397,7 +436,7 @@ __global__ void head_backward(const float *dz, const bf16 *w4, const float *pre3
   const float wk = (float)w4[c0 + tx];
   for (uint32_t i = ty; i < 64; i += 4) {
     const size_t o = (size_t)(r0 + i) * h3 + c0 + tx;
-    const bf16 v = (bf16)(dz[r0 + i] * wk * dgelu(pre3[o]));
+    const bf16 v = (bf16)(dz[r0 + i] * wk * (float)dg3[o]);
     d3[o] = v;
     t[i][tx] = v;
   }
@@ -529,7 +568,7 @@ struct qcf_mlp {
   bf16 *d1 = nullptr, *d2 = nullptr, *d3 = nullptr;
   bf16 *xT = nullptr, *a1T = nullptr, *a2T = nullptr, *dT = nullptr;  // batch-minor copies
   bf16 *WT = nullptr;  // transposed weights for the input gradients
-  float *pre1 = nullptr, *pre2 = nullptr, *pre3 = nullptr;
+  bf16 *dg1 = nullptr, *dg2 = nullptr, *dg3 = nullptr;  // gelu' at the pre-activations
   float *y = nullptr, *z = nullptr, *dz = nullptr, *part = nullptr, *lossbuf = nullptr;
   float *lpart = nullptr;  // per-workgroup loss partials of the head
   float *kpart = nullptr;  // split-K partial sums of the weight gradients
@@ -562,15 +601,15 @@ int forward(qcf_mlp *M, uint32_t rows, bool train, hipStream_t s) {
   const float *b1 = M->w + M->off[1], *b2 = M->w + M->off[3], *b3 = M->w + M->off[5];
   // training also keeps the batch-minor copies of a1 and a2 that the weight
   // gradients contract (written by the epilogue)
-  auto layer = [&](const bf16 *X, int kin, const bf16 *W, const float *b, int nout, float *pre,
+  auto layer = [&](const bf16 *X, int kin, const bf16 *W, const float *b, int nout, bf16 *dg,
                    bf16 *act, bf16 *actT) {
-    g = GemmArgs{X, W, (int)rows, nout, kin, kin, kin, nout, pre, act, b, nullptr, actT};
+    g = GemmArgs{X, W, (int)rows, nout, kin, kin, kin, nout, nullptr, act, b, nullptr, actT, dg};
     return train ? launch_gemm<true, true, kEpiBiasGelu>(g, s)
                  : launch_gemm<true, true, kEpiBiasGeluInfer>(g, s);
   };
-  QCF_TRY(layer(M->x, M->kin, W1, b1, M->h1, M->pre1, M->a1, train ? M->a1T : nullptr));
-  QCF_TRY(layer(M->a1, M->h1, W2, b2, M->h2, M->pre2, M->a2, train ? M->a2T : nullptr));
-  QCF_TRY(layer(M->a2, M->h2, W3, b3, M->h3, M->pre3, M->a3, nullptr));
+  QCF_TRY(layer(M->x, M->kin, W1, b1, M->h1, M->dg1, M->a1, train ? M->a1T : nullptr));
+  QCF_TRY(layer(M->a1, M->h1, W2, b2, M->h2, M->dg2, M->a2, train ? M->a2T : nullptr));
+  QCF_TRY(layer(M->a2, M->h2, W3, b3, M->h3, M->dg3, M->a3, nullptr));
   return QCF_OK;
 }
 
@@ -605,7 +644,7 @@ int fwd_bwd(qcf_mlp *M, float *loss, hipStream_t s) {
   hipLaunchKernelGGL(head_forward, dim3(nhead), dim3(256), 0, s, M->a3, w4, M->w + M->off[7],
                      M->h3, B, M->z, M->y, M->dz, M->lpart, 1.0f / (float)B);
   hipLaunchKernelGGL(head_backward, dim3(M->h3 / 64, B / 64), dim3(256), 0, s, M->dz, w4,
-                     M->pre3, M->h3, B, M->d3, M->dT);
+                     M->dg3, M->h3, B, M->d3, M->dT);
   QCF_HIP_CHECK(hipGetLastError());
   float *G = M->g;
   // output layer: grad w4 = a3^T dz, grad b4 = sum dz (and the loss)
@@ -614,13 +653,13 @@ int fwd_bwd(qcf_mlp *M, float *loss, hipStream_t s) {
                      G + M->off[7], loss);
   const bf16 *W2 = M->wb + M->off[2], *W3 = M->wb + M->off[4];
   // one layer back, with dT = d^T already in M->dT: grad W = dT aT^T, grad b
-  // = row sums of dT, and (unless first) d_prev = (d W) * gelu'(pre_prev),
+  // = row sums of dT, and (unless first) d_prev = (d W) * dg_prev,
   // whose epilogue also writes d_prev^T into M->dT (every reader of the old
   // contents is earlier on the stream)
   auto back = [&](const bf16 *d, uint32_t hout, const bf16 *aT, uint32_t hin, float *gW,
-                  float *gb, const bf16 *W, const float *pre_prev, bf16 *d_prev) -> int {
+                  float *gb, const bf16 *W, const bf16 *dg_prev, bf16 *d_prev) -> int {
     GemmArgs g{M->dT, aT, (int)hout, (int)hin, (int)B, (int)B, (int)B, (int)hin,
-               gW, nullptr, nullptr, nullptr, nullptr};
+               gW, nullptr, nullptr, nullptr, nullptr, nullptr};
     // few output tiles, long K: split K so that ~512 workgroups run
     const int tiles = (int)((hout / kTile) * (hin / kTile));
     int sk = 1;
@@ -632,13 +671,13 @@ int fwd_bwd(qcf_mlp *M, float *loss, hipStream_t s) {
       // W^T (hin x hout) so both operands are read K-contiguous
       QCF_TRY(transpose(W, M->WT, hout, hin, s));
       g = GemmArgs{d, M->WT, (int)B, (int)hin, (int)hout, (int)hout, (int)hout, (int)hin,
-                   nullptr, d_prev, nullptr, pre_prev, M->dT};
+                   nullptr, d_prev, nullptr, dg_prev, M->dT, nullptr};
       QCF_TRY((launch_gemm<true, true, kEpiDGelu>(g, s)));
     }
     return QCF_OK;
   };
-  QCF_TRY(back(M->d3, M->h3, M->a2T, M->h2, G + M->off[4], G + M->off[5], W3, M->pre2, M->d2));
-  QCF_TRY(back(M->d2, M->h2, M->a1T, M->h1, G + M->off[2], G + M->off[3], W2, M->pre1, M->d1));
+  QCF_TRY(back(M->d3, M->h3, M->a2T, M->h2, G + M->off[4], G + M->off[5], W3, M->dg2, M->d2));
+  QCF_TRY(back(M->d2, M->h2, M->a1T, M->h1, G + M->off[2], G + M->off[3], W2, M->dg1, M->d1));
   QCF_TRY(back(M->d1, M->h1, M->xT, M->kin, G + M->off[0], G + M->off[1], nullptr, nullptr,
                nullptr));
   return QCF_OK;
@@ -649,7 +688,7 @@ int gather(qcf_mlp *M, const uint64_t *rec, size_t ld, uint64_t s0, uint64_t nsh
   if (s0 % 64) return set_error(QCF_ERR_ARGUMENT, "first shot %llu is not a multiple of 64",
                                 (unsigned long long)s0);
   // one workgroup per 64 shots and 128 input columns
-  hipLaunchKernelGGL(gather_inputs, dim3(rows / 64, (M->kin + 127) / 128), dim3(128), 0, s, rec,
+  hipLaunchKernelGGL(gather_inputs, dim3(rows / 64, M->kin / 128), dim3(256), 0, s, rec,
                      ld, s0, nshots,
                      M->det_cols, M->ndet, M->kin, M->obs_col, train ? M->y : nullptr,
                      (uint16_t *)M->x, train ? (uint16_t *)M->xT : nullptr, rows);
@@ -714,9 +753,9 @@ extern "C" int qcf_mlp_create(const uint32_t *det_bits, uint32_t ndet, uint32_t 
   A(&M->WT, std::max((size_t)h2 * h1, (size_t)h3 * h2));
   A(&M->a2T, (size_t)batch * h2);
   A(&M->dT, (size_t)batch * std::max(std::max(h1, h2), h3));
-  A(&M->pre1, (size_t)batch * h1);
-  A(&M->pre2, (size_t)batch * h2);
-  A(&M->pre3, (size_t)batch * h3);
+  A(&M->dg1, (size_t)batch * h1);
+  A(&M->dg2, (size_t)batch * h2);
+  A(&M->dg3, (size_t)batch * h3);
   A(&M->y, R);
   A(&M->z, R);
   A(&M->dz, R);
@@ -836,7 +875,7 @@ extern "C" int qcf_debug_gemm_bf16(int layout, int M, int N, int K, const void *
                                    float *C, void *stream) {
   clear_error();
   GemmArgs g{(const bf16 *)A, (const bf16 *)B, M, N, K, 0, 0, N, C, nullptr, nullptr, nullptr,
-             nullptr};
+             nullptr, nullptr};
   hipStream_t s = (hipStream_t)stream;
   switch (layout) {
     case 0:  // A[M][K], B[N][K]
