@@ -390,6 +390,12 @@ int qcf_debug_jit_compile(const qcf_program *prog, int variant,
 int qcf_debug_wave_scan(const uint32_t *in, uint32_t *out, size_t nwaves,
                         void *stream);
 
+/* The host-side check that precedes every kernel launch of the library, run
+ * for `threads` threads per block against the persistent batch kernel (its
+ * __launch_bounds__ is 64), without launching it: QCF_OK, or
+ * QCF_ERR_ARGUMENT naming the kernel (self-check). */
+int qcf_debug_check_launch(unsigned threads);
+
 /* ---- reference-layout converters (to_gpu / to_cpu counterparts,
  *      ext/QuantumCliffordGPUExt/adapters.jl:7-41) --------------------- */
 

@@ -45,31 +45,48 @@ def _embed_device_header() -> None:
             f.write(out)
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
-    cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE_FLAGS, f"-I{GEN}", "-c", src, "-o", obj]
+# Host-only sanitizer build (build(sanitize=True)): AddressSanitizer and
+# UndefinedBehaviorSanitizer on the host code of the C++ sources (the
+# circuit compiler, the matching decoder, the kernel generator, the runtime),
+# each -fsanitize= after -Xarch_host so that no device code is built with it.
+SAN_OUT = os.path.join(HERE, "qcf", "libqcf_san.so")
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-fno-sanitize-recover=all",
+             "-g"]
+
+
+def _compile(src: str, sanitize: bool = False) -> str:
+    san = sanitize and src.endswith(".cpp")
+    obj = os.path.join(BUILD, os.path.basename(src) + (".san.o" if san else ".o"))
+    cmd = [HIPCC, "-x", "hip", *COMMON, *DEVICE_FLAGS, *(SAN_FLAGS if san else []), f"-I{GEN}",
+           "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr}")
     return obj
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, sanitize: bool = False) -> str:
+    """libqcf.so, or with `sanitize` qcf/libqcf_san.so: the same library with
+    the host C++ sanitized, loaded instead of libqcf.so when QCF_LIBRARY
+    names it and the sanitizer runtime is preloaded (CPU test runs only)."""
     os.makedirs(BUILD, exist_ok=True)
     _embed_device_header()
     srcs = _sources()
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = OUT + ".tmp"
+        objs = list(ex.map(lambda s: _compile(s, sanitize), srcs))
+    out = SAN_OUT if sanitize else OUT
+    tmp = out + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+           *(SAN_FLAGS + ["-Xarch_host", "-shared-libsan"] if sanitize else []),
            "-L/opt/rocm/lib", "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, OUT)
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {OUT}")
-    return OUT
+        print(f"built {out}")
+    return out
 
 
 def build_oracle(verbose: bool = False) -> str:
@@ -86,6 +103,9 @@ def build_oracle(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
+    if "--sanitize" in sys.argv:
+        build(verbose=True, sanitize=True)
+        sys.exit(0)
     build(verbose=True)
     if "--no-oracle" not in sys.argv:
         build_oracle(verbose=True)

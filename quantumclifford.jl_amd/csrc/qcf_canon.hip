@@ -410,7 +410,7 @@ int launch_factor(const uint64_t *xzs, size_t ld, uint32_t nrows, uint32_t c, ui
   if (lds > 64u * 1024u)
     QCF_HIP_CHECK(hipFuncSetAttribute((const void *)canon_factor<NT, J>,
                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL((canon_factor<NT, J>), dim3(1), dim3(NT), lds, stream, xzs, ld, nrows, c, nb,
+  QCF_LAUNCH((canon_factor<NT, J>), dim3(1), dim3(NT), lds, stream, xzs, ld, nrows, c, nb,
                      inv, mask, piv, st, cnt, mark_rx);
   return QCF_OK;
 }
@@ -448,11 +448,12 @@ int dispatch_factor(uint32_t J, const uint64_t *xzs, size_t ld, uint32_t n, uint
 }
 
 template <int G>
-void launch_apply(uint64_t *xzs, uint8_t *phases, size_t ld, size_t W, uint32_t nrows,
-                  const uint64_t *mask, const uint64_t *snap, const CanonState *st,
-                  int phases_on, uint32_t grid, hipStream_t stream) {
-  hipLaunchKernelGGL(canon_apply<G>, dim3(grid), dim3(256), 0, stream, xzs, phases, ld, W, nrows,
-                     mask, snap, st, phases_on);
+int launch_apply(uint64_t *xzs, uint8_t *phases, size_t ld, size_t W, uint32_t nrows,
+                 const uint64_t *mask, const uint64_t *snap, const CanonState *st,
+                 int phases_on, uint32_t grid, hipStream_t stream) {
+  QCF_LAUNCH(canon_apply<G>, dim3(grid), dim3(256), 0, stream, xzs, phases, ld, W, nrows,
+             mask, snap, st, phases_on);
+  return QCF_OK;
 }
 
 }  // namespace
@@ -528,7 +529,7 @@ int canon_run(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nwords, size_t n
   uint64_t *tmp = (uint64_t *)(base + Lo.o_tmp);
   uint8_t *tph = (uint8_t *)(base + Lo.o_tph);
   const uint32_t n = (uint32_t)nrows;
-  hipLaunchKernelGGL(canon_init, dim3((n + 255) / 256), dim3(256), 0, stream, inv, n, st);
+  QCF_LAUNCH(canon_init, dim3((n + 255) / 256), dim3(256), 0, stream, inv, n, st);
   int dev = 0, cus = 0;
   QCF_HIP_CHECK(hipGetDevice(&dev));
   QCF_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -553,30 +554,30 @@ int canon_run(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nwords, size_t n
         QCF_TRY(dispatch_factor<512>(J, xzs, ld, n, c, nb, inv, mask, piv, st, cnt, mark, stream));
       else
         QCF_TRY(dispatch_factor<1024>(J, xzs, ld, n, c, nb, inv, mask, piv, st, cnt, mark, stream));
-      hipLaunchKernelGGL(canon_snap, dim3(snap_grid), dim3(64 * kSnapWaves), kSnapLds, stream,
+      QCF_LAUNCH(canon_snap, dim3(snap_grid), dim3(64 * kSnapWaves), kSnapLds, stream,
                          xzs, phases, ld, W, mask, piv, st, snap, cnt, phases_on);
       switch (G) {
         case 64:
-          launch_apply<64>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream);
+          QCF_TRY(launch_apply<64>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream));
           break;
         case 32:
-          launch_apply<32>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream);
+          QCF_TRY(launch_apply<32>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream));
           break;
         case 16:
-          launch_apply<16>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream);
+          QCF_TRY(launch_apply<16>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream));
           break;
         default:
-          launch_apply<8>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream);
+          QCF_TRY(launch_apply<8>(xzs, phases, ld, W, n, mask, snap, st, phases_on, apply_grid, stream));
       }
     }
   }
   const size_t total = nrows * 2 * W;
   const uint32_t pgrid = (uint32_t)std::min<size_t>((total + 255) / 256, (size_t)cus * 16);
-  hipLaunchKernelGGL(canon_gather, dim3(pgrid), dim3(256), 0, stream, xzs, phases, ld, W, n, inv,
+  QCF_LAUNCH(canon_gather, dim3(pgrid), dim3(256), 0, stream, xzs, phases, ld, W, n, inv,
                      tmp, phases_on ? tph : nullptr);
-  hipLaunchKernelGGL(canon_scatter_back, dim3(pgrid), dim3(256), 0, stream, xzs, phases, ld, W, n,
+  QCF_LAUNCH(canon_scatter_back, dim3(pgrid), dim3(256), 0, stream, xzs, phases, ld, W, n,
                      tmp, phases_on ? tph : nullptr);
-  if (ranks) hipLaunchKernelGGL(canon_ranks, dim3(1), dim3(64), 0, stream, st, ranks);
+  if (ranks) QCF_LAUNCH(canon_ranks, dim3(1), dim3(64), 0, stream, st, ranks);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
 }
@@ -618,7 +619,7 @@ extern "C" int qcf_canonicalize(uint64_t *xzs, uint8_t *phases, size_t ld, size_
   hipStream_t stream = (hipStream_t)stream_;
   if (nrows == 0) {
     if (ranks) {
-      hipLaunchKernelGGL(canon_ranks, dim3(1), dim3(64), 0, stream, nullptr, ranks);
+      QCF_LAUNCH(canon_ranks, dim3(1), dim3(64), 0, stream, nullptr, ranks);
       QCF_HIP_CHECK(hipGetLastError());
     }
     return QCF_OK;
@@ -637,7 +638,7 @@ extern "C" int qcf_canonicalize_noalloc(uint64_t *xzs, uint8_t *phases, size_t l
   hipStream_t stream = (hipStream_t)stream_;
   if (nrows == 0) {
     if (ranks) {
-      hipLaunchKernelGGL(canon_ranks, dim3(1), dim3(64), 0, stream, nullptr, ranks);
+      QCF_LAUNCH(canon_ranks, dim3(1), dim3(64), 0, stream, nullptr, ranks);
       QCF_HIP_CHECK(hipGetLastError());
     }
     return QCF_OK;

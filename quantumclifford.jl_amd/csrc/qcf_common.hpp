@@ -31,6 +31,25 @@ void clear_error();
     if (qcf_s_ != QCF_OK) return qcf_s_;                                     \
   } while (0)
 
+// ------------------------------------------------------------- launches --
+// Every kernel launch goes through QCF_LAUNCH, which first checks on the host
+// that the block size is within the kernel's __launch_bounds__ (the
+// maxThreadsPerBlock of its compiled code, looked up once per kernel): a
+// larger block fails to launch, and a kernel whose indexing assumes its
+// bound would run out of range if it did.  check_block returns
+// QCF_ERR_ARGUMENT naming the kernel.
+int check_block(const void *kernel, unsigned threads, const char *name);
+// The same for a kernel loaded from a code object (hipModuleLaunchKernel).
+int check_block_module(hipFunction_t kernel, unsigned threads, const char *name);
+
+#define QCF_LAUNCH(kernel, grid, block, lds, stream, ...)                              \
+  do {                                                                                \
+    const dim3 qcf_b_ = (block);                                                      \
+    QCF_TRY(::qcf::check_block((const void *)(kernel), qcf_b_.x * qcf_b_.y * qcf_b_.z, \
+                               #kernel));                                             \
+    hipLaunchKernelGGL(kernel, grid, qcf_b_, lds, stream, __VA_ARGS__);               \
+  } while (0)
+
 // Per-device scratch used by reductions that span several workgroups: a ring
 // of (sum, done) counter pairs that the last workgroup of a launch resets to
 // zero after reading, so no clearing launch is needed between calls.

@@ -483,10 +483,10 @@ int allow_lds(const void *fn, uint32_t lds_bytes) {
 template <int MODE>
 int launch_pf(const KParams &kp, uint32_t lds_bytes, uint32_t grid, hipStream_t s) {
   if (kp.scratch) {
-    hipLaunchKernelGGL((pf_kernel<MODE, kPlAtomic>), dim3(grid), dim3(64), 0, s, kp);
+    QCF_LAUNCH((pf_kernel<MODE, kPlAtomic>), dim3(grid), dim3(64), 0, s, kp);
   } else {
     QCF_TRY(allow_lds((const void *)pf_kernel<MODE, kPlLds>, lds_bytes));
-    hipLaunchKernelGGL((pf_kernel<MODE, kPlLds>), dim3(grid), dim3(64), lds_bytes, s, kp);
+    QCF_LAUNCH((pf_kernel<MODE, kPlLds>), dim3(grid), dim3(64), lds_bytes, s, kp);
   }
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
@@ -725,7 +725,7 @@ extern "C" int qcf_debug_philox(const uint32_t *in, uint32_t *out, size_t n,
                                 void *stream) {
   clear_error();
   if (n == 0) return QCF_OK;
-  hipLaunchKernelGGL(philox_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+  QCF_LAUNCH(philox_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, in, out, n);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
@@ -735,10 +735,15 @@ extern "C" int qcf_debug_wave_scan(const uint32_t *in, uint32_t *out, size_t nwa
                                    void *stream) {
   clear_error();
   if (nwaves == 0) return QCF_OK;
-  hipLaunchKernelGGL(scan_kernel, dim3((unsigned)nwaves), dim3(64), 0, (hipStream_t)stream, in,
+  QCF_LAUNCH(scan_kernel, dim3((unsigned)nwaves), dim3(64), 0, (hipStream_t)stream, in,
                      out);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
+}
+
+extern "C" int qcf_debug_check_launch(unsigned threads) {
+  clear_error();
+  return check_block((const void *)pf_batch_kernel, threads, "pf_batch_kernel");
 }
 
 extern "C" int qcf_record_to_bool(const uint64_t *meas, size_t ld, uint64_t shots,
@@ -747,7 +752,7 @@ extern "C" int qcf_record_to_bool(const uint64_t *meas, size_t ld, uint64_t shot
   if (shots == 0 || nbits == 0) return QCF_OK;
   if (ld < (shots + 63) / 64) return set_error(QCF_ERR_DIMENSION, "record stride too short");
   dim3 grid((unsigned)((shots + 255) / 256), nbits);
-  hipLaunchKernelGGL(record_to_bool_kernel, grid, dim3(256), 0, (hipStream_t)stream, meas, ld,
+  QCF_LAUNCH(record_to_bool_kernel, grid, dim3(256), 0, (hipStream_t)stream, meas, ld,
                      shots, nbits, out);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
@@ -759,7 +764,7 @@ extern "C" int qcf_record_from_bool(const uint8_t *in, uint64_t shots, uint32_t 
   if (shots == 0 || nbits == 0) return QCF_OK;
   if (ld < (shots + 63) / 64) return set_error(QCF_ERR_DIMENSION, "record stride too short");
   dim3 grid((unsigned)((ld + 255) / 256), nbits);
-  hipLaunchKernelGGL(record_from_bool_kernel, grid, dim3(256), 0, (hipStream_t)stream, in,
+  QCF_LAUNCH(record_from_bool_kernel, grid, dim3(256), 0, (hipStream_t)stream, in,
                      shots, nbits, meas, ld);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
@@ -772,7 +777,7 @@ extern "C" int qcf_frame_to_reference(const uint64_t *frame, size_t ld, uint64_t
   if (ld < (shots + 63) / 64) return set_error(QCF_ERR_DIMENSION, "frame stride too short");
   const uint32_t W = (nq + 63) / 64;
   dim3 grid((unsigned)((shots + 255) / 256), 2 * W);
-  hipLaunchKernelGGL(frame_to_reference_kernel, grid, dim3(256), 0, (hipStream_t)stream,
+  QCF_LAUNCH(frame_to_reference_kernel, grid, dim3(256), 0, (hipStream_t)stream,
                      frame, ld, shots, nq, xzs);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
@@ -784,7 +789,7 @@ extern "C" int qcf_frame_from_reference(const uint64_t *xzs, uint64_t shots, uin
   if (shots == 0 || nq == 0) return QCF_OK;
   if (ld < (shots + 63) / 64) return set_error(QCF_ERR_DIMENSION, "frame stride too short");
   dim3 grid((unsigned)((ld + 255) / 256), 2 * nq);
-  hipLaunchKernelGGL(frame_from_reference_kernel, grid, dim3(256), 0, (hipStream_t)stream,
+  QCF_LAUNCH(frame_from_reference_kernel, grid, dim3(256), 0, (hipStream_t)stream,
                      xzs, shots, nq, frame, ld);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
@@ -1131,7 +1136,7 @@ int batch_launch(qcf_batch *B, hipStream_t s) {
     const BatchArgs A{B->d_params, B->d_items,        (uint32_t)B->items.size(), B->nshare,
                       B->d_queue,  B->d_shared,       B->d_shared_first,         total,
                       2u + B->nshare};
-    hipLaunchKernelGGL(pf_batch_kernel, dim3(B->grid), dim3(64), B->lds_bytes, s, A);
+    QCF_LAUNCH(pf_batch_kernel, dim3(B->grid), dim3(64), B->lds_bytes, s, A);
     QCF_HIP_CHECK(hipGetLastError());
     return QCF_OK;
   };

@@ -1260,6 +1260,7 @@ int jit_launch(void *function, const JitParams &kp, uint32_t grid, uint32_t lds_
                hipStream_t stream) {
   JitParams p = kp;
   void *args[] = {&p};
+  QCF_TRY(check_block_module((hipFunction_t)function, 64, "qcf_jit_kernel"));
   const hipError_t e = hipModuleLaunchKernel((hipFunction_t)function, grid, 1, 1, 64, 1, 1,
                                              lds_bytes, stream, args, nullptr);
   if (e != hipSuccess) {

@@ -277,16 +277,17 @@ int gemm_stages() {
 }
 
 template <bool AKM, bool BKM, int EPI, int TM, int TN>
-void launch_tiled(const GemmArgs &gp, bool wide, bool one, int splitk, hipStream_t s) {
+int launch_tiled(const GemmArgs &gp, bool wide, bool one, int splitk, hipStream_t s) {
   const dim3 grid(gp.N / TN, gp.M / TM, splitk);
   if (wide && one)
-    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 64, 1, TM, TN>), grid, dim3(256), 0, s, gp);
+    QCF_LAUNCH((gemm_bf16<AKM, BKM, EPI, 64, 1, TM, TN>), grid, dim3(256), 0, s, gp);
   else if (wide)
-    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 64, 2, TM, TN>), grid, dim3(256), 0, s, gp);
+    QCF_LAUNCH((gemm_bf16<AKM, BKM, EPI, 64, 2, TM, TN>), grid, dim3(256), 0, s, gp);
   else if (one)
-    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 32, 1, TM, TN>), grid, dim3(256), 0, s, gp);
+    QCF_LAUNCH((gemm_bf16<AKM, BKM, EPI, 32, 1, TM, TN>), grid, dim3(256), 0, s, gp);
   else
-    hipLaunchKernelGGL((gemm_bf16<AKM, BKM, EPI, 32, 2, TM, TN>), grid, dim3(256), 0, s, gp);
+    QCF_LAUNCH((gemm_bf16<AKM, BKM, EPI, 32, 2, TM, TN>), grid, dim3(256), 0, s, gp);
+  return QCF_OK;
 }
 
 // `splitk` > 1 (fp32 store only) runs that many K slices into `part`
@@ -306,10 +307,10 @@ int launch_gemm(const GemmArgs &g, hipStream_t s, int splitk = 1, float *part = 
   // 128 x 128 tiles: 128 x 256, 256 x 128, 128 x 64 and 64 x 128 measured
   // slower or no faster on the decoder's shapes (M = 8192, N = K = 1024:
   // 577, 539, 574, 547 against 606 TFLOP/s), as did 128-wide K stages (547)
-  launch_tiled<AKM, BKM, EPI, kTile, kTile>(gp, wide, one, splitk, s);
+  QCF_TRY((launch_tiled<AKM, BKM, EPI, kTile, kTile>(gp, wide, one, splitk, s)));
   if (splitk > 1) {
     const size_t n = (size_t)g.M * g.ldc;
-    hipLaunchKernelGGL(sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part,
+    QCF_LAUNCH(sum_slices, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, part,
                        (uint32_t)splitk, n, g.Cf);
   }
   QCF_HIP_CHECK(hipGetLastError());
@@ -616,16 +617,16 @@ int forward(qcf_mlp *M, uint32_t rows, bool train, hipStream_t s) {
 int colsum(qcf_mlp *M, const bf16 *D, const float *scale, uint32_t rows, uint32_t cols,
            float *out, hipStream_t s) {
   const uint32_t nch = (rows + kColChunk - 1) / kColChunk;
-  hipLaunchKernelGGL(colsum_partial, dim3((cols + 63) / 64, nch), dim3(256), 0, s, D, scale,
+  QCF_LAUNCH(colsum_partial, dim3((cols + 63) / 64, nch), dim3(256), 0, s, D, scale,
                      rows, cols, M->part);
-  hipLaunchKernelGGL(colsum_final, dim3((cols + 63) / 64), dim3(256), 0, s, M->part, nch, cols,
+  QCF_LAUNCH(colsum_final, dim3((cols + 63) / 64), dim3(256), 0, s, M->part, nch, cols,
                      out);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
 }
 
 int transpose(const bf16 *src, bf16 *dst, uint32_t rows, uint32_t cols, hipStream_t s) {
-  hipLaunchKernelGGL(transpose_bf16, dim3(cols / 64, rows / 64), dim3(256), 0, s,
+  QCF_LAUNCH(transpose_bf16, dim3(cols / 64, rows / 64), dim3(256), 0, s,
                      (const uint16_t *)src, (uint16_t *)dst, rows, cols);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
@@ -641,15 +642,15 @@ int fwd_bwd(qcf_mlp *M, float *loss, hipStream_t s) {
   QCF_TRY(forward(M, B, true, s));
   const bf16 *w4 = M->wb + M->off[6];
   const uint32_t nhead = B / (4 * kHeadRows);
-  hipLaunchKernelGGL(head_forward, dim3(nhead), dim3(256), 0, s, M->a3, w4, M->w + M->off[7],
+  QCF_LAUNCH(head_forward, dim3(nhead), dim3(256), 0, s, M->a3, w4, M->w + M->off[7],
                      M->h3, B, M->z, M->y, M->dz, M->lpart, 1.0f / (float)B);
-  hipLaunchKernelGGL(head_backward, dim3(M->h3 / 64, B / 64), dim3(256), 0, s, M->dz, w4,
+  QCF_LAUNCH(head_backward, dim3(M->h3 / 64, B / 64), dim3(256), 0, s, M->dz, w4,
                      M->dg3, M->h3, B, M->d3, M->dT);
   QCF_HIP_CHECK(hipGetLastError());
   float *G = M->g;
   // output layer: grad w4 = a3^T dz, grad b4 = sum dz (and the loss)
   QCF_TRY(colsum(M, M->a3, M->dz, B, M->h3, G + M->off[6], s));
-  hipLaunchKernelGGL(head_sums, dim3(1), dim3(256), 0, s, M->dz, B, M->lpart, nhead,
+  QCF_LAUNCH(head_sums, dim3(1), dim3(256), 0, s, M->dz, B, M->lpart, nhead,
                      G + M->off[7], loss);
   const bf16 *W2 = M->wb + M->off[2], *W3 = M->wb + M->off[4];
   // one layer back, with dT = d^T already in M->dT: grad W = dT aT^T, grad b
@@ -665,7 +666,7 @@ int fwd_bwd(qcf_mlp *M, float *loss, hipStream_t s) {
     int sk = 1;
     while (sk * tiles < 512 && (int)B % (2 * sk * 256) == 0 && 2 * sk <= kMaxSplitK) sk *= 2;
     QCF_TRY((launch_gemm<true, true, kEpiStoreF32>(g, s, sk, M->kpart)));
-    hipLaunchKernelGGL(row_sums, dim3(hout), dim3(256), 0, s, M->dT, B, gb);
+    QCF_LAUNCH(row_sums, dim3(hout), dim3(256), 0, s, M->dT, B, gb);
     QCF_HIP_CHECK(hipGetLastError());
     if (d_prev) {
       // W^T (hin x hout) so both operands are read K-contiguous
@@ -688,7 +689,7 @@ int gather(qcf_mlp *M, const uint64_t *rec, size_t ld, uint64_t s0, uint64_t nsh
   if (s0 % 64) return set_error(QCF_ERR_ARGUMENT, "first shot %llu is not a multiple of 64",
                                 (unsigned long long)s0);
   // one workgroup per 64 shots and 128 input columns
-  hipLaunchKernelGGL(gather_inputs, dim3(rows / 64, M->kin / 128), dim3(256), 0, s, rec,
+  QCF_LAUNCH(gather_inputs, dim3(rows / 64, M->kin / 128), dim3(256), 0, s, rec,
                      ld, s0, nshots,
                      M->det_cols, M->ndet, M->kin, M->obs_col, train ? M->y : nullptr,
                      (uint16_t *)M->x, train ? (uint16_t *)M->xT : nullptr, rows);
@@ -798,7 +799,7 @@ extern "C" int qcf_mlp_set_params(qcf_mlp *m, const float *host, void *stream) {
   QCF_HIP_CHECK(hipMemcpyAsync(m->w, host, m->nparam * 4, hipMemcpyHostToDevice, s));
   QCF_HIP_CHECK(hipMemsetAsync(m->mo, 0, m->nparam * 4, s));
   QCF_HIP_CHECK(hipMemsetAsync(m->ve, 0, m->nparam * 4, s));
-  hipLaunchKernelGGL(to_bf16, dim3((unsigned)((m->nparam + 255) / 256)), dim3(256), 0, s, m->w,
+  QCF_LAUNCH(to_bf16, dim3((unsigned)((m->nparam + 255) / 256)), dim3(256), 0, s, m->w,
                      m->wb, m->nparam);
   QCF_HIP_CHECK(hipGetLastError());
   QCF_HIP_CHECK(hipStreamSynchronize(s));
@@ -845,7 +846,7 @@ extern "C" int qcf_mlp_train_step(qcf_mlp *m, const uint64_t *rec, size_t ld, ui
   m->step += 1;
   const float c1 = 1.0f / (1.0f - std::pow(beta1, (float)m->step));
   const float c2 = 1.0f / (1.0f - std::pow(beta2, (float)m->step));
-  hipLaunchKernelGGL(adam_step, dim3((unsigned)((m->nparam + 255) / 256)), dim3(256), 0, s, m->w,
+  QCF_LAUNCH(adam_step, dim3((unsigned)((m->nparam + 255) / 256)), dim3(256), 0, s, m->w,
                      m->mo, m->ve, m->g, m->wb, m->nparam, lr, beta1, beta2, eps, c1, c2);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
@@ -863,7 +864,7 @@ extern "C" int qcf_mlp_infer(qcf_mlp *m, const uint64_t *rec, size_t ld, uint64_
     const uint32_t rows128 = (rows + 127) / 128 * 128;
     QCF_TRY(gather(m, rec, ld, s0, shots, rows128, false, s));
     QCF_TRY(forward(m, rows128, false, s));
-    hipLaunchKernelGGL(head_forward, dim3(rows128 / (4 * kHeadRows)), dim3(256), 0, s, m->a3,
+    QCF_LAUNCH(head_forward, dim3(rows128 / (4 * kHeadRows)), dim3(256), 0, s, m->a3,
                        m->wb + m->off[6], m->w + m->off[7], m->h3, rows, logits + s0,
                        (const float *)nullptr, (float *)nullptr, (float *)nullptr, 0.0f);
     QCF_HIP_CHECK(hipGetLastError());

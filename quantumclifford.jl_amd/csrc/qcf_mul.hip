@@ -253,20 +253,20 @@ int launch_vec(uint64_t *dest, const uint64_t *src, size_t nw, int phases,
   if (blocks == 0) blocks = 1;
   if (aligned) {
     if (phases)
-      hipLaunchKernelGGL(mul_vec_x4<true>, dim3(blocks), dim3(threads), 0,
+      QCF_LAUNCH(mul_vec_x4<true>, dim3(blocks), dim3(threads), 0,
                          stream, dest, src, nw, right, slot, out_s, dest_phase,
                          src_phase);
     else
-      hipLaunchKernelGGL(mul_vec_x4<false>, dim3(blocks), dim3(threads), 0,
+      QCF_LAUNCH(mul_vec_x4<false>, dim3(blocks), dim3(threads), 0,
                          stream, dest, src, nw, right, slot, out_s, dest_phase,
                          src_phase);
   } else {
     if (phases)
-      hipLaunchKernelGGL(mul_vec_u64<true>, dim3(blocks), dim3(threads), 0,
+      QCF_LAUNCH(mul_vec_u64<true>, dim3(blocks), dim3(threads), 0,
                          stream, dest, src, nw, right, slot, out_s, dest_phase,
                          src_phase);
     else
-      hipLaunchKernelGGL(mul_vec_u64<false>, dim3(blocks), dim3(threads), 0,
+      QCF_LAUNCH(mul_vec_u64<false>, dim3(blocks), dim3(threads), 0,
                          stream, dest, src, nw, right, slot, out_s, dest_phase,
                          src_phase);
   }
@@ -275,7 +275,7 @@ int launch_vec(uint64_t *dest, const uint64_t *src, size_t nw, int phases,
 }
 
 template <int RL>
-void launch_rows_rl(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nw,
+int launch_rows_rl(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nw,
                     size_t nrows, const uint64_t *src,
                     const uint8_t *src_phase, size_t skip,
                     const uint8_t *row_mask, int phases_on, int right,
@@ -284,13 +284,14 @@ void launch_rows_rl(uint64_t *xzs, uint8_t *phases, size_t ld, size_t nw,
   const size_t rows_per_block = threads / RL;
   const size_t blocks = (nrows + rows_per_block - 1) / rows_per_block;
   if (phases_on)
-    hipLaunchKernelGGL((mul_rows_kernel<RL, true>), dim3(blocks), dim3(threads),
+    QCF_LAUNCH((mul_rows_kernel<RL, true>), dim3(blocks), dim3(threads),
                        0, stream, xzs, phases, ld, nw, nrows, src, src_phase,
                        skip, row_mask, right);
   else
-    hipLaunchKernelGGL((mul_rows_kernel<RL, false>), dim3(blocks),
+    QCF_LAUNCH((mul_rows_kernel<RL, false>), dim3(blocks),
                        dim3(threads), 0, stream, xzs, phases, ld, nw, nrows,
                        src, src_phase, skip, row_mask, right);
+  return QCF_OK;
 }
 
 }  // namespace
@@ -353,14 +354,14 @@ extern "C" int qcf_mul_rows(uint64_t *xzs, uint8_t *phases, size_t ld,
   }
   hipStream_t s = (hipStream_t)stream;
   if (nwords <= 16)
-    launch_rows_rl<16>(xzs, phases, ld, nwords, nrows, src, src_phase, skip,
-                       row_mask, phases_on, right, s);
+    QCF_TRY(launch_rows_rl<16>(xzs, phases, ld, nwords, nrows, src, src_phase, skip,
+                               row_mask, phases_on, right, s));
   else if (nwords <= 32)
-    launch_rows_rl<32>(xzs, phases, ld, nwords, nrows, src, src_phase, skip,
-                       row_mask, phases_on, right, s);
+    QCF_TRY(launch_rows_rl<32>(xzs, phases, ld, nwords, nrows, src, src_phase, skip,
+                               row_mask, phases_on, right, s));
   else
-    launch_rows_rl<64>(xzs, phases, ld, nwords, nrows, src, src_phase, skip,
-                       row_mask, phases_on, right, s);
+    QCF_TRY(launch_rows_rl<64>(xzs, phases, ld, nwords, nrows, src, src_phase, skip,
+                               row_mask, phases_on, right, s));
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
 }

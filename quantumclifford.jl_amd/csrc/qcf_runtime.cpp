@@ -28,6 +28,54 @@ void clear_error() { g_last_error.clear(); }
 static std::mutex g_dev_mu;
 static std::vector<DeviceState *> g_dev;
 
+namespace {
+std::mutex g_bounds_mu;
+std::vector<std::pair<const void *, int>> g_bounds;  // kernel -> maxThreadsPerBlock
+
+int bounds_check(const void *key, int maxt, unsigned threads, const char *name) {
+  if (maxt > 0 && threads > (unsigned)maxt)
+    return set_error(QCF_ERR_ARGUMENT, "%s: %u threads per block exceed the kernel's launch bound of %d",
+                     name, threads, maxt);
+  (void)key;
+  return QCF_OK;
+}
+
+bool cached_bound(const void *key, int *maxt) {
+  std::lock_guard<std::mutex> lk(g_bounds_mu);
+  for (const auto &e : g_bounds)
+    if (e.first == key) {
+      *maxt = e.second;
+      return true;
+    }
+  return false;
+}
+
+void cache_bound(const void *key, int maxt) {
+  std::lock_guard<std::mutex> lk(g_bounds_mu);
+  g_bounds.emplace_back(key, maxt);
+}
+}  // namespace
+
+int check_block(const void *kernel, unsigned threads, const char *name) {
+  int maxt = 0;
+  if (!cached_bound(kernel, &maxt)) {
+    hipFuncAttributes a{};
+    QCF_HIP_CHECK(hipFuncGetAttributes(&a, kernel));
+    maxt = a.maxThreadsPerBlock;
+    cache_bound(kernel, maxt);
+  }
+  return bounds_check(kernel, maxt, threads, name);
+}
+
+int check_block_module(hipFunction_t kernel, unsigned threads, const char *name) {
+  int maxt = 0;
+  if (!cached_bound((const void *)kernel, &maxt)) {
+    QCF_HIP_CHECK(hipFuncGetAttribute(&maxt, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, kernel));
+    cache_bound((const void *)kernel, maxt);
+  }
+  return bounds_check((const void *)kernel, maxt, threads, name);
+}
+
 int device_state(DeviceState **out) {
   int dev = 0;
   QCF_HIP_CHECK(hipGetDevice(&dev));

@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libqcf.so")
+# QCF_LIBRARY may name another build of the same library (the sanitized one
+# the CPU tests can run against); there is no non-HIP path either way.
+LIB_PATH = os.environ.get("QCF_LIBRARY") or os.path.join(_HERE, "libqcf.so")
 
 
 class QCFError(RuntimeError):
@@ -94,6 +96,7 @@ _SIGNATURES = {
     "qcf_autotune": (_I, [_P, _SZ, _P, C.POINTER(Tune), _P]),
     "qcf_debug_philox": (_I, [_P, _P, _SZ, _P]),
     "qcf_debug_wave_scan": (_I, [_P, _P, _SZ, _P]),
+    "qcf_debug_check_launch": (_I, [C.c_uint]),
     "qcf_debug_jit_source": (_I, [_P, _I, _P, _SZ, C.POINTER(_SZ)]),
     "qcf_debug_jit_compile": (_I, [_P, _I, C.POINTER(_SZ)]),
     "qcf_record_to_bool": (_I, [_P, _SZ, _U64, _U32, _P, _P]),

@@ -498,6 +498,15 @@ def test_shared_tiles_over_repeated_launches(gpu, steal):
             assert torch.equal(b.records[k][:n], singles[k][:n]), (run, k, p.nq, s)
 
 
+def test_launch_bound_is_checked_on_the_host(gpu):
+    """Every launch first checks its block size against the kernel's
+    __launch_bounds__ on the host; the check itself, run against the
+    persistent batch kernel (bound 64) without launching it."""
+    qcf._lib.call("qcf_debug_check_launch", 64)
+    with pytest.raises(qcf.ArgumentError, match="pf_batch_kernel"):
+        qcf._lib.call("qcf_debug_check_launch", 128)
+
+
 def test_wave_scan_is_exact(gpu):
     """The DPP prefix sum that places noise events equals numpy's cumsum."""
     rng = np.random.default_rng(1)
