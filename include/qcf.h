@@ -402,23 +402,27 @@ int qcf_debug_check_launch(unsigned threads);
 /*
  * Shot-packed record (nbits columns of ld words) -> Matrix{Bool}(shots, nbits)
  * column-major bytes, as returned by measurements(::PauliFrame)
- * (src/pauli_frames.jl:280).  And the inverse.
+ * (src/pauli_frames.jl:280).  And the inverse.  `ldr` is the Bool matrix's
+ * column stride in elements (0 = shots): a view of a frame's rows r
+ * (Base.view(::PauliFrame, r), src/pauli_frames.jl:17) passes the parent's
+ * pointer advanced by first(r)-1 and the parent's row count.
  */
 int qcf_record_to_bool(const uint64_t *meas, size_t ld, uint64_t shots,
-                       uint32_t nbits, uint8_t *out, void *stream);
-int qcf_record_from_bool(const uint8_t *in, uint64_t shots, uint32_t nbits,
+                       uint32_t nbits, uint8_t *out, size_t ldr, void *stream);
+int qcf_record_from_bool(const uint8_t *in, size_t ldr, uint64_t shots, uint32_t nbits,
                          uint64_t *meas, size_t ld, void *stream);
 
 /*
  * Shot-packed frame (2*nq planes of ld words) <-> reference frame layout:
  * the `fastcolumn` xzs of PauliFrame.frame, i.e. a shots x 2W column-major
  * UInt64 matrix with W = ceil(nq/64) (src/pauli_frames.jl:32,
- * src/fastmemlayout.jl:47).  Pad bits beyond nq are written as zero by
- * qcf_frame_to_reference and ignored by qcf_frame_from_reference.
+ * src/fastmemlayout.jl:47), `ldr` elements between its columns (0 = shots;
+ * a view passes its parent's, as above).  Pad bits beyond nq are written as
+ * zero by qcf_frame_to_reference and ignored by qcf_frame_from_reference.
  */
 int qcf_frame_to_reference(const uint64_t *frame, size_t ld, uint64_t shots,
-                           uint32_t nq, uint64_t *xzs, void *stream);
-int qcf_frame_from_reference(const uint64_t *xzs, uint64_t shots, uint32_t nq,
+                           uint32_t nq, uint64_t *xzs, size_t ldr, void *stream);
+int qcf_frame_from_reference(const uint64_t *xzs, size_t ldr, uint64_t shots, uint32_t nq,
                              uint64_t *frame, size_t ld, void *stream);
 
 /* ---- matching decoder (host) ------------------------------------------- */

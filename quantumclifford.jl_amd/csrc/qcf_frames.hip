@@ -376,16 +376,18 @@ __global__ __launch_bounds__(64) void scan_kernel(const uint32_t *in, uint32_t *
 }
 
 // ---- reference-layout converters
+// The reference-side matrices are column-major with `ldr` elements between
+// columns (ldr > shots for a view of a larger frame's rows).
 __global__ void record_to_bool_kernel(const uint64_t *meas, size_t ld,
                                       uint64_t shots, uint32_t nbits,
-                                      uint8_t *out) {
+                                      uint8_t *out, size_t ldr) {
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t b = blockIdx.y;
   if (s >= shots || b >= nbits) return;
-  out[(uint64_t)b * shots + s] = (uint8_t)((meas[(uint64_t)b * ld + s / 64] >> (s % 64)) & 1u);
+  out[(uint64_t)b * ldr + s] = (uint8_t)((meas[(uint64_t)b * ld + s / 64] >> (s % 64)) & 1u);
 }
 
-__global__ void record_from_bool_kernel(const uint8_t *in, uint64_t shots,
+__global__ void record_from_bool_kernel(const uint8_t *in, size_t ldr, uint64_t shots,
                                         uint32_t nbits, uint64_t *meas,
                                         size_t ld) {
   const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -394,7 +396,7 @@ __global__ void record_from_bool_kernel(const uint8_t *in, uint64_t shots,
   uint64_t v = 0;
   for (int i = 0; i < 64; ++i) {
     const uint64_t s = w * 64 + i;
-    if (s < shots && in[(uint64_t)b * shots + s]) v |= 1ull << i;
+    if (s < shots && in[(uint64_t)b * ldr + s]) v |= 1ull << i;
   }
   meas[(uint64_t)b * ld + w] = v;
 }
@@ -403,7 +405,7 @@ __global__ void record_from_bool_kernel(const uint8_t *in, uint64_t shots,
 // x bits of qubits 64j+1..64j+64, column W+j the z bits.
 __global__ void frame_to_reference_kernel(const uint64_t *frame, size_t ld,
                                           uint64_t shots, uint32_t nq,
-                                          uint64_t *xzs) {
+                                          uint64_t *xzs, size_t ldr) {
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t j = blockIdx.y;  // 0 .. 2W-1
   const uint32_t W = (nq + 63) / 64;
@@ -416,10 +418,10 @@ __global__ void frame_to_reference_kernel(const uint64_t *frame, size_t ld,
     const uint64_t plane = (uint64_t)half * nq + q;
     v |= ((frame[plane * ld + s / 64] >> (s % 64)) & 1ull) << i;
   }
-  xzs[(uint64_t)j * shots + s] = v;
+  xzs[(uint64_t)j * ldr + s] = v;
 }
 
-__global__ void frame_from_reference_kernel(const uint64_t *xzs, uint64_t shots,
+__global__ void frame_from_reference_kernel(const uint64_t *xzs, size_t ldr, uint64_t shots,
                                             uint32_t nq, uint64_t *frame,
                                             size_t ld) {
   const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -431,7 +433,7 @@ __global__ void frame_from_reference_kernel(const uint64_t *xzs, uint64_t shots,
   uint64_t v = 0;
   for (int i = 0; i < 64; ++i) {
     const uint64_t s = w * 64 + i;
-    if (s < shots) v |= ((xzs[(uint64_t)j * shots + s] >> (q % 64)) & 1ull) << i;
+    if (s < shots) v |= ((xzs[(uint64_t)j * ldr + s] >> (q % 64)) & 1ull) << i;
   }
   frame[(uint64_t)plane * ld + w] = v;
 }
@@ -746,51 +748,64 @@ extern "C" int qcf_debug_check_launch(unsigned threads) {
   return check_block((const void *)pf_batch_kernel, threads, "pf_batch_kernel");
 }
 
+// Column stride of a reference-side matrix: 0 means `shots` (a whole matrix).
+int ref_stride(size_t *ldr, uint64_t shots) {
+  if (*ldr == 0) *ldr = shots;
+  if (*ldr < shots)
+    return set_error(QCF_ERR_DIMENSION, "reference column stride %zu < %llu shots", *ldr,
+                     (unsigned long long)shots);
+  return QCF_OK;
+}
+
 extern "C" int qcf_record_to_bool(const uint64_t *meas, size_t ld, uint64_t shots,
-                                  uint32_t nbits, uint8_t *out, void *stream) {
+                                  uint32_t nbits, uint8_t *out, size_t ldr, void *stream) {
   clear_error();
   if (shots == 0 || nbits == 0) return QCF_OK;
   if (ld < (shots + 63) / 64) return set_error(QCF_ERR_DIMENSION, "record stride too short");
+  QCF_TRY(ref_stride(&ldr, shots));
   dim3 grid((unsigned)((shots + 255) / 256), nbits);
   QCF_LAUNCH(record_to_bool_kernel, grid, dim3(256), 0, (hipStream_t)stream, meas, ld,
-                     shots, nbits, out);
+                     shots, nbits, out, ldr);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
 }
 
-extern "C" int qcf_record_from_bool(const uint8_t *in, uint64_t shots, uint32_t nbits,
-                                    uint64_t *meas, size_t ld, void *stream) {
+extern "C" int qcf_record_from_bool(const uint8_t *in, size_t ldr, uint64_t shots,
+                                    uint32_t nbits, uint64_t *meas, size_t ld, void *stream) {
   clear_error();
   if (shots == 0 || nbits == 0) return QCF_OK;
   if (ld < (shots + 63) / 64) return set_error(QCF_ERR_DIMENSION, "record stride too short");
+  QCF_TRY(ref_stride(&ldr, shots));
   dim3 grid((unsigned)((ld + 255) / 256), nbits);
-  QCF_LAUNCH(record_from_bool_kernel, grid, dim3(256), 0, (hipStream_t)stream, in,
+  QCF_LAUNCH(record_from_bool_kernel, grid, dim3(256), 0, (hipStream_t)stream, in, ldr,
                      shots, nbits, meas, ld);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
 }
 
 extern "C" int qcf_frame_to_reference(const uint64_t *frame, size_t ld, uint64_t shots,
-                                      uint32_t nq, uint64_t *xzs, void *stream) {
+                                      uint32_t nq, uint64_t *xzs, size_t ldr, void *stream) {
   clear_error();
   if (shots == 0 || nq == 0) return QCF_OK;
   if (ld < (shots + 63) / 64) return set_error(QCF_ERR_DIMENSION, "frame stride too short");
+  QCF_TRY(ref_stride(&ldr, shots));
   const uint32_t W = (nq + 63) / 64;
   dim3 grid((unsigned)((shots + 255) / 256), 2 * W);
   QCF_LAUNCH(frame_to_reference_kernel, grid, dim3(256), 0, (hipStream_t)stream,
-                     frame, ld, shots, nq, xzs);
+                     frame, ld, shots, nq, xzs, ldr);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
 }
 
-extern "C" int qcf_frame_from_reference(const uint64_t *xzs, uint64_t shots, uint32_t nq,
-                                        uint64_t *frame, size_t ld, void *stream) {
+extern "C" int qcf_frame_from_reference(const uint64_t *xzs, size_t ldr, uint64_t shots,
+                                        uint32_t nq, uint64_t *frame, size_t ld, void *stream) {
   clear_error();
   if (shots == 0 || nq == 0) return QCF_OK;
   if (ld < (shots + 63) / 64) return set_error(QCF_ERR_DIMENSION, "frame stride too short");
+  QCF_TRY(ref_stride(&ldr, shots));
   dim3 grid((unsigned)((ld + 255) / 256), 2 * nq);
   QCF_LAUNCH(frame_from_reference_kernel, grid, dim3(256), 0, (hipStream_t)stream,
-                     xzs, shots, nq, frame, ld);
+                     xzs, ldr, shots, nq, frame, ld);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
 }

@@ -99,10 +99,10 @@ _SIGNATURES = {
     "qcf_debug_check_launch": (_I, [C.c_uint]),
     "qcf_debug_jit_source": (_I, [_P, _I, _P, _SZ, C.POINTER(_SZ)]),
     "qcf_debug_jit_compile": (_I, [_P, _I, C.POINTER(_SZ)]),
-    "qcf_record_to_bool": (_I, [_P, _SZ, _U64, _U32, _P, _P]),
-    "qcf_record_from_bool": (_I, [_P, _U64, _U32, _P, _SZ, _P]),
-    "qcf_frame_to_reference": (_I, [_P, _SZ, _U64, _U32, _P, _P]),
-    "qcf_frame_from_reference": (_I, [_P, _U64, _U32, _P, _SZ, _P]),
+    "qcf_record_to_bool": (_I, [_P, _SZ, _U64, _U32, _P, _SZ, _P]),
+    "qcf_record_from_bool": (_I, [_P, _SZ, _U64, _U32, _P, _SZ, _P]),
+    "qcf_frame_to_reference": (_I, [_P, _SZ, _U64, _U32, _P, _SZ, _P]),
+    "qcf_frame_from_reference": (_I, [_P, _SZ, _U64, _U32, _P, _SZ, _P]),
     "qcf_program_bitops": (C.c_double, [_P]),
     "qcf_dem_build": (_I, [_P, _P, _SZ, _P, _SZ, C.POINTER(_P)]),
     "qcf_dem_free": (_I, [_P]),

@@ -301,7 +301,7 @@ def _relative_measurements(frame: PauliFrame) -> torch.Tensor:
     out = torch.empty(max(1, frame.nbits) * max(1, frame.shots), dtype=torch.uint8,
                       device=frame.record.device)
     call("qcf_record_to_bool", _ptr(frame.record), frame.ld, frame.shots, frame.nbits,
-         _ptr(out), _stream())
+         _ptr(out), 0, _stream())
     m = out[: frame.nbits * frame.shots].view(frame.nbits, frame.shots).t()
     return m.bool()
 
@@ -333,11 +333,11 @@ def frame_to_reference(frame: PauliFrame) -> torch.Tensor:
     W = (frame.nq + 63) // 64
     out = torch.empty(2 * W * max(1, frame.shots), dtype=torch.int64, device=frame.frame.device)
     call("qcf_frame_to_reference", _ptr(frame.frame), frame.ld, frame.shots, frame.nq,
-         _ptr(out), _stream())
+         _ptr(out), 0, _stream())
     return out.view(2 * W, frame.shots)
 
 
 def frame_from_reference(xzs: torch.Tensor, frame: PauliFrame) -> PauliFrame:
-    call("qcf_frame_from_reference", _ptr(xzs.contiguous()), frame.shots, frame.nq,
+    call("qcf_frame_from_reference", _ptr(xzs.contiguous()), 0, frame.shots, frame.nq,
          _ptr(frame.frame), frame.ld, _stream())
     return frame

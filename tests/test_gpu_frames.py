@@ -498,6 +498,44 @@ def test_shared_tiles_over_repeated_launches(gpu, steal):
             assert torch.equal(b.records[k][:n], singles[k][:n]), (run, k, p.nq, s)
 
 
+def test_reference_converters_on_a_view_of_a_larger_frame(gpu):
+    """The converters address the reference matrices with a column stride, so a
+    view of rows b:b+n of a P-shot frame (src/pauli_frames.jl:17) converts in
+    place: the view's columns hold the frame, every other element is left
+    alone, and converting back gives the frame again."""
+    import ctypes as C
+    c = circuits.rotated_surface_code(3, 3, 2e-2)
+    prog = qcf.Program(c.ops)
+    n, P, b = 3000, 5000, 700
+    f = qcf.pftrajectories(prog, n, seed=8)
+    W = (f.nq + 63) // 64
+    st = qcf.frames._stream()
+    parent = torch.full((2 * W, P), -1, dtype=torch.int64, device="cuda")
+    qcf._lib.call("qcf_frame_to_reference", qcf.frames._ptr(f.frame), f.ld, n, f.nq,
+                  C.c_void_p(parent.data_ptr() + 8 * b), P, st)
+    want = qcf.frame_to_reference(f)
+    assert torch.equal(parent[:, b:b + n], want)
+    assert (parent[:, :b] == -1).all() and (parent[:, b + n:] == -1).all()
+    g = qcf.PauliFrame(n, f.nq - 1, f.nbits, init=False)
+    qcf._lib.call("qcf_frame_from_reference", C.c_void_p(parent.data_ptr() + 8 * b), P, n,
+                  f.nq, qcf.frames._ptr(g.frame), g.ld, st)
+    assert np.array_equal(g.frame_host(), f.frame_host())
+    bools = torch.full((f.nbits, P), 7, dtype=torch.uint8, device="cuda")
+    qcf._lib.call("qcf_record_to_bool", qcf.frames._ptr(f.record), f.ld, n, f.nbits,
+                  C.c_void_p(bools.data_ptr() + b), P, st)
+    assert torch.equal(bools[:, b:b + n].t().bool(), qcf.measurements(f))
+    assert (bools[:, :b] == 7).all() and (bools[:, b + n:] == 7).all()
+    rec = torch.zeros_like(f.record)
+    qcf._lib.call("qcf_record_from_bool", C.c_void_p(bools.data_ptr() + b), P, n, f.nbits,
+                  qcf.frames._ptr(rec), f.ld, st)
+    nw = f.nbits * f.ld
+    assert torch.equal(rec[:nw].view(f.nbits, f.ld)[:, : (n + 63) // 64],
+                       f.record[:nw].view(f.nbits, f.ld)[:, : (n + 63) // 64])
+    with pytest.raises(qcf.DimensionMismatch):
+        qcf._lib.call("qcf_frame_to_reference", qcf.frames._ptr(f.frame), f.ld, n, f.nq,
+                      C.c_void_p(parent.data_ptr()), n - 1, st)
+
+
 def test_launch_bound_is_checked_on_the_host(gpu):
     """Every launch first checks its block size against the kernel's
     __launch_bounds__ on the host; the check itself, run against the
