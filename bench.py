@@ -372,6 +372,27 @@ def cpu_baseline(target_s=12.0):
                 mul_left=mul, canonicalize=canon)
 
 
+def bench_mlp_train(steps: int):
+    """The decoder's training step alone (for profiling): `steps` batches of
+    8192 device-sampled d=5 x 25 shots after 32 warm-up batches; ms per step
+    including the sampling of every 16th batch, and the matrix-core rate."""
+    from qcf import circuits, mlp
+
+    def make(p):
+        return circuits.rotated_surface_code(5, 25, p)
+
+    c = make(1e-3)
+    dec = mlp.MLPDecoder(c.det_bits, c.obs_bits[0], batch=8192, infer_batch=65536, seed=0)
+    mlp.train(dec, make, steps=32, p_range=(5e-4, 5e-3), lr=3e-4, seed=1)
+    tr = mlp.train(dec, make, steps=steps, p_range=(5e-4, 5e-3), lr=3e-4, seed=2)
+    layers = ((dec.kin, 1024), (1024, 1024), (1024, 512))
+    flops_step = 2 * 8192 * (2 * sum(a * b for a, b in layers) +
+                             sum(a * b for a, b in layers[1:]))
+    ms = tr["seconds"] * 1e3 / steps
+    return dict(steps=steps, ms_per_step=ms, tflops=flops_step / (ms * 1e-3) / 1e12,
+                gflop_per_step=flops_step / 1e9)
+
+
 def bench_mlp(steps: int, eval_shots: int):
     """Perceptron decoder (600 -> 1024 -> 1024 -> 512 -> 1, BF16 on the matrix
     cores) trained for `steps` batches of 8192 device-sampled d=5 x 25 shots
@@ -462,7 +483,7 @@ def main():
     ap.add_argument("--mlp-steps", type=int, default=24414)  # 2e8 shots of 8192
     ap.add_argument("--mlp-eval-shots", type=int, default=10**7)
     ap.add_argument("--only", choices=("reference_circuits", "canonicalize", "mul_left_1gq",
-                                       "mul_config", "batched"),
+                                       "mul_config", "batched", "mlp_train"),
                     help="run one extra alone and print its JSON (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -477,7 +498,8 @@ def main():
               "canonicalize": lambda: bench_canonicalize(5, 1),
               "mul_left_1gq": lambda: bench_mul(10, 2),
               "mul_config": lambda: bench_mul_config(10, 2),
-              "batched": lambda: bench_batched(dmax=a.batched_dmax)}[a.only]
+              "batched": lambda: bench_batched(dmax=a.batched_dmax),
+              "mlp_train": lambda: bench_mlp_train(min(a.mlp_steps, 2000))}[a.only]
         print(json.dumps({a.only: fn()}), flush=True)
         return
     log("frames: d=5 x 25 rounds, p=1e-3 ...")

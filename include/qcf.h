@@ -352,7 +352,9 @@ int qcf_pf_sample_batched(const qcf_program *const *progs, size_t n,
 
 /* Pick qcf_tune for a set of circuits by timing batched launches of them on
  * the current device (on scratch records; the choice never changes results,
- * the RNG being keyed on absolute shots).  Candidates: 1 or 2 helper
+ * the RNG being keyed on absolute shots): one warm launch and the median of
+ * three timed ones per candidate (QCF_AUTOTUNE_LOG=1 prints them).
+ * Candidates: 1 or 2 helper
  * streams; the persistent kernel sharing the tiles of no specialised launch,
  * or of those of the largest one, two or three frame sizes among the ones it
  * can share, at 8 or 16 waves per CU; and, when it has circuits of its own

@@ -1336,6 +1336,12 @@ extern "C" int qcf_pf_sample_batched(const qcf_program *const *progs, size_t n,
   return st;
 }
 
+// QCF_AUTOTUNE_LOG=1 prints every candidate's time to stderr.
+static bool autotune_log() {
+  const char *e = getenv("QCF_AUTOTUNE_LOG");
+  return e && e[0] == '1';
+}
+
 extern "C" int qcf_autotune(const qcf_program *const *progs, size_t n,
                             const uint64_t *shots, qcf_tune *out, void *stream) {
   clear_error();
@@ -1358,15 +1364,24 @@ extern "C" int qcf_autotune(const qcf_program *const *progs, size_t n,
     st = set_error(QCF_ERR_HIP, "autotune events");
   float best = 1e30f;
   qcf_tune best_t{0, 1, 0, 0, 0, 0, 0};
+  // one warm launch, then the median of three timed ones: a single timing
+  // can catch a fast moment of the device and pick a shape that is slow
   auto time_one = [&](qcf_batch *B, const qcf_tune &t) -> int {
     QCF_TRY(qcf::set_tune(B, &t));
     QCF_TRY(qcf::batch_launch(B, s));  // warm
-    if (hipEventRecord(e0, s) != hipSuccess) return set_error(QCF_ERR_HIP, "event");
-    QCF_TRY(qcf::batch_launch(B, s));
-    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
-      return set_error(QCF_ERR_HIP, "event");
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
+    float v[3];
+    for (float &x : v) {
+      if (hipEventRecord(e0, s) != hipSuccess) return set_error(QCF_ERR_HIP, "event");
+      QCF_TRY(qcf::batch_launch(B, s));
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+        return set_error(QCF_ERR_HIP, "event");
+      (void)hipEventElapsedTime(&x, e0, e1);
+    }
+    std::sort(v, v + 3);
+    const float ms = v[1];
+    if (autotune_log())
+      fprintf(stderr, "qcf_autotune: wpc %u streams %u concurrent %u steal_planes %u: %.2f ms\n",
+              t.waves_per_cu, t.streams, t.concurrent, t.steal_planes, ms);
     if (ms < best) {
       best = ms;
       best_t = t;
