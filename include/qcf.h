@@ -269,6 +269,14 @@ int qcf_program_get_info(const qcf_program *prog, qcf_program_info *info);
  * So the bits of a shot depend on (seed, its absolute tile) and not on the
  * launch shape, batching or kernel: a run split at multiples of 2048 shots
  * (with shot_offset) reproduces the one-run bits.
+ * Workspace: launches whose frames live in global memory (frames beyond
+ * what the specialised kernels hold on chip, and replay/export launches of
+ * frames over 160 KB) use a library-owned workspace per (device, stream):
+ * calls on one stream run in order, calls on different streams never share
+ * one.  It grows on demand behind a synchronisation of that stream; while
+ * the stream is being captured into a graph it cannot grow, and the call
+ * returns QCF_ERR_ARGUMENT (make one uncaptured call of the same size
+ * first).
  * Errors: QCF_ERR_ARGUMENT for a native shot_offset that is not a multiple
  * of 2048, QCF_ERR_DIMENSION for ld < ceil(shots/64).
  */
