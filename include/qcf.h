@@ -260,8 +260,10 @@ int qcf_program_get_info(const qcf_program *prog, qcf_program_info *info);
  *
  * Native draws (no reference counterpart: the reference draws from Julia's
  * task-local Xoshiro, src/pauli_frames.jl:47-56,110-113,136-174):
- *   - gauge words (initZ!, the sMRZ z-flip): one xoshiro128** stream per
- *     absolute 32-shot word, seeded by Philox4x32-10(seed, word);
+ *   - gauge words (initZ!, the sMRZ z-flip): per absolute 32-shot word, a
+ *     Weyl sequence with its own odd increment, each term scrambled by the
+ *     murmur3 32-bit finalizer (SplittableRandom's construction), start and
+ *     increment from Philox4x32-10(seed, word);
  *   - noise events: one stream of event positions per absolute 2048-shot
  *     tile along its (site ordinal, shot) sequence, drawn 64 at a time (one
  *     geometric gap per lane from a per-lane xoshiro128** stream seeded by

@@ -11,12 +11,14 @@
 //
 // Randomness (native mode) -- keyed only on (seed, absolute shot position):
 //   * gauge words (initZ!, the sMRZ z-flip, src/pauli_frames.jl:47-56,110-113):
-//     a per-lane xoshiro128** stream seeded from Philox4x32-10(seed, word);
-//   * noise events: per rate class, event positions along the (site ordinal,
-//     shot) sequence of the wave's 2048-shot tile are drawn 64 at a time, one
-//     geometric gap per lane from Philox4x32-10(seed, tile, batch, class,
-//     lane) and a wave prefix sum.  At a site the lanes holding events that
-//     fall in this site flip the target shot's bits with LDS atomics.
+//     per lane, a Weyl sequence with a murmur3-finalized output, start and
+//     increment from Philox4x32-10(seed, absolute 32-shot word);
+//   * noise events: event positions along the (site ordinal, shot) sequence
+//     of the wave's 2048-shot tile are drawn 64 at a time, one geometric gap
+//     per lane from that lane's xoshiro128** stream, seeded once per tile by
+//     Philox4x32-10(seed, absolute tile, lane), and a wave prefix sum.  At a
+//     site the lanes holding events that fall in this site flip the target
+//     shot's bits with LDS atomics.
 // Replay mode reads every draw from caller-supplied mask planes instead.
 #include <hip/hip_runtime.h>
 
@@ -115,7 +117,7 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
   const bool live = gw < P.nwbuf;
   const uint32_t lmask = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1 ? P.lastmask : 0xFFFFFFFFu);
   const uint64_t goff = gw;
-  qcfd::Xoshiro gauge{0, 0, 0, 1};
+  qcfd::Gauge gauge{0, 1};
   if (MODE != kReplay) gauge = qcfd::gauge_stream(P.seed, gw + P.tile0 * 64u);
   EvState ev0 = qcfd::ev_init(P.seed, blk + P.tile0, lane);
   if (MODE != kReplay && P.nclasses > 0) ev0 = qcfd::gen_batch(ev0, P.cls_c[0]);

@@ -991,7 +991,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     char *const recb = (char *)P.rec;
     const uint32_t gw4 = gw * 4u;
     (void)ld4; (void)recb; (void)gw4;
-    qcfd::Xoshiro g = qcfd::gauge_stream(P.seed, gw + P.tile0 * 64u);
+    qcfd::Gauge g = qcfd::gauge_stream(P.seed, gw + P.tile0 * 64u);
 )QCFJIT";
   src << "    qcfd::EvState ev = qcfd::ev_init(P.seed, blk + P.tile0, l);\n"
          "    if (P.has_class) ev = qcfd::gen_batch(ev, P.c);\n"

@@ -15,6 +15,9 @@ and the default (specialised-kernel) path against the oracle directly.
 * A record of the default launch equals the oracle replaying the draws the
   interpreter exports for the same seed (the specialised kernels cannot
   export, so this ties them to the oracle without the interpreter's output).
+* Gauge draws (the z-flip of a reset, src/pauli_frames.jl:110-113) are
+  uniform and pairwise independent across consecutive draws, adjacent shots
+  of a word, adjacent words and adjacent tiles.
 """
 import numpy as np
 import pytest
@@ -116,3 +119,28 @@ def test_default_launch_equals_oracle_replay(gpu):
     got = oracle.unpack_shots(f.record_host(), shots)
     assert np.array_equal(got, om[: prog.nbits])
     assert got[np.asarray(c.det_bits) - 1].any()
+
+
+def test_gauge_draws_uniform_and_pairwise_independent(gpu):
+    """R resets of one qubit, each followed by an X measurement: after sMRZ the
+    frame's z bit is the gauge draw, which sMX records, so the record holds R
+    consecutive gauge draws per shot.  At 10^7 shots every draw is 1 with
+    probability 1/2 and the correlation of any of the pairs below is 0, each
+    within 5 sigma (sigma = 1/sqrt(pairs))."""
+    R = 8
+    circ = []
+    for k in range(R):
+        circ += [qcf.sMRZ(1, 0), qcf.sMX(1, k + 1)]
+    prog = qcf.Program(circ)
+    f = qcf.pftrajectories(prog, SHOTS, seed=77, keep_frame=False)
+    counts = popcount_planes(f.record, R, f.ld)
+    assert_within_5_sigma(counts, [0.5] * R, SHOTS, "gauge draws")
+    bits = oracle.unpack_shots(f.record_host()[:R], SHOTS).astype(np.float64) * 2 - 1
+    pairs = {"consecutive draws": (bits[:-1], bits[1:]),
+             "adjacent shots": (bits[:, :-1], bits[:, 1:]),
+             "adjacent words (shot s, s + 32)": (bits[:, :-32], bits[:, 32:]),
+             "adjacent tiles (shot s, s + 2048)": (bits[:, :-2048], bits[:, 2048:])}
+    for what, (a, b) in pairs.items():
+        c = (a * b).mean(axis=1)
+        n = a.shape[1]
+        assert np.abs(c).max() * np.sqrt(n) < 5.0, (what, float(np.abs(c).max() * np.sqrt(n)))
