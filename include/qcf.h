@@ -225,6 +225,21 @@ int qcf_program_build(const qcf_op *ops, size_t nops, const uint32_t *args,
 int qcf_program_free(qcf_program *prog);
 int qcf_program_get_info(const qcf_program *prog, qcf_program_info *info);
 
+/*
+ * Gauge draws of a fresh, record-only launch (QCF_PF_FRESH without a frame
+ * buffer).  No counterpart in the reference: it describes what this library
+ * leaves out.  out[r] for every replay plane r < info.nreplay: 0 = a noise
+ * plane, 1 = a gauge draw (initZ! z word, sMRZ/sMRX z-flip) such a launch
+ * makes, 2 = a gauge draw it leaves out because the draws kept before it
+ * already span every record pattern it could produce: the record is linear
+ * over GF(2) in the gauge bits, so its distribution is unchanged.  A native
+ * launch makes no draw there and a mask export leaves that plane as the
+ * caller zeroed it; replay applies every plane.  Launches that return the
+ * frame draw everything (the frame's gauge part spans more).
+ * Errors: QCF_ERR_ARGUMENT, QCF_ERR_DIMENSION (n < info.nreplay).
+ */
+int qcf_program_gauge_planes(const qcf_program *prog, uint8_t *out, size_t n);
+
 /* qcf_pf_sample flags */
 #define QCF_PF_FRESH 1u        /* start from PauliFrame(...) + initZ!: x=0, z random,
                                   record zeroed (pftrajectories(circuit; trajectories)) */
@@ -362,14 +377,16 @@ int qcf_pf_sample_batched(const qcf_program *const *progs, size_t n,
 
 /* Pick qcf_tune for a set of circuits by timing batched launches of them on
  * the current device (on scratch records; the choice never changes results,
- * the RNG being keyed on absolute shots): one warm launch and the median of
- * three timed ones per candidate (QCF_AUTOTUNE_LOG=1 prints them).
- * Candidates: 1 or 2 helper
+ * the RNG being keyed on absolute shots).  Each candidate runs as the graph
+ * qcf_batch_launch_graph replays: one warm replay and the median of three
+ * timed ones (QCF_AUTOTUNE_LOG=1 prints them).  Candidates: 1 or 2 helper
  * streams; the persistent kernel sharing the tiles of no specialised launch,
  * or of those of the largest one, two or three frame sizes among the ones it
  * can share, at 8 or 16 waves per CU; and, when it has circuits of its own
  * and shares nothing, starting it beside the specialised launches or after
- * them. */
+ * them.  The fastest is then timed three more times against the library
+ * defaults, alternately; unless it wins on both the median and the slowest
+ * of those, the all-zero tune (the defaults) is returned. */
 int qcf_autotune(const qcf_program *const *progs, size_t n,
                  const uint64_t *shots, qcf_tune *out, void *stream);
 

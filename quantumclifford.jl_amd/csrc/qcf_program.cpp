@@ -628,12 +628,198 @@ qcf_program::~qcf_program() {
 std::vector<qcf::Instr> qcf::variant_code(const qcf_program *P, int variant) {
   std::vector<Instr> code;
   const auto &pro = (variant & 1) ? P->prologue_fresh : P->prologue_inplace;
+  // a fresh run that returns only the record leaves out the gauge draws that
+  // cannot change its distribution (mark_dead_gauges)
+  const bool thin = (variant & 3) == 1;
+  const size_t p0 = code.size();
   code.insert(code.end(), pro.begin(), pro.end());
+  if (thin && P->gauge_dead_pro.size() == pro.size())
+    for (size_t i = 0; i < pro.size(); ++i)
+      if (P->gauge_dead_pro[i]) code[p0 + i].op |= kNoGauge;
   while (code.size() % kChunk) code.push_back(Instr{I_END, 0, 0, 0, 0, 0, 0, 0});
+  const size_t b0 = code.size();
   code.insert(code.end(), P->body.begin(), P->body.end());
+  if (thin && P->gauge_dead_body.size() == P->body.size())
+    for (size_t i = 0; i < P->body.size(); ++i)
+      if (P->gauge_dead_body[i]) code[b0 + i].op |= kNoGauge;
   if (variant & 2) code.insert(code.end(), P->epilogue_store.begin(), P->epilogue_store.end());
   if (code.empty()) code.push_back(Instr{I_END, 0, 0, 0, 0, 0, 0, 0});
   return code;
+}
+
+// Which gauge draws a fresh, record-only run needs.
+//
+// Over GF(2) the frame is linear in its random inputs: gates, measurements,
+// resets and classical XORs are linear maps of the planes, a noise site
+// XORs flips drawn independently of the frame, and each gauge draw (initZ!,
+// the sMRZ/sMRX z-flip) XORs an independent uniform word into a z plane.  So
+// the record of a shot is R = G g + N, with g the vector of gauge bits
+// (uniform on GF(2)^k), G the record bits each gauge bit reaches, and N the
+// noise part, independent of g.  G g is uniform on the column span of G, and
+// so is G' g' for any set of columns of G that spans it.  Keeping the draws
+// whose columns are independent of the columns kept before them (in program
+// order) and dropping the rest therefore leaves the distribution of every
+// record, jointly over all its bits, exactly as it was: only which uniform
+// word lands where changes.  The final frame is not covered (its gauge part
+// spans more), so launches that return the frame draw everything.
+//
+// G comes from running the program symbolically: each plane and record
+// column holds the set of gauge draws XORed into it (a bitset), following the
+// instruction semantics of the interpreter (qcf_frames.hip run_tile).  The
+// column basis is a greedy XOR basis over record bits.  Programs whose
+// bitsets or elimination would be very large keep every draw.
+void qcf::mark_dead_gauges(qcf_program *P) {
+  P->gauge_dead_pro.clear();
+  P->gauge_dead_body.clear();
+  auto is_gauge = [](const Instr &in) {
+    const uint32_t op = in.op & 0xFFu;
+    return op == I_INIT || op == I_RESETZ || op == I_MR || op == I_MRX;
+  };
+  size_t k = 0;
+  for (const Instr &in : P->prologue_fresh) k += is_gauge(in);
+  for (const Instr &in : P->body) k += is_gauge(in);
+  P->ngauge = (uint32_t)k;
+  P->ngauge_live = (uint32_t)k;
+  if (k == 0) return;
+  const size_t W = (k + 63) / 64;           // words per gauge bitset
+  const size_t NB = std::max<uint32_t>(P->nbits, 1u);
+  const size_t WB = (NB + 63) / 64;         // words per record-bit bitset
+  const double cells = (double)(P->nplanes + NB + 1) * W + (double)k * WB;
+  const double elim = (double)k * (double)std::min(k, NB) * (double)WB;
+  if (cells > 32e6 || elim > 4e9) return;   // keep every draw
+  std::vector<uint64_t> pl((size_t)P->nplanes * W, 0), rec(NB * W, 0), acc(W, 0);
+  auto PLp = [&](uint32_t p) { return &pl[(size_t)p * W]; };
+  auto RCp = [&](uint32_t b) { return &rec[(size_t)b * W]; };
+  auto cpy = [&](uint64_t *d, const uint64_t *s) { std::copy(s, s + W, d); };
+  auto xr = [&](uint64_t *d, const uint64_t *s) {
+    for (size_t w = 0; w < W; ++w) d[w] ^= s[w];
+  };
+  auto zero = [&](uint64_t *d) { std::fill(d, d + W, 0ull); };
+  std::vector<uint64_t> v(W), t(W), src(7 * W);
+  size_t g = 0;  // next gauge index
+  auto run = [&](const std::vector<Instr> &code) {
+    for (size_t i = 0; i < code.size(); ++i) {
+      const Instr &in = code[i];
+      const uint32_t op = in.op & 0xFFu;
+      switch (op) {
+        case I_XORN: {
+          const uint32_t n = in.op >> 8;
+          const uint32_t *w = &in.a;
+          for (uint32_t j = 0; j < n; ++j) cpy(&src[j * W], PLp(w[j] >> 16));
+          for (uint32_t j = 0; j < n; ++j) xr(PLp(w[j] & 0xFFFFu), &src[j * W]);
+          break;
+        }
+        case I_NOISEG: i += in.op >> 8; break;  // noise: independent of g
+        case I_MEAS:
+          cpy(v.data(), PLp(in.a));
+          cpy(RCp(in.b), v.data());
+          if (in.c != kNone) cpy(PLp(in.c), v.data());
+          break;
+        case I_XSTORE2:
+          cpy(v.data(), PLp(in.a));
+          xr(v.data(), PLp(in.b));
+          cpy(RCp(in.c), v.data());
+          if (in.d != kNone) cpy(PLp(in.d), v.data());
+          break;
+        case I_RESETZ:
+          zero(PLp(in.a));
+          PLp(in.b)[g / 64] ^= 1ull << (g % 64);
+          ++g;
+          break;
+        case I_ACC_LDS: cpy(acc.data(), PLp(in.a)); break;
+        case I_ACCX_LDS: xr(acc.data(), PLp(in.a)); break;
+        case I_ACC_REC: cpy(acc.data(), RCp(in.a)); break;
+        case I_ACCX_REC: xr(acc.data(), RCp(in.a)); break;
+        case I_ACC_STORE:
+          cpy(RCp(in.a), acc.data());
+          if (in.b != kNone) cpy(PLp(in.b), acc.data());
+          break;
+        case I_REC_ZERO: zero(RCp(in.a)); break;
+        case I_INIT:
+          zero(PLp(in.a));
+          zero(PLp(in.b));
+          PLp(in.b)[g / 64] ^= 1ull << (g % 64);
+          ++g;
+          break;
+        case I_LOAD:
+        case I_ZERO: zero(PLp(in.a)); break;
+        case I_LIN1: {
+          cpy(v.data(), PLp(in.a));
+          cpy(t.data(), PLp(in.b));
+          uint64_t *x = PLp(in.a), *z = PLp(in.b);
+          zero(x);
+          zero(z);
+          if (in.c & 1u) xr(x, v.data());
+          if (in.c & 2u) xr(x, t.data());
+          if (in.c & 4u) xr(z, v.data());
+          if (in.c & 8u) xr(z, t.data());
+          break;
+        }
+        case I_MR:
+        case I_MRX: {
+          cpy(v.data(), PLp(in.a));
+          if (in.d != kNone) cpy(RCp(in.d), v.data());
+          if (op == I_MRX) {
+            cpy(t.data(), v.data());
+            xr(t.data(), PLp(in.f));
+            cpy(RCp(in.g), t.data());
+          }
+          if (in.e != kNone) cpy(PLp(in.e), v.data());
+          zero(PLp(in.a));
+          PLp(in.b)[g / 64] ^= 1ull << (g % 64);
+          ++g;
+          break;
+        }
+        default: break;  // I_END, I_STORE
+      }
+    }
+  };
+  run(P->prologue_fresh);
+  run(P->body);
+  // columns of G: gauge draw j -> the record bits it reaches
+  std::vector<uint64_t> col((size_t)k * WB, 0);
+  for (size_t b = 0; b < P->nbits; ++b) {
+    const uint64_t *r = RCp((uint32_t)b);
+    for (size_t w = 0; w < W; ++w)
+      for (uint64_t m = r[w]; m; m &= m - 1) {
+        const size_t j = w * 64 + __builtin_ctzll(m);
+        col[j * WB + b / 64] |= 1ull << (b % 64);
+      }
+  }
+  // greedy XOR basis in program order, pivots on the lowest set bit
+  std::vector<int64_t> piv(NB, -1);  // record bit -> basis vector
+  std::vector<uint64_t> basis;
+  std::vector<uint8_t> live(k, 0);
+  size_t nb = 0;
+  // (a basis vector's lowest set bit is its pivot, so reducing by it clears
+  // that bit and touches no lower one)
+  for (size_t j = 0; j < k; ++j) {
+    uint64_t *c = &col[j * WB];
+    for (size_t w = 0; w < WB && !live[j];) {
+      if (!c[w]) {
+        ++w;
+        continue;
+      }
+      const size_t b = w * 64 + __builtin_ctzll(c[w]);
+      if (piv[b] < 0) {  // independent of the draws kept so far: kept
+        piv[b] = (int64_t)nb++;
+        basis.insert(basis.end(), c, c + WB);
+        live[j] = 1;
+      } else {
+        const uint64_t *e = &basis[(size_t)piv[b] * WB];
+        for (size_t u = w; u < WB; ++u) c[u] ^= e[u];
+      }
+    }
+  }
+  size_t j = 0, nlive = 0;
+  P->gauge_dead_pro.assign(P->prologue_fresh.size(), 0);
+  for (size_t i = 0; i < P->prologue_fresh.size(); ++i)
+    if (is_gauge(P->prologue_fresh[i])) P->gauge_dead_pro[i] = !live[j++];
+  P->gauge_dead_body.assign(P->body.size(), 0);
+  for (size_t i = 0; i < P->body.size(); ++i)
+    if (is_gauge(P->body[i])) P->gauge_dead_body[i] = !live[j++];
+  for (uint8_t x : live) nlive += x;
+  P->ngauge_live = (uint32_t)nlive;
 }
 
 extern "C" int qcf_program_build(const qcf_op *ops, size_t nops,
@@ -876,6 +1062,7 @@ extern "C" int qcf_program_build(const qcf_op *ops, size_t nops,
     P->epilogue_store.push_back(Instr{I_STORE, C.xs[q], q, 0, 0, 0, 0, 0});
     P->epilogue_store.push_back(Instr{I_STORE, C.zs[q], nq + q, 0, 0, 0, 0, 0});
   }
+  mark_dead_gauges(P);
   *out = P;
   return QCF_OK;
 }
@@ -897,6 +1084,26 @@ extern "C" int qcf_program_get_info(const qcf_program *P, qcf_program_info *info
   info->nreplay = P->ninit_rs + P->nbody_rs;
   info->lds_bytes = P->nplanes * 64u * 4u;
   info->bitops_per_shot = P->bitops;
+  return QCF_OK;
+}
+
+extern "C" int qcf_program_gauge_planes(const qcf_program *P, uint8_t *out, size_t n) {
+  clear_error();
+  if (!P || (!out && n)) return set_error(QCF_ERR_ARGUMENT, "null pointer");
+  const size_t nrep = (size_t)P->ninit_rs + P->nbody_rs;
+  if (n < nrep) return set_error(QCF_ERR_DIMENSION, "gauge plane array %zu < %zu", n, nrep);
+  std::fill(out, out + n, (uint8_t)0);
+  auto mark = [&](const std::vector<Instr> &code, const std::vector<uint8_t> &dead,
+                  uint32_t base) {
+    for (size_t i = 0; i < code.size(); ++i) {
+      const uint32_t op = code[i].op & 0xFFu;
+      if (op != I_INIT && op != I_RESETZ && op != I_MR && op != I_MRX) continue;
+      const size_t r = (size_t)base + code[i].c;
+      if (r < n) out[r] = (i < dead.size() && dead[i]) ? 2 : 1;
+    }
+  };
+  mark(P->prologue_fresh, P->gauge_dead_pro, 0u);
+  mark(P->body, P->gauge_dead_body, P->ninit_rs);
   return QCF_OK;
 }
 

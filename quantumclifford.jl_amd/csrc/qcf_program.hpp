@@ -1,6 +1,6 @@
 // Device program of the Pauli-frame sampler.
 //
-// A circuit is lowered on the host into a flat stream of 16-byte
+// A circuit is lowered on the host into a flat stream of 32-byte
 // instructions over "planes".  A plane is one 32-bit word per lane of a
 // 64-lane wave, i.e. the x bit or the z bit of one qubit for 2048 shots, or a
 // classical bit kept on chip.  Gates whose bit map is a permutation of planes
@@ -55,6 +55,14 @@ enum InstrOp : uint32_t {
   I_MRX,        // I_MR, then the detector rec[g] = v ^ P[f]
 };
 
+// Flag in the op word of I_INIT, I_RESETZ, I_MR and I_MRX (fresh record-only
+// code only, see variant_code): the instruction's gauge draw reaches no record
+// bit beyond what the draws kept before it already span, so a native launch
+// makes no draw there (the z plane is left alone, or zero for I_INIT) and a
+// mask export leaves that mask plane as the caller zeroed it.  Replay still
+// applies the mask it is given.
+constexpr uint32_t kNoGauge = 1u << 8;
+
 // 32-byte device instruction.  Thresholds of I_SITE are 2^31-scaled and
 // compared with the top 31 bits of a draw.
 struct Instr {
@@ -91,6 +99,9 @@ namespace qcf {
 // Device code of one launch variant: bit 0 = fresh prologue (else in-place
 // loads), bit 1 = frame store epilogue; the body starts on a chunk boundary.
 std::vector<Instr> variant_code(const qcf_program *P, int variant);
+// Marks the gauge draws of the fresh prologue and the body that a fresh,
+// record-only run can leave out (gauge_dead_pro / gauge_dead_body).
+void mark_dead_gauges(qcf_program *P);
 }  // namespace qcf
 
 // Host-side compiled program (opaque to C callers).
@@ -105,6 +116,10 @@ struct qcf_program {
   std::vector<qcf::Instr> prologue_inplace; // LOAD
   std::vector<qcf::Instr> body;
   std::vector<qcf::Instr> epilogue_store;   // STORE
+  // per instruction of prologue_fresh / body: 1 = gauge draw that a fresh,
+  // record-only run skips (empty when the analysis was not run)
+  std::vector<uint8_t> gauge_dead_pro, gauge_dead_body;
+  uint32_t ngauge = 0, ngauge_live = 0;  // gauge draws of a fresh run; those kept
   std::vector<qcf::Site> sites;
   std::vector<qcf::ClassInfo> classes;
   // device copies, created lazily per device and variant

@@ -86,6 +86,7 @@ _SIGNATURES = {
     "qcf_program_build": (_I, [_P, _SZ, _P, _SZ, _U32, _U32, C.POINTER(_P)]),
     "qcf_program_free": (_I, [_P]),
     "qcf_program_get_info": (_I, [_P, C.POINTER(ProgramInfo)]),
+    "qcf_program_gauge_planes": (_I, [_P, _P, _SZ]),
     "qcf_pf_sample": (_I, [_P, _U64, _U64, _U64, _P, _P, _P, _SZ, _U32, _P]),
     "qcf_plane_words": (_SZ, [_U64]),
     "qcf_batch_create": (_I, [_P, _SZ, _P, _P, _P, _P, C.POINTER(Tune), C.POINTER(_P)]),

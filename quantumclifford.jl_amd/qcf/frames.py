@@ -103,6 +103,13 @@ class Program:
     def bitops_per_shot(self) -> float:
         return self.info.bitops_per_shot
 
+    def gauge_planes(self):
+        """Per replay plane: 0 noise, 1 gauge draw a fresh record-only launch
+        makes, 2 gauge draw it leaves out (qcf_program_gauge_planes)."""
+        out = np.zeros(max(1, self.nreplay), dtype=np.uint8)
+        _lib.call("qcf_program_gauge_planes", self.handle, out.ctypes.data, out.size)
+        return out[: self.nreplay]
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h and h.value and _lib._lib is not None:

@@ -3,7 +3,8 @@ oracle one per shot via replay masks, and the exact record marginals they
 determine.
 
 Replay planes are laid out as libqcf documents them: the initZ! z words of
-qubits 1..nq, then one plane per sMRZ/sMRX z-flip, two (x, z) per
+qubits 1..nq, then one plane per sMRZ/sMRX z-flip (and per PauliMeasurement,
+whose ancilla is measured and reset), two (x, z) per
 single-qubit noise site and four (x1, z1, x2, z2) per two-qubit depolarizing
 site, in op order.  The Pauli choices of a site are mutually exclusive
 (src/pauli_frames.jl:136-174; the two-qubit channel follows
@@ -27,7 +28,7 @@ def program_faults(prog):
     plane, site = prog.nq, 0
     for op in prog.ops:
         k = int(op["kind"])
-        if k in (K["MRZ"], K["MRX"]):
+        if k in (K["MRZ"], K["MRX"], K["PAULI_MEASUREMENT"]):  # the last resets its ancilla
             gauge.append(plane)
             plane += 1
         elif k == K["PAULI_ERROR"]:
