@@ -132,14 +132,17 @@ __device__ __forceinline__ EvState ev_init(uint64_t seed, uint32_t tile, uint32_
 // (failures before it, floor(log2(u) * c) with c = 1/log2(1-p)), the uniform
 // for the Pauli choice and the 31-bit uniform for thinning.  Gaps beyond 2^24
 // trials advance without an event (null, uacc = 0xFFFFFFFF; memorylessness
-// keeps the process exact).
+// keeps the process exact).  A program whose sites all accept every event of
+// their class or none (thresholds 2^31 or 0) needs no thinning uniform
+// (thin = 0): uacc is then 0 for every event, which passes 2^31 and fails 0,
+// and the stream advances by two draws per event instead of three.
 struct Draw {
   uint32_t incr, upau, uacc;
 };
-__device__ __forceinline__ Draw draw_event(Xoshiro &rng, float c) {
+__device__ __forceinline__ Draw draw_event(Xoshiro &rng, float c, uint32_t thin) {
   const uint32_t r0 = rng.next();
   const uint32_t r1 = rng.next();
-  const uint32_t r2 = rng.next();
+  const uint32_t r2 = thin ? rng.next() : 0u;
   const float uf = ((float)(r0 >> 8) + 0.5f) * (1.0f / 16777216.0f);
   const float g = __log2f(uf) * c;
   Draw d;
@@ -155,8 +158,8 @@ __device__ __forceinline__ Draw draw_event(Xoshiro &rng, float c) {
 }
 
 // Draw the next 64 events: one gap per lane and a wave prefix sum.
-__device__ __forceinline__ EvState gen_batch(EvState ev, float c) {
-  const Draw d = draw_event(ev.rng, c);
+__device__ __forceinline__ EvState gen_batch(EvState ev, float c, uint32_t thin) {
+  const Draw d = draw_event(ev.rng, c, thin);
   ev.pos = ev.last + wave_inclusive_scan(d.incr);
   ev.last = readlane(ev.pos, 63);
   ev.upau = d.upau;
@@ -190,7 +193,7 @@ template <class Flip>
 __device__ __forceinline__ void group_events(EvState &ev, uint32_t ord0, uint32_t G,
                                              uint32_t base, uint32_t e0, uint32_t e1,
                                              uint32_t e2, uint32_t e3, uint32_t pay, float c,
-                                             Flip flip) {
+                                             uint32_t thin, Flip flip) {
   const uint32_t lo = ord0 * kTileShots;
   const uint32_t span = G * kTileShots;
   while (true) {
@@ -218,7 +221,7 @@ __device__ __forceinline__ void group_events(EvState &ev, uint32_t ord0, uint32_
       }
     }
     if (ev.last >= lo + span) break;
-    ev = gen_batch(ev, c);
+    ev = gen_batch(ev, c, thin);
   }
 }
 
@@ -288,7 +291,8 @@ __device__ __forceinline__ void flips_r(const EvState &ev, uint32_t rel, uint32_
 template <bool TWO, bool ONE, bool W2L, bool W3L, bool P8, class Flip>
 __device__ __forceinline__ void group_events_r(EvState &ev, uint32_t ord0, uint32_t G,
                                                uint32_t pa, uint32_t pb, uint32_t w2,
-                                               uint32_t w3, float c, PrefR<P8> pf, Flip flip) {
+                                               uint32_t w3, float c, uint32_t thin,
+                                               PrefR<P8> pf, Flip flip) {
   const uint32_t lo = ord0 * kTileShots;
   const uint32_t span = G * kTileShots;
   // the prefetched values are used on every path, so no permute is left
@@ -303,7 +307,7 @@ __device__ __forceinline__ void group_events_r(EvState &ev, uint32_t ord0, uint3
       flips_r<TWO, ONE, W2L, W3L, P8>(ev, rel, pf.a, pf.b, pf.t, flip);
   }
   while (ev.last < lo + span) {
-    ev = gen_batch(ev, c);
+    ev = gen_batch(ev, c, thin);
     const uint32_t rel = ev.pos - lo;
     const bool inr = rel < span;
     if (__builtin_amdgcn_ballot_w64(inr && ev.uacc != 0xFFFFFFFFu) != 0) {

@@ -56,6 +56,7 @@ struct KParams {
   uint32_t sstride;    // 32-bit words of global plane storage per wave slot
   uint32_t nclasses;
   float cls_c[kMaxClasses];
+  uint32_t thin;       // some site thins its class's events (qcf_program::thin)
 };
 
 using qcfd::EvState;
@@ -120,7 +121,7 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
   qcfd::Gauge gauge{0, 1};
   if (MODE != kReplay) gauge = qcfd::gauge_stream(P.seed, gw + P.tile0 * 64u);
   EvState ev0 = qcfd::ev_init(P.seed, blk + P.tile0, lane);
-  if (MODE != kReplay && P.nclasses > 0) ev0 = qcfd::gen_batch(ev0, P.cls_c[0]);
+  if (MODE != kReplay && P.nclasses > 0) ev0 = qcfd::gen_batch(ev0, P.cls_c[0], P.thin);
   // Every global load in the instruction loop is waited for on the spot, so
   // no load is ever pending where control flow merges (which would make the
   // compiler wait for all outstanding record stores there).
@@ -206,7 +207,7 @@ __device__ __forceinline__ void run_tile(const KParams &P, uint32_t blk,
             export_flip<MODE>(P, P.body_rbase + rs + j, blk * 64u + tl, bit);
           };
           qcfd::group_events(ev0, readlane(cA.y, k), G, k + 1, cA.y, cA.z, cB.x, cB.y, cA.w,
-                             P.cls_c[0], flip);
+                             P.cls_c[0], P.thin, flip);
         }
         k += G;
         continue;
@@ -672,6 +673,7 @@ int pf_launch_one(const qcf_program *prog, uint64_t shots, uint64_t shot_offset,
   kp.nclasses = (uint32_t)P->classes.size();
   for (uint32_t c = 0; c < kMaxClasses; ++c)
     kp.cls_c[c] = c < P->classes.size() ? P->classes[c].c : 0.f;
+  kp.thin = P->thin ? 1u : 0u;
   kp.nplanes = P->nplanes;
   DeviceState *st;
   QCF_TRY(device_state(&st));
@@ -892,6 +894,7 @@ int build_kparams(const qcf_program *prog, uint64_t shots, uint64_t seed,
   kp->nclasses = (uint32_t)P->classes.size();
   for (uint32_t c = 0; c < kMaxClasses; ++c)
     kp->cls_c[c] = c < P->classes.size() ? P->classes[c].c : 0.f;
+  kp->thin = P->thin ? 1u : 0u;
   return QCF_OK;
 }
 

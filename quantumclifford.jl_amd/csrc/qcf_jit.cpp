@@ -296,6 +296,8 @@ constexpr uint32_t kSplitMaxRegs = 400u;
 constexpr size_t kMaxLargeFrameGroups = 256;
 
 bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds_bytes) {
+  // the event stream's thinning draw, a constant of the kernel
+  const char *const thin = P->thin ? "1u" : "0u";
   std::vector<Instr> code;
   for (const Instr &in : variant_code(P, variant))
     if ((in.op & 0xFFu) != I_END) code.push_back(in);  // chunk padding
@@ -757,14 +759,14 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
             if (preload) {
               if (!prefetched) emit_prefetch(gi, in_loop, body);
               body << "    if (P.has_class) qcfd::group_events_r" << pass_tmpl(gi) << "("
-                   << pass_args(gi, in_loop) << ", P.c, pf" << gi << ", flip);\n";
+                   << pass_args(gi, in_loop) << ", P.c, " << thin << ", pf" << gi << ", flip);\n";
               prefetched = false;
             } else {
               if (gi + 1 < gend)
                 body << "    uint32_t n0, n1, n2, n3;\n" << load_group(gi + 1, "n");
               body << "    if (P.has_class) qcfd::group_events(ev, " << g.ord0
                    << (in_loop ? "u + ob, " : "u, ") << g.count
-                   << "u, 0u, t0, t1, t2, t3, 0u, P.c, flip);\n";
+                   << "u, 0u, t0, t1, t2, t3, 0u, P.c, " << thin << ", flip);\n";
               if (gi + 1 < gend) body << "    t0 = n0; t1 = n1; t2 = n2; t3 = n3;\n";
             }
             if (regs || split) {
@@ -1002,7 +1004,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     qcfd::Gauge g = qcfd::gauge_stream(P.seed, gw + P.tile0 * 64u);
 )QCFJIT";
   src << "    qcfd::EvState ev = qcfd::ev_init(P.seed, blk + P.tile0, l);\n"
-         "    if (P.has_class) ev = qcfd::gen_batch(ev, P.c);\n"
+         "    if (P.has_class) ev = qcfd::gen_batch(ev, P.c, " << thin << ");\n"
          "    uint32_t acc = 0u;\n    (void)acc;\n";
   if (halfm)
     src << "    const bool live = lh && gw < P.nwbuf;\n"

@@ -1041,6 +1041,8 @@ extern "C" int qcf_program_build(const qcf_op *ops, size_t nops,
     delete P;
     return st;
   }
+  for (const Site &s : P->sites)
+    if (s.accept != 0u && s.accept != (1u << 31)) P->thin = true;
   C.peephole();
   P->nplanes = renumber_slots(P->body, 2 * nq);
   C.group_sites();

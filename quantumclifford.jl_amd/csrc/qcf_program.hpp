@@ -122,6 +122,9 @@ struct qcf_program {
   uint32_t ngauge = 0, ngauge_live = 0;  // gauge draws of a fresh run; those kept
   std::vector<qcf::Site> sites;
   std::vector<qcf::ClassInfo> classes;
+  // some site keeps only part of its class's events (0 < accept < 2^31): the
+  // native event stream then draws a thinning uniform per event
+  bool thin = false;
   // device copies, created lazily per device and variant
   struct DeviceCopy {
     int device = -1;
