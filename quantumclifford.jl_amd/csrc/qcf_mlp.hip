@@ -11,8 +11,9 @@
 // Every dense contraction runs in one hand-written GEMM kernel on the gfx950
 // matrix cores (v_mfma_f32_32x32x16_bf16): 128x128 output tiles per
 // 256-thread workgroup (four waves, 64x64 each as 2x2 MFMA tiles), K in
-// stages of 64 through one LDS buffer with the next stage prefetched into
-// registers, tiles renumbered so that each XCD works on one band of rows.
+// stages of 64 through one LDS buffer with the stage after next prefetched
+// into registers, tiles renumbered so that each XCD works on one band of
+// rows.
 // Operands are read K-contiguous (or, in the debug entry, transposed).
 // Epilogues fuse bias + GELU, keeping the GELU derivative for the backward
 // pass (forward), the multiplication by that derivative (backward), and
@@ -60,13 +61,29 @@ struct GemmArgs {
   bf16 *Cg;            // kEpiBiasGelu: gelu'(p), for the backward pass
 };
 
-__device__ __forceinline__ float gelu(float x) {
-  return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f));
-}
-__device__ __forceinline__ float dgelu(float x) {
-  const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-  const float pdf = 0.39894228040143268f * __expf(-0.5f * x * x);
-  return cdf + x * pdf;
+// GELU(p) = p Phi(p) and GELU'(p) = Phi(p) + p phi(p), phi the standard
+// normal density.  erfc comes from Abramowitz & Stegun, Handbook of
+// Mathematical Functions, 7.1.26: for x >= 0, erfc(x) = (a1 t + a2 t^2 +
+// a3 t^3 + a4 t^4 + a5 t^5) e^{-x^2} with t = 1 / (1 + 0.3275911 x), absolute
+// error below 1.5e-7 -- far inside the BF16 rounding of both outputs.  With
+// x = |p| / sqrt(2), e^{-x^2} = e^{-p^2/2} is also what phi(p) needs, so both
+// outputs cost one exponential and one reciprocal (the exact erff and expf
+// took about 80 instructions per element and dominated the forward
+// epilogue).
+struct GeluPair {
+  float act, der;
+};
+__device__ __forceinline__ GeluPair gelu_pair(float p) {
+  const float x = fabsf(p) * 0.70710678118654752f;
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, x, 1.0f));
+  float q = fmaf(t, 1.061405429f, -1.453152027f);
+  q = fmaf(t, q, 1.421413741f);
+  q = fmaf(t, q, -0.284496736f);
+  q = fmaf(t, q, 0.254829592f);
+  const float e = __builtin_amdgcn_exp2f(-x * x * 1.4426950408889634f);  // e^{-x^2}
+  const float h = 0.5f * t * q * e;      // erfc(x) / 2 = Phi(-|p|)
+  const float cdf = p >= 0.0f ? 1.0f - h : h;
+  return GeluPair{p * cdf, fmaf(p, 0.39894228040143268f * e, cdf)};
 }
 
 // One operand tile (ROWS rows of the output dimension x BK of K) moves from
@@ -126,12 +143,21 @@ __device__ __forceinline__ void store_tile(bf16 (*S)[BK + 8], int t,
 // (TM/2) x (TN/2) as 32 x 32 MFMA tiles.  Requires M, N multiples of TM, TN
 // and K a multiple of BK (checked on the host).
 template <bool A_KMAJOR, bool B_KMAJOR, int EPI, int BK, int STAGES, int TM, int TN>
-__global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
-  // STAGES == 2: double-buffered LDS, one barrier per K stage;
-  // STAGES == 1: one LDS buffer (half the LDS, twice the resident
-  // workgroups), two barriers per stage; both prefetch the next stage into
-  // registers while the matrix cores work on the current one
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+gemm_bf16(GemmArgs g) {
+  // STAGES == 2: double-buffered LDS, one barrier per K stage, the next
+  // stage prefetched into registers; STAGES == 1: one LDS buffer (half the
+  // LDS, twice the resident workgroups), two barriers per stage, the stage
+  // after next prefetched into registers
   constexpr int FM = TM / 64, FN = TN / 64;  // 32 x 32 tiles per wave
+  // Output orientation.  The input-gradient epilogue reads the GELU
+  // derivative at every output: with a lane per row and four consecutive
+  // columns it takes one 8-byte load per four outputs (36.1 -> 26.9 us per
+  // launch at M = 8192, N = K = 1024).  The other epilogues store only, and
+  // a lane per column writes 64- and 128-byte runs of a row per instruction,
+  // which measured faster than 8- and 16-byte pieces of 32 rows (forward
+  // 26.0 against 30.0 us, weight gradients 24.5 against 26.3).
+  constexpr bool kRowLanes = EPI == kEpiDGelu;
   // operand stages As, Bs; after the K loop the same LDS holds the output
   // tile's transpose for the optional batch-minor copy (rows padded)
   constexpr int kTSp = TM + 8;
@@ -161,27 +187,9 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-  TileRegs<TM, BK> ra;
-  TileRegs<TN, BK> rb;
-  load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, kbase, t, ra);
-  load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, kbase, t, rb);
-  if (STAGES == 2) {
-    store_tile<A_KMAJOR, TM, BK>(As[0], t, ra);
-    store_tile<B_KMAJOR, TN, BK>(Bs[0], t, rb);
-    __syncthreads();
-  }
   const int KT = kz / BK;
-  for (int kt = 0; kt < KT; ++kt) {
-    const int cur = STAGES == 2 ? (kt & 1) : 0;
-    if (STAGES == 1) {
-      store_tile<A_KMAJOR, TM, BK>(As[0], t, ra);
-      store_tile<B_KMAJOR, TN, BK>(Bs[0], t, rb);
-      __syncthreads();
-    }
-    if (kt + 1 < KT) {
-      load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, kbase + (kt + 1) * BK, t, ra);
-      load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, kbase + (kt + 1) * BK, t, rb);
-    }
+  // the matrix-core work of one K stage from LDS buffer `cur`
+  auto mma_stage = [&](int cur) {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 16) {
       bf16x8 a[FM], b[FN];
@@ -191,59 +199,140 @@ __global__ void __launch_bounds__(256) gemm_bf16(GemmArgs g) {
 #pragma unroll
       for (int j = 0; j < FN; ++j)
         b[j] = *(const bf16x8 *)&Bs[cur][wn * (TN / 2) + j * 32 + (lane & 31)][kk + 8 * (lane >> 5)];
+      // kRowLanes: B as the matrix core's first operand, so the product is
+      // the tile's transpose and each lane holds one output row m and runs
+      // of four consecutive columns n
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = kRowLanes
+                          ? __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j], 0, 0, 0)
+                          : __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
-    if (STAGES == 2 && kt + 1 < KT) {
-      store_tile<A_KMAJOR, TM, BK>(As[cur ^ 1], t, ra);
-      store_tile<B_KMAJOR, TN, BK>(Bs[cur ^ 1], t, rb);
+  };
+  if (STAGES == 1) {
+    // One LDS buffer, the global loads two K stages ahead in two register
+    // sets: a stage's tiles are requested while the matrix cores work on the
+    // two stages before it, so a load has two stages of MFMA time to arrive
+    // (one stage ahead, the wave waited on nearly every load).
+    TileRegs<TM, BK> ra0, ra1;
+    TileRegs<TN, BK> rb0, rb1;
+    load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, kbase, t, ra0);
+    load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, kbase, t, rb0);
+    if (KT > 1) {
+      load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, kbase + BK, t, ra1);
+      load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, kbase + BK, t, rb1);
     }
+    auto stage = [&](int kt, TileRegs<TM, BK> &ra, TileRegs<TN, BK> &rb) {
+      store_tile<A_KMAJOR, TM, BK>(As[0], t, ra);
+      store_tile<B_KMAJOR, TN, BK>(Bs[0], t, rb);
+      __syncthreads();
+      if (kt + 2 < KT) {  // refill the set just stored
+        load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, kbase + (kt + 2) * BK, t, ra);
+        load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, kbase + (kt + 2) * BK, t, rb);
+      }
+      mma_stage(0);
+      __syncthreads();
+    };
+    for (int kt = 0; kt < KT; kt += 2) {
+      stage(kt, ra0, rb0);
+      if (kt + 1 < KT) stage(kt + 1, ra1, rb1);
+    }
+  } else {
+    TileRegs<TM, BK> ra;
+    TileRegs<TN, BK> rb;
+    load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, kbase, t, ra);
+    load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, kbase, t, rb);
+    store_tile<A_KMAJOR, TM, BK>(As[0], t, ra);
+    store_tile<B_KMAJOR, TN, BK>(Bs[0], t, rb);
     __syncthreads();
+    for (int kt = 0; kt < KT; ++kt) {
+      const int cur = kt & 1;
+      if (kt + 1 < KT) {
+        load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, kbase + (kt + 1) * BK, t, ra);
+        load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, kbase + (kt + 1) * BK, t, rb);
+      }
+      mma_stage(cur);
+      if (kt + 1 < KT) {
+        store_tile<A_KMAJOR, TM, BK>(As[cur ^ 1], t, ra);
+        store_tile<B_KMAJOR, TN, BK>(Bs[cur ^ 1], t, rb);
+      }
+      __syncthreads();
+    }
   }
-  // epilogue: lane holds column (lane & 31) of each 32x32 tile, rows
-  // 8*(r/4) + 4*(lane/32) + r%4
+  if constexpr (kRowLanes) {
+    // in each 32 x 32 tile the lane holds row m = (lane & 31) and, for
+    // q = 0..3, the four consecutive columns n = 8q + 4 (lane / 32) + u
+    // (accumulator element 4q + u)
 #pragma unroll
-  for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < FN; ++j) {
-      const int n = n0 + wn * (TN / 2) + j * 32 + (lane & 31);
-      const float bn = (EPI == kEpiBiasGelu || EPI == kEpiBiasGeluInfer) ? g.bias[n] : 0.0f;
+      for (int j = 0; j < FN; ++j) {
+        const int m = m0 + wm * (TM / 2) + i * 32 + (lane & 31);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        // r = 4q .. 4q+3 are four consecutive rows m
-        const int mq = m0 + wm * (TM / 2) + i * 32 + 8 * q + 4 * (lane >> 5);
-        uint16_t h[4];
+        for (int q = 0; q < 4; ++q) {
+          const int nq = n0 + wn * (TN / 2) + j * 32 + 8 * q + 4 * (lane >> 5);
+          const size_t o = (size_t)m * g.ldc + nq;
+          const uint2 dg = *(const uint2 *)(g.dg_in + o);
+          const uint32_t dw[2] = {dg.x, dg.y};
+          uint16_t h[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int r = 4 * q + u, m = mq + u;
-          const size_t o = (size_t)m * g.ldc + n;
-          const float v = acc[i][j][r];
-          bf16 b = (bf16)0.0f;
-          if (EPI == kEpiStoreF32) {
-            g.Cf[o + (size_t)blockIdx.z * g.M * g.ldc] = v;
-          } else if (EPI == kEpiBiasGelu) {
-            const float p = v + bn;
-            b = (bf16)gelu(p);
-            g.Cg[o] = (bf16)dgelu(p);
-          } else if (EPI == kEpiBiasGeluInfer) {
-            b = (bf16)gelu(v + bn);
-          } else {
-            b = (bf16)(v * (float)g.dg_in[o]);
+          for (int u = 0; u < 4; ++u) {
+            const float du = __builtin_bit_cast(float, (dw[u >> 1] >> (16 * (u & 1))) << 16);
+            h[u] = __builtin_bit_cast(uint16_t, (bf16)(acc[i][j][4 * q + u] * du));
           }
-          if (EPI != kEpiStoreF32) g.Cb[o] = b;
-          h[u] = __builtin_bit_cast(uint16_t, b);
-        }
-        if (EPI != kEpiStoreF32 && g.CbT) {
-          // the tile's transpose goes through LDS: T[n - n0][m - m0]
-          uint16_t *T = (uint16_t *)lds_raw;
-          *(uint2 *)(T + (size_t)(n - n0) * kTSp + (mq - m0)) =
+          *(uint2 *)(g.Cb + o) =
               uint2{h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16)};
+          if (g.CbT) {
+            // the tile's transpose goes through LDS: T[n - n0][m - m0]
+            uint16_t *T = (uint16_t *)lds_raw;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) T[(size_t)(nq + u - n0) * kTSp + (m - m0)] = h[u];
+          }
         }
       }
-    }
+  } else {
+    // in each 32 x 32 tile the lane holds column n = (lane & 31) and rows
+    // 8 (r / 4) + 4 (lane / 32) + r % 4 (accumulator element r)
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int n = n0 + wn * (TN / 2) + j * 32 + (lane & 31);
+        const float bn = (EPI == kEpiBiasGelu || EPI == kEpiBiasGeluInfer) ? g.bias[n] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          // r = 4q .. 4q+3 are four consecutive rows m
+          const int mq = m0 + wm * (TM / 2) + i * 32 + 8 * q + 4 * (lane >> 5);
+          uint16_t h[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int r = 4 * q + u, m = mq + u;
+            const size_t o = (size_t)m * g.ldc + n;
+            const float v = acc[i][j][r];
+            bf16 b = (bf16)0.0f;
+            if (EPI == kEpiStoreF32) {
+              g.Cf[o + (size_t)blockIdx.z * g.M * g.ldc] = v;
+            } else if (EPI == kEpiBiasGelu) {
+              const GeluPair gp = gelu_pair(v + bn);
+              b = (bf16)gp.act;
+              g.Cg[o] = (bf16)gp.der;
+            } else {
+              b = (bf16)gelu_pair(v + bn).act;
+            }
+            if (EPI != kEpiStoreF32) g.Cb[o] = b;
+            h[u] = __builtin_bit_cast(uint16_t, b);
+          }
+          if (EPI != kEpiStoreF32 && g.CbT) {
+            // the tile's transpose goes through LDS: T[n - n0][m - m0]
+            uint16_t *T = (uint16_t *)lds_raw;
+            *(uint2 *)(T + (size_t)(n - n0) * kTSp + (mq - m0)) =
+                uint2{h[0] | ((uint32_t)h[1] << 16), h[2] | ((uint32_t)h[3] << 16)};
+          }
+        }
+      }
+  }
   if (EPI != kEpiStoreF32 && g.CbT) {
     // TN rows n of TM consecutive m: 16 bytes per thread
     __syncthreads();
@@ -272,6 +361,20 @@ int gemm_stages() {
   static const int v = [] {
     const char *e = getenv("QCF_GEMM_STAGES");
     return e && e[0] == '2' ? 2 : 1;
+  }();
+  return v;
+}
+
+// Workgroups a split-K weight gradient aims at (QCF_SPLITK_WGS, default
+// 256, one per CU).  Batch 8192: the three weight-gradient GEMMs took 24.5,
+// 23.9 and 35.1 us per launch at targets 512, 256 and 128, and the slice
+// sums after them 8.8, 5.6 and 4.9 us (twice the slices, twice the fp32
+// partials written and read back).
+int splitk_workgroups() {
+  static const int v = [] {
+    const char *e = getenv("QCF_SPLITK_WGS");
+    const int x = e ? atoi(e) : 0;
+    return x > 0 ? x : 256;
   }();
   return v;
 }
@@ -661,10 +764,12 @@ int fwd_bwd(qcf_mlp *M, float *loss, hipStream_t s) {
                   float *gb, const bf16 *W, const bf16 *dg_prev, bf16 *d_prev) -> int {
     GemmArgs g{M->dT, aT, (int)hout, (int)hin, (int)B, (int)B, (int)B, (int)hin,
                gW, nullptr, nullptr, nullptr, nullptr, nullptr};
-    // few output tiles, long K: split K so that ~512 workgroups run
+    // few output tiles, long K: split K so that ~splitk_workgroups() run
     const int tiles = (int)((hout / kTile) * (hin / kTile));
     int sk = 1;
-    while (sk * tiles < 512 && (int)B % (2 * sk * 256) == 0 && 2 * sk <= kMaxSplitK) sk *= 2;
+    while (sk * tiles < splitk_workgroups() && (int)B % (2 * sk * 256) == 0 &&
+           2 * sk <= kMaxSplitK)
+      sk *= 2;
     QCF_TRY((launch_gemm<true, true, kEpiStoreF32>(g, s, sk, M->kpart)));
     QCF_LAUNCH(row_sums, dim3(hout), dim3(256), 0, s, M->dT, B, gb);
     QCF_HIP_CHECK(hipGetLastError());
