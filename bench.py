@@ -27,9 +27,9 @@ for _p in (ROOT, PKG):
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per shot of the d=5 x 25 launch from rocprofv3 --pmc (profiles/
 # pf_jit_kernel_d5x25_traffic.md): WRITE_SIZE 1,496,639.5 KiB and FETCH_SIZE
-# 287.6 KiB per 10^7-shot dispatch, scaled by what the counters read for a
+# 217.8 KiB per 10^7-shot dispatch, scaled by what the counters read for a
 # 1 GiB copy (WRITE_SIZE 1.0x, FETCH_SIZE 0.5x of the bytes moved).
-TRAFFIC_BYTES_PER_SHOT = (1496639.5 * 1.0 + 287.6 * 2.0) * 1024 / 1e7
+TRAFFIC_BYTES_PER_SHOT = (1496639.5 * 1.0 + 217.8 * 2.0) * 1024 / 1e7
 INT_VALU_PEAK = 256 * 64 * 2.4e9  # int32 lane-ops/s (256 CUs x 64 lanes x 2.4 GHz)
 METRIC = "batched vs per-circuit shots/s, d=5 surface 25 rds; NN vs MWPM LER"
 
@@ -393,6 +393,29 @@ def bench_mlp_train(steps: int):
                 gflop_per_step=flops_step / 1e9)
 
 
+def bench_gemm(M=8192, N=1024, K=1024, reps=50):
+    """The decoder's GEMM kernel alone on a plain product (fp32 output, no
+    epilogue work): C = A B^T with A (M x K) and B (N x K) in BF16, timed
+    over `reps` launches (for counter runs)."""
+    import ctypes as C
+    import torch
+    from qcf import _lib
+    g = torch.Generator(device="cuda").manual_seed(0)
+    A = (torch.randn(M, K, device="cuda", generator=g) * 0.1).to(torch.bfloat16)
+    B = (torch.randn(N, K, device="cuda", generator=g) * 0.1).to(torch.bfloat16)
+    Cm = torch.empty(M, N, device="cuda")
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def go():
+        _lib.call("qcf_debug_gemm_bf16", 0, M, N, K, C.c_void_p(A.data_ptr()),
+                  C.c_void_p(B.data_ptr()), C.c_void_p(Cm.data_ptr()), st)
+
+    ms = time_launches(go, reps, 3)
+    err = float((Cm - A.float() @ B.float().t()).abs().max())
+    return dict(M=M, N=N, K=K, us=ms * 1e3, tflops=2 * M * N * K / (ms * 1e-3) / 1e12,
+                max_abs_err_vs_fp32=err)
+
+
 def bench_mlp(steps: int, eval_shots: int):
     """Perceptron decoder (600 -> 1024 -> 1024 -> 512 -> 1, BF16 on the matrix
     cores) trained for `steps` batches of 8192 device-sampled d=5 x 25 shots
@@ -486,7 +509,7 @@ def main():
     ap.add_argument("--mlp-steps", type=int, default=24414)  # 2e8 shots of 8192
     ap.add_argument("--mlp-eval-shots", type=int, default=10**7)
     ap.add_argument("--only", choices=("reference_circuits", "canonicalize", "mul_left_1gq",
-                                       "mul_config", "batched", "mlp_train"),
+                                       "mul_config", "batched", "mlp_train", "gemm"),
                     help="run one extra alone and print its JSON (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -502,7 +525,8 @@ def main():
               "mul_left_1gq": lambda: bench_mul(10, 2),
               "mul_config": lambda: bench_mul_config(10, 2),
               "batched": lambda: bench_batched(dmax=a.batched_dmax),
-              "mlp_train": lambda: bench_mlp_train(min(a.mlp_steps, 2000))}[a.only]
+              "mlp_train": lambda: bench_mlp_train(min(a.mlp_steps, 2000)),
+              "gemm": lambda: bench_gemm(reps=a.steps if a.steps < 200 else 50)}[a.only]
         print(json.dumps({a.only: fn()}), flush=True)
         return
     log("frames: d=5 x 25 rounds, p=1e-3 ...")
