@@ -414,6 +414,25 @@ int qcf_debug_jit_source(const qcf_program *prog, int variant, char *buf,
 int qcf_debug_jit_compile(const qcf_program *prog, int variant,
                           size_t *code_bytes);
 
+/* The sampler's two 32-bit generators run on the device from given states,
+ * `k` outputs per state into out[k * i + j] (self-checks against published
+ * known answers):
+ *   qcf_debug_xoshiro128ss: the per-lane noise-event stream, xoshiro128**
+ *     (Blackman and Vigna) from state[4 i .. 4 i + 3] = s0..s3;
+ *   qcf_debug_gauge: the gauge words from (start, increment) =
+ *     state[2 i], state[2 i + 1]: output j is fmix32(start + (j + 1) *
+ *     increment), fmix32 the MurmurHash3 finalizer;
+ *   qcf_debug_gauge_stream: the gauge words of absolute 32-shot word
+ *     words[i] under `seed`, as initZ! and the resets draw them: start and
+ *     increment from Philox4x32-10(counter (word, 0, 0, 0x47415547), key
+ *     (seed low, seed high)) words x and y, the increment made odd and, when
+ *     it has fewer than 12 bit transitions, XORed with 0xAAAAAAAA. */
+int qcf_debug_xoshiro128ss(const uint32_t *state, uint32_t *out, size_t n, uint32_t k,
+                           void *stream);
+int qcf_debug_gauge(const uint32_t *state, uint32_t *out, size_t n, uint32_t k, void *stream);
+int qcf_debug_gauge_stream(uint64_t seed, const uint32_t *words, uint32_t *out, size_t n,
+                           uint32_t k, void *stream);
+
 /* Inclusive prefix sums over each group of 64 words of `in`, computed by the
  * wave-wide scan the sampler uses to place noise events (self-check). */
 int qcf_debug_wave_scan(const uint32_t *in, uint32_t *out, size_t nwaves,

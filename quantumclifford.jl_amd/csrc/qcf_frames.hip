@@ -370,6 +370,32 @@ __global__ void philox_kernel(const uint32_t *in, uint32_t *out, size_t n) {
   out[4 * i + 3] = r.w;
 }
 
+// ---- generator self-check kernels: the per-lane event stream (xoshiro128**)
+// and the gauge words (Weyl sequence + murmur3 finalizer), each run from a
+// given state for `k` outputs
+__global__ void xoshiro_kernel(const uint32_t *state, uint32_t *out, size_t n, uint32_t k) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t *s = state + 4 * i;
+  qcfd::Xoshiro x{s[0], s[1], s[2], s[3]};
+  for (uint32_t j = 0; j < k; ++j) out[(size_t)k * i + j] = x.next();
+}
+
+__global__ void gauge_kernel(const uint32_t *state, uint32_t *out, size_t n, uint32_t k) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  qcfd::Gauge g{state[2 * i], state[2 * i + 1]};
+  for (uint32_t j = 0; j < k; ++j) out[(size_t)k * i + j] = g.next();
+}
+
+__global__ void gauge_stream_kernel(uint64_t seed, const uint32_t *words, uint32_t *out,
+                                    size_t n, uint32_t k) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  qcfd::Gauge g = qcfd::gauge_stream(seed, words[i]);
+  for (uint32_t j = 0; j < k; ++j) out[(size_t)k * i + j] = g.next();
+}
+
 // ---- wave scan self-check kernel (one wave per 64 inputs)
 __global__ __launch_bounds__(64) void scan_kernel(const uint32_t *in, uint32_t *out) {
   const uint32_t i = blockIdx.x * 64u + threadIdx.x;
@@ -731,6 +757,36 @@ extern "C" int qcf_debug_philox(const uint32_t *in, uint32_t *out, size_t n,
   if (n == 0) return QCF_OK;
   QCF_LAUNCH(philox_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, in, out, n);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+extern "C" int qcf_debug_xoshiro128ss(const uint32_t *state, uint32_t *out, size_t n,
+                                      uint32_t k, void *stream) {
+  clear_error();
+  if (n == 0 || k == 0) return QCF_OK;
+  QCF_LAUNCH(xoshiro_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+             (hipStream_t)stream, state, out, n, k);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+extern "C" int qcf_debug_gauge(const uint32_t *state, uint32_t *out, size_t n, uint32_t k,
+                               void *stream) {
+  clear_error();
+  if (n == 0 || k == 0) return QCF_OK;
+  QCF_LAUNCH(gauge_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+             (hipStream_t)stream, state, out, n, k);
+  QCF_HIP_CHECK(hipGetLastError());
+  return QCF_OK;
+}
+
+extern "C" int qcf_debug_gauge_stream(uint64_t seed, const uint32_t *words, uint32_t *out,
+                                      size_t n, uint32_t k, void *stream) {
+  clear_error();
+  if (n == 0 || k == 0) return QCF_OK;
+  QCF_LAUNCH(gauge_stream_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+             (hipStream_t)stream, seed, words, out, n, k);
   QCF_HIP_CHECK(hipGetLastError());
   return QCF_OK;
 }

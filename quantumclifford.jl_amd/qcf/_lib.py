@@ -97,6 +97,9 @@ _SIGNATURES = {
     "qcf_autotune": (_I, [_P, _SZ, _P, C.POINTER(Tune), _P]),
     "qcf_debug_philox": (_I, [_P, _P, _SZ, _P]),
     "qcf_debug_wave_scan": (_I, [_P, _P, _SZ, _P]),
+    "qcf_debug_xoshiro128ss": (_I, [_P, _P, _SZ, _U32, _P]),
+    "qcf_debug_gauge": (_I, [_P, _P, _SZ, _U32, _P]),
+    "qcf_debug_gauge_stream": (_I, [_U64, _P, _P, _SZ, _U32, _P]),
     "qcf_debug_check_launch": (_I, [C.c_uint]),
     "qcf_debug_jit_source": (_I, [_P, _I, _P, _SZ, C.POINTER(_SZ)]),
     "qcf_debug_jit_compile": (_I, [_P, _I, C.POINTER(_SZ)]),
