@@ -216,29 +216,34 @@ gemm_bf16(GemmArgs g) {
     // sets: a stage's tiles are requested while the matrix cores work on the
     // two stages before it, so a load has two stages of MFMA time to arrive
     // (one stage ahead, the wave waited on nearly every load).
+    // The refills are unconditional (stages past the end reload the last
+    // one, never stored): with a load under a branch the compiler's wait
+    // counting lost track of which set was in flight across the loop and
+    // waited for both sets before every store, one stage ahead after all.
     TileRegs<TM, BK> ra0, ra1;
     TileRegs<TN, BK> rb0, rb1;
-    load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, kbase, t, ra0);
-    load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, kbase, t, rb0);
-    if (KT > 1) {
-      load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, kbase + BK, t, ra1);
-      load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, kbase + BK, t, rb1);
-    }
-    auto stage = [&](int kt, TileRegs<TM, BK> &ra, TileRegs<TN, BK> &rb) {
+    auto k_of = [&](int kt) { return kbase + (kt < KT ? kt : KT - 1) * BK; };
+    load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, k_of(0), t, ra0);
+    load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, k_of(0), t, rb0);
+    load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, k_of(1), t, ra1);
+    load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, k_of(1), t, rb1);
+    auto stage = [&](int kt, TileRegs<TM, BK> &ra, TileRegs<TN, BK> &rb, bool refill) {
       store_tile<A_KMAJOR, TM, BK>(As[0], t, ra);
       store_tile<B_KMAJOR, TN, BK>(Bs[0], t, rb);
       __syncthreads();
-      if (kt + 2 < KT) {  // refill the set just stored
-        load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, kbase + (kt + 2) * BK, t, ra);
-        load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, kbase + (kt + 2) * BK, t, rb);
+      if (refill) {  // the set just stored takes the stage after next
+        load_tile<A_KMAJOR, TM, BK>(g.A, g.lda, m0, k_of(kt + 2), t, ra);
+        load_tile<B_KMAJOR, TN, BK>(g.B, g.ldb, n0, k_of(kt + 2), t, rb);
       }
       mma_stage(0);
       __syncthreads();
     };
-    for (int kt = 0; kt < KT; kt += 2) {
-      stage(kt, ra0, rb0);
-      if (kt + 1 < KT) stage(kt + 1, ra1, rb1);
+    int kt = 0;
+    for (; kt + 1 < KT; kt += 2) {
+      stage(kt, ra0, rb0, true);
+      stage(kt + 1, ra1, rb1, true);
     }
+    if (kt < KT) stage(kt, ra0, rb0, false);
   } else {
     TileRegs<TM, BK> ra;
     TileRegs<TN, BK> rb;
