@@ -263,10 +263,13 @@ int qcf_program_gauge_planes(const qcf_program *prog, uint8_t *out, size_t n);
  *            planes of `ld` words that replace every random draw in program
  *            order -- the replayed-mask mode.  Planes 0..nqubits-1 are the
  *            initZ! z words of qubits 1..nqubits (read only with
- *            QCF_PF_FRESH); then one plane per sMRZ z-flip, two (x flip,
+ *            QCF_PF_FRESH); then one plane per sMRZ/sMRX z-flip (and per
+ *            PauliMeasurement, which resets its ancilla), two (x flip,
  *            z flip) per single-qubit noise site and four (x1,z1,x2,z2) per
  *            DEPOLARIZE2 site, in op order.  With QCF_PF_EXPORT_MASKS the
- *            native draws are written there instead.
+ *            native draws are written there instead; a fresh launch without
+ *            a frame buffer leaves the planes of the gauge draws it skips
+ *            (qcf_program_gauge_planes) as the caller zeroed them.
  *   ld     : words per plane, >= ceil(shots/64); qcf_plane_words(shots)
  *            is the recommended stride
  *   shot_offset : absolute index of this launch's first shot in the caller's
@@ -282,10 +285,15 @@ int qcf_program_gauge_planes(const qcf_program *prog, uint8_t *out, size_t n);
  *   - noise events: one stream of event positions per absolute 2048-shot
  *     tile along its (site ordinal, shot) sequence, drawn 64 at a time (one
  *     geometric gap per lane from a per-lane xoshiro128** stream seeded by
- *     Philox4x32-10(seed, tile, lane), then a wave prefix sum).
+ *     Philox4x32-10(seed, tile, lane), then a wave prefix sum); per event a
+ *     Pauli-choice draw and, only in programs where some site keeps part of
+ *     its rate class's events, a thinning draw.
  * So the bits of a shot depend on (seed, its absolute tile) and not on the
  * launch shape, batching or kernel: a run split at multiples of 2048 shots
- * (with shot_offset) reproduces the one-run bits.
+ * (with shot_offset) reproduces the one-run bits.  A fresh launch without a
+ * frame buffer makes only the gauge draws that can reach the record (see
+ * qcf_program_gauge_planes): its records have the distribution of a launch
+ * that draws all of them, not the same bits.
  * Workspace: launches whose frames live in global memory (frames beyond
  * what the specialised kernels hold on chip, and replay/export launches of
  * frames over 160 KB) use a library-owned workspace per (device, stream):
