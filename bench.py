@@ -259,11 +259,15 @@ def bench_reference_circuits(steps, warmup):
                           C.c_void_p(rec.data_ptr()), None, ld, 1,
                           C.c_void_p(torch.cuda.current_stream().cuda_stream))
 
-        ms = time_launches(step, steps, warmup)
+        # device time per launch: 20 launches captured in one graph (at 2^19
+        # shots and below a launch from Python takes longer than the kernel)
+        ms_call = time_launches(step, steps, warmup)
+        ms = time_graph(step, 20, max(2, steps))
         sps = shots / (ms * 1e-3)
         out[name] = dict(shots=shots, ms=ms, shots_s=sps, nqubits=prog.nq - 1,
                          record_bits=prog.nbits, bitops_s=prog.bitops_per_shot * sps,
-                         rec_gbs=prog.nbits / 8 * sps / 1e9)
+                         rec_gbs=prog.nbits / 8 * sps / 1e9, ms_per_python_call=ms_call,
+                         timing="device time per launch inside a captured graph of 20")
         log(f"  {name}: {out[name]}")
         del rec
     return out

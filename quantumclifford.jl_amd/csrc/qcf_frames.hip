@@ -585,7 +585,15 @@ int jit_plan(qcf_program *P, const KParams &kp, uint64_t shots, int variant, boo
   // (four) passes per tile.
   const bool try_two = split_ok && (global || 2u * P->nq * 256u > 80u * 1024u);
   int js = QCF_ERR_UNSUPPORTED;
-  if (try_two) {
+  // a noiseless fresh launch that returns only the record: the linear kernel
+  // (variant bit 8), which keeps no frame, whatever the frame's size
+  bool linear = false;
+  if (variant == 1 && P->linear_ok && jit_linear_enabled()) {
+    js = jit_get(P, 1 | (v & 8) | 256, &jk);
+    if (js == QCF_OK) linear = true;
+    else clear_error();
+  }
+  if (js != QCF_OK && try_two) {
     js = jit_get(P, v | 16 | 32, &jk);
     if (js != QCF_OK) clear_error();
   }
@@ -602,7 +610,7 @@ int jit_plan(qcf_program *P, const KParams &kp, uint64_t shots, int variant, boo
   }
   if (js != QCF_OK && !try_split) js = jit_get(P, v, &jk);
   QCF_TRY(js);
-  const bool gjit = global && !try_split;  // specialised kernel on global planes
+  const bool gjit = global && !try_split && !linear;  // specialised kernel on global planes
   JitParams jp{};
   jp.rec = kp.rec;
   jp.frame = kp.frame;

@@ -230,6 +230,11 @@ Loop find_loop(const std::vector<Instr> &code) {
 // Instructions a specialised kernel spells out: the stream with its repeated
 // stretch written once.
 size_t emitted_size(const qcf_program *P, int variant) {
+  if (variant & 256) {  // linear mode: one store per column, one XOR per kept draw used
+    size_t n = P->ngauge_live + P->nbits;
+    for (const auto &r : P->rec_gauge) n += r.size();
+    return n;
+  }
   std::vector<Instr> code;
   for (const Instr &in : variant_code(P, variant))
     if ((in.op & 0xFFu) != I_END) code.push_back(in);
@@ -295,8 +300,132 @@ constexpr uint32_t kSplitMaxRegs = 400u;
 // Noise groups (event passes) a split or part-tile kernel may emit.
 constexpr size_t kMaxLargeFrameGroups = 256;
 
+// Pieces of every specialised kernel: the loop over the tiles a workgroup
+// takes (dealt out per XCD, or drawn from a shared counter; in a
+// multi-circuit launch each tile looks up its circuit), the per-tile
+// locals (record addressing, gauge stream), and the end of the kernel.
+const char *const kTileLoopHead = R"QCFJIT(
+  const uint32_t ntiles = P0.ncirc ? P0.ntiles : P0.nblocks;
+  const uint32_t nx = P0.xcds ? P0.xcds : 1u;
+  const uint32_t per = gridDim.x / nx, xi = blockIdx.x % nx, wi = blockIdx.x / nx;
+  const uint32_t span = (ntiles + nx - 1u) / nx;
+  for (uint32_t t = wi;; t += per) {
+    uint32_t tile;
+    if (P0.steal) {  // shared with the persistent batch kernel: take the next tile
+      uint32_t v = 0u;
+      if (l == 0u) v = atomicAdd(P0.steal, 1u);
+      tile = __builtin_amdgcn_readfirstlane(v);
+    } else {
+      if (t >= span) break;
+      tile = xi * span + t;
+    }
+    if (tile >= ntiles) break;
+    // the circuit of this tile in a multi-circuit launch: the last i with
+    // first[i] <= tile (uniform, so scalar loads, which never wait behind
+    // the vector record stores)
+    KJ P = P0;
+    uint32_t blk = tile;
+    if (P0.ncirc) {
+      // the table is read-only during the launch: constant address space
+      typedef __attribute__((address_space(4))) const uint32_t cu32;
+      typedef __attribute__((address_space(4))) const KJ cKJ;
+      const cu32 *const first = (const cu32 *)P0.first;
+      uint32_t lo = 0u, hi = P0.ncirc;
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (first[mid] <= tile) lo = mid;
+        else hi = mid;
+      }
+      const cKJ *const Q = (const cKJ *)P0.multi + lo;
+      P.rec = Q->rec;
+      P.frame = Q->frame;
+      P.seed = Q->seed;
+      P.ld32 = Q->ld32;
+      P.nw32 = Q->nw32;
+      P.nwbuf = Q->nwbuf;
+      P.lastmask = Q->lastmask;
+      P.c = Q->c;
+      P.has_class = Q->has_class;
+      P.nfull = Q->nfull;
+      P.tile0 = Q->tile0;
+      blk = tile - first[lo];
+    }
+)QCFJIT";
+const char *const kTileLocals = R"QCFJIT(
+    // an opaque per-tile copy of the plane stride keeps the compiler from
+    // hoisting every record address out of the tile loop
+    uint64_t ld = P.ld32;
+    asm volatile("" : "+s"(ld));
+    uint32_t ld4 = (uint32_t)ld * 4u;  // plane stride in bytes (rec32 kernels)
+    asm volatile("" : "+s"(ld4));
+    char *const recb = (char *)P.rec;
+    const uint32_t gw4 = gw * 4u;
+    (void)ld4; (void)recb; (void)gw4;
+    qcfd::Gauge g = qcfd::gauge_stream(P.seed, gw + P.tile0 * 64u);
+)QCFJIT";
+const char *const kTileLoopTail = R"QCFJIT(  }
+  if (P0.sync && l == 0u) {  // the last workgroup of the sharing launches clears the counters
+    __threadfence();
+    if (atomicAdd(&P0.sync[1], 1u) == P0.sync_total - 1u)
+      for (uint32_t i = 0; i < P0.sync_words; ++i)
+        __hip_atomic_store(&P0.sync[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+)QCFJIT";
+
+// Linear mode (variant bit 8): a noiseless program run fresh, returning only
+// the record.  Its record is then linear over GF(2) in the gauge draws it
+// makes (the kept ones, see mark_dead_gauges): column b is the XOR of the
+// kept draws in rec_gauge[b].  The kernel draws those words from the same
+// per-word gauge stream in the same order as the frame kernels and the
+// interpreter, and stores every column's XOR -- no frame at all, so the same
+// bits in a fraction of the work (the reference's d=11 benchmark circuit: 220
+// words and 2,928 XORs for 2,751 columns per 32-shot word, where the frame
+// runs 17,802 operations over 884 planes).
+bool generate_linear(const qcf_program *P, int variant, std::string *out, uint32_t *lds_bytes) {
+  if (!P->linear_ok || (variant & 3) != 1 || P->rec_gauge.size() != P->nbits) return false;
+  const bool rec32 = (variant & 8) != 0;
+  const bool nt = jit_nt_stores();
+  size_t nnz = 0;
+  for (const auto &r : P->rec_gauge) nnz += r.size();
+  std::ostringstream body;
+  for (uint32_t k = 0; k < P->ngauge_live; ++k)
+    body << "      const uint32_t G" << k << " = g.next();\n";
+  for (uint32_t col = 0; col < P->nbits; ++col) {
+    std::string v;
+    for (uint32_t x : P->rec_gauge[col]) v += (v.empty() ? "G" : " ^ G") + std::to_string(x);
+    if (v.empty()) v = "0u";
+    const std::string m =
+        rec32 ? "*(uint32_t *)(recb + (uint32_t)(" + std::to_string(col) + "u * ld4 + gw4))"
+              : "P.rec[(uint64_t)" + std::to_string(col) + "u * ld + gw]";
+    const std::string val = "(" + v + ") & lm";
+    body << "      " << (nt ? "__builtin_nontemporal_store(" + val + ", &" + m + ");"
+                           : m + " = " + val + ";")
+         << "\n";
+  }
+  std::ostringstream src;
+  src << "#define QCF_RTC 1\n// linear: " << P->ngauge_live << " gauge words, " << nnz
+      << " XORs, " << P->nbits << " record columns\n";
+  src << kDeviceSrc << kKernelPrelude;
+  src << "extern \"C\" __global__ void __launch_bounds__(64) qcf_jit_kernel(KJ P0) {\n"
+      << "  const uint32_t l = threadIdx.x;\n";
+  src << kTileLoopHead;
+  src << "    const uint32_t gw = blk * 64u + l;\n";
+  src << kTileLocals;
+  // lanes inside the plane stride store (zeros past the shots)
+  src << "    const bool live = gw < P.nwbuf;\n"
+      << "    const uint32_t lm = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1u ? P.lastmask : "
+         "0xFFFFFFFFu);\n"
+      << "    if (live) {\n" << body.str() << "    }\n";
+  src << kTileLoopTail;
+  *out = src.str();
+  if (lds_bytes) *lds_bytes = 0;
+  return true;
+}
+
 bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds_bytes) {
   // the event stream's thinning draw, a constant of the kernel
+  if (variant & 256) return generate_linear(P, variant, out, lds_bytes);
   const char *const thin = P->thin ? "1u" : "0u";
   std::vector<Instr> code;
   for (const Instr &in : variant_code(P, variant))
@@ -933,53 +1062,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   // X, workgroup i runs on XCD i mod X, and XCD x takes the x-th contiguous
   // run of tiles, so each XCD's L2 writes whole contiguous stretches of every
   // record column.
-  src << R"QCFJIT(
-  const uint32_t ntiles = P0.ncirc ? P0.ntiles : P0.nblocks;
-  const uint32_t nx = P0.xcds ? P0.xcds : 1u;
-  const uint32_t per = gridDim.x / nx, xi = blockIdx.x % nx, wi = blockIdx.x / nx;
-  const uint32_t span = (ntiles + nx - 1u) / nx;
-  for (uint32_t t = wi;; t += per) {
-    uint32_t tile;
-    if (P0.steal) {  // shared with the persistent batch kernel: take the next tile
-      uint32_t v = 0u;
-      if (l == 0u) v = atomicAdd(P0.steal, 1u);
-      tile = __builtin_amdgcn_readfirstlane(v);
-    } else {
-      if (t >= span) break;
-      tile = xi * span + t;
-    }
-    if (tile >= ntiles) break;
-    // the circuit of this tile in a multi-circuit launch: the last i with
-    // first[i] <= tile (uniform, so scalar loads, which never wait behind
-    // the vector record stores)
-    KJ P = P0;
-    uint32_t blk = tile;
-    if (P0.ncirc) {
-      // the table is read-only during the launch: constant address space
-      typedef __attribute__((address_space(4))) const uint32_t cu32;
-      typedef __attribute__((address_space(4))) const KJ cKJ;
-      const cu32 *const first = (const cu32 *)P0.first;
-      uint32_t lo = 0u, hi = P0.ncirc;
-      while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (first[mid] <= tile) lo = mid;
-        else hi = mid;
-      }
-      const cKJ *const Q = (const cKJ *)P0.multi + lo;
-      P.rec = Q->rec;
-      P.frame = Q->frame;
-      P.seed = Q->seed;
-      P.ld32 = Q->ld32;
-      P.nw32 = Q->nw32;
-      P.nwbuf = Q->nwbuf;
-      P.lastmask = Q->lastmask;
-      P.c = Q->c;
-      P.has_class = Q->has_class;
-      P.nfull = Q->nfull;
-      P.tile0 = Q->tile0;
-      blk = tile - first[lo];
-    }
-)QCFJIT";
+  src << kTileLoopHead;
   // part-tile mode: 64/pw passes over the tile, each with the shot words of
   // one part in lanes 0..pw-1 (the event stream is drawn for the whole tile
   // in every pass); passes past the plane stride hold no shots and end it
@@ -991,18 +1074,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
         << "    const FlipPart<" << pw << "> flip{L, hf};\n";
   else
     src << "    const uint32_t gw = blk * 64u + l;\n";
-  src << R"QCFJIT(
-    // an opaque per-tile copy of the plane stride keeps the compiler from
-    // hoisting every record address out of the tile loop
-    uint64_t ld = P.ld32;
-    asm volatile("" : "+s"(ld));
-    uint32_t ld4 = (uint32_t)ld * 4u;  // plane stride in bytes (rec32 kernels)
-    asm volatile("" : "+s"(ld4));
-    char *const recb = (char *)P.rec;
-    const uint32_t gw4 = gw * 4u;
-    (void)ld4; (void)recb; (void)gw4;
-    qcfd::Gauge g = qcfd::gauge_stream(P.seed, gw + P.tile0 * 64u);
-)QCFJIT";
+  src << kTileLocals;
   src << "    qcfd::EvState ev = qcfd::ev_init(P.seed, blk + P.tile0, l);\n"
          "    if (P.has_class) ev = qcfd::gen_batch(ev, P.c, " << thin << ");\n"
          "    uint32_t acc = 0u;\n    (void)acc;\n";
@@ -1018,15 +1090,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
         << "    const uint32_t lm = gw >= P.nw32 ? 0u : (gw == P.nw32 - 1u ? P.lastmask : "
            "0xFFFFFFFFu);\n"
         << part.str() << "    }\n";
-  src << R"QCFJIT(  }
-  if (P0.sync && l == 0u) {  // the last workgroup of the sharing launches clears the counters
-    __threadfence();
-    if (atomicAdd(&P0.sync[1], 1u) == P0.sync_total - 1u)
-      for (uint32_t i = 0; i < P0.sync_words; ++i)
-        __hip_atomic_store(&P0.sync[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-)QCFJIT";
+  src << kTileLoopTail;
   *out = src.str();
   if (lds_bytes)
     *lds_bytes = glob    ? 0u
@@ -1173,6 +1237,13 @@ uint32_t jit_hoist() {
   return e ? (uint32_t)strtoul(e, nullptr, 10) : 64u;
 }
 
+// QCF_JIT_LINEAR=0 runs noiseless record-only launches on the frame kernels
+// instead of the linear one.
+bool jit_linear_enabled() {
+  const char *e = getenv("QCF_JIT_LINEAR");
+  return !(e && e[0] == '0');
+}
+
 // QCF_JIT_PREFETCH=0 leaves the first site-entry permute of every noise pass
 // at the pass instead of after the previous pass.
 bool jit_prefetch() {
@@ -1287,7 +1358,7 @@ extern "C" int qcf_debug_jit_source(const qcf_program *prog, int variant, char *
   qcf::clear_error();
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src;
-  if (!qcf::generate(prog, variant & 255, &src, nullptr))
+  if (!qcf::generate(prog, variant & 511, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   if (needed) *needed = src.size() + 1;
   if (buf && cap) {
@@ -1303,7 +1374,7 @@ extern "C" int qcf_debug_jit_compile(const qcf_program *prog, int variant, size_
   if (!prog) return qcf::set_error(QCF_ERR_ARGUMENT, "null program");
   std::string src, log;
   std::vector<char> bin;
-  if (!qcf::generate(prog, variant & 255, &src, nullptr))
+  if (!qcf::generate(prog, variant & 511, &src, nullptr))
     return qcf::set_error(QCF_ERR_UNSUPPORTED, "op not specialisable");
   QCF_TRY(qcf::compile(src, &bin, &log));
   if (code_bytes) *code_bytes = bin.size();

@@ -55,6 +55,7 @@ uint32_t jit_slots();
 size_t jit_max_instr();
 uint32_t jit_hoist();
 bool jit_prefetch();
+bool jit_linear_enabled();
 
 struct JitKernelInfo {
   void *function = nullptr;     // hipFunction_t

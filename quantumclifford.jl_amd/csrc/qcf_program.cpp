@@ -680,6 +680,8 @@ void qcf::mark_dead_gauges(qcf_program *P) {
   for (const Instr &in : P->body) k += is_gauge(in);
   P->ngauge = (uint32_t)k;
   P->ngauge_live = (uint32_t)k;
+  P->linear_ok = false;
+  P->rec_gauge.clear();
   if (k == 0) return;
   const size_t W = (k + 63) / 64;           // words per gauge bitset
   const size_t NB = std::max<uint32_t>(P->nbits, 1u);
@@ -820,6 +822,23 @@ void qcf::mark_dead_gauges(qcf_program *P) {
     if (is_gauge(P->body[i])) P->gauge_dead_body[i] = !live[j++];
   for (uint8_t x : live) nlive += x;
   P->ngauge_live = (uint32_t)nlive;
+  // noiseless: record column b = XOR of the kept draws among those reaching it
+  P->linear_ok = false;
+  P->rec_gauge.clear();
+  if (P->sites.empty()) {
+    std::vector<uint32_t> pos(k, 0);
+    for (size_t i = 0, n = 0; i < k; ++i) pos[i] = live[i] ? (uint32_t)n++ : 0u;
+    P->rec_gauge.assign(P->nbits, {});
+    for (uint32_t b = 0; b < P->nbits; ++b) {
+      const uint64_t *r = RCp(b);
+      for (size_t w = 0; w < W; ++w)
+        for (uint64_t m = r[w]; m; m &= m - 1) {
+          const size_t i = w * 64 + __builtin_ctzll(m);
+          if (live[i]) P->rec_gauge[b].push_back(pos[i]);
+        }
+    }
+    P->linear_ok = true;
+  }
 }
 
 extern "C" int qcf_program_build(const qcf_op *ops, size_t nops,

@@ -120,6 +120,11 @@ struct qcf_program {
   // record-only run skips (empty when the analysis was not run)
   std::vector<uint8_t> gauge_dead_pro, gauge_dead_body;
   uint32_t ngauge = 0, ngauge_live = 0;  // gauge draws of a fresh run; those kept
+  // noiseless programs (no noise site): the record of a fresh record-only run
+  // is linear in the kept gauge words -- rec_gauge[b] lists, for record
+  // column b, the positions (in draw order) of the kept draws it XORs
+  bool linear_ok = false;
+  std::vector<std::vector<uint32_t>> rec_gauge;
   std::vector<qcf::Site> sites;
   std::vector<qcf::ClassInfo> classes;
   // some site keeps only part of its class's events (0 < accept < 2^31): the

@@ -200,3 +200,56 @@ def test_many_bit_xor_and_pauli_measurement_specialised_equals_interpreter(gpu, 
     (ra, fa), (rb, fb) = _both(monkeypatch, run)
     assert np.array_equal(ra, rb) and np.array_equal(fa, fb)
     assert oracle.unpack_shots(ra, shots)[20].any()
+
+
+def _three(monkeypatch, run):
+    """Linear kernel (the default for noiseless record-only launches), frame
+    kernel (QCF_JIT_LINEAR=0) and interpreter, in that order."""
+    monkeypatch.setenv("QCF_REQUIRE_JIT", "1")
+    monkeypatch.delenv("QCF_JIT_LINEAR", raising=False)
+    lin = run()
+    monkeypatch.setenv("QCF_JIT_LINEAR", "0")
+    frame = run()
+    monkeypatch.delenv("QCF_JIT_LINEAR")
+    monkeypatch.delenv("QCF_REQUIRE_JIT")
+    monkeypatch.setenv("QCF_NO_JIT", "1")
+    interp = run()
+    monkeypatch.delenv("QCF_NO_JIT")
+    torch.cuda.synchronize()
+    return lin, frame, interp
+
+
+@pytest.mark.parametrize("shots", [1, 2047, 3 * 2048 + 77])
+def test_linear_kernel_on_the_reference_benchmark_circuit(gpu, monkeypatch, shots):
+    """The noiseless d=11 benchmark circuit (benchmark/surface-code-circuit.jl),
+    record only: the linear kernel, the split frame kernel and the interpreter
+    give the same bits, and the stabilizer checks of every round after the
+    first are deterministic (the record's gauge part repeats)."""
+    prog = qcf.Program(circuits.unrotated_surface_code_benchmark(11, 12))
+
+    def run():
+        return qcf.pftrajectories(prog, shots, seed=123, keep_frame=False).record_host()
+
+    lin, frame, interp = _three(monkeypatch, run)
+    assert np.array_equal(lin, frame)
+    assert np.array_equal(lin, interp)
+    assert oracle.unpack_shots(lin, shots).any()  # gauge-random outcomes exist
+
+
+@pytest.mark.parametrize("shots", [100, 2049])
+def test_linear_kernel_on_random_noiseless_circuits(gpu, monkeypatch, shots):
+    """Every gate, sMZ/sMRZ/sMX/sMRX, PauliMeasurement and classical XOR
+    (noise ops removed), record only."""
+    rng = np.random.default_rng(900 + shots)
+    noise = (qcf.ops.NoiseOp, qcf.ops.NoiseOpAll, qcf.ops.NoisyGate)
+    for trial in range(4):
+        n, nbits = int(rng.integers(2, 9)), int(rng.integers(1, 12))
+        circ = [op for op in random_circuit(rng, n, nbits, 150) if not isinstance(op, noise)]
+        prog = qcf.Program(circ, nqubits=n, nbits=nbits)
+
+        def run():
+            return qcf.pftrajectories(prog, shots, seed=trial, keep_frame=False).record_host()
+
+        lin, frame, interp = _three(monkeypatch, run)
+        assert np.array_equal(lin, frame), trial
+        assert np.array_equal(lin, interp), trial

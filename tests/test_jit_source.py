@@ -114,3 +114,31 @@ def test_large_frames_with_many_noise_groups_stay_on_the_interpreter():
     for variant in (1 | 4 | 8 | 16, 1 | 4 | 8 | 64, 1 | 4 | 8 | 128):
         with pytest.raises(_lib.MethodError):
             source(prog, variant)
+
+
+LINEAR = 1 | 8 | 256  # fresh, record only, 32-bit record offsets, linear mode
+
+
+def test_linear_source_for_the_reference_benchmark_circuit():
+    """Variant bit 8: a noiseless fresh record-only launch keeps no frame; its
+    columns are XORs of the kept gauge words (the reference's d=11 benchmark
+    circuit: 220 words, 2,928 XORs, 2,751 columns) and the kernel compiles."""
+    prog = qcf.Program(circuits.unrotated_surface_code_benchmark(11, 12))
+    src = source(prog, LINEAR)
+    assert "// linear: 220 gauge words, 2928 XORs, 2751 record columns" in src
+    kernel = src.split("qcf_jit_kernel")[-1]
+    assert "L[" not in kernel  # no LDS planes
+    assert kernel.count("g.next()") == 220
+    n = C.c_size_t()
+    _lib.call("qcf_debug_jit_compile", prog.handle, LINEAR, C.byref(n))
+    assert n.value > 0
+
+
+def test_linear_mode_needs_a_noiseless_record_only_launch():
+    noisy = qcf.Program(circuits.rotated_surface_code(3, 3, 1e-3).ops)
+    with pytest.raises(_lib.MethodError):
+        source(noisy, LINEAR)
+    clean = qcf.Program(circuits.rotated_surface_code(3, 3, 0.0, noise=False).ops)
+    source(clean, LINEAR)
+    with pytest.raises(_lib.MethodError):  # the frame is returned: no linear mode
+        source(clean, LINEAR | 2)
