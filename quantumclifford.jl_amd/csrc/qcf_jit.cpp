@@ -227,6 +227,27 @@ Loop find_loop(const std::vector<Instr> &code) {
   return best;
 }
 
+// The instruction stream a specialised kernel runs: the variant's code
+// without the chunk padding, and without the noise groups when no site has a
+// nonzero rate (a specialised kernel draws natively, so such sites never
+// fire: their event passes and, in register mode, their slot exchanges
+// would be dead work).
+std::vector<Instr> kernel_code(const qcf_program *P, int variant) {
+  const bool silent = P->classes.empty();
+  std::vector<Instr> code;
+  const std::vector<Instr> all = variant_code(P, variant);
+  for (size_t i = 0; i < all.size(); ++i) {
+    const uint32_t op = all[i].op & 0xFFu;
+    if (op == I_END) continue;
+    if (silent && op == I_NOISEG) {
+      i += all[i].op >> 8;  // the group's site entries
+      continue;
+    }
+    code.push_back(all[i]);
+  }
+  return code;
+}
+
 // Instructions a specialised kernel spells out: the stream with its repeated
 // stretch written once.
 size_t emitted_size(const qcf_program *P, int variant) {
@@ -235,9 +256,7 @@ size_t emitted_size(const qcf_program *P, int variant) {
     for (const auto &r : P->rec_gauge) n += r.size();
     return n;
   }
-  std::vector<Instr> code;
-  for (const Instr &in : variant_code(P, variant))
-    if ((in.op & 0xFFu) != I_END) code.push_back(in);
+  const std::vector<Instr> code = kernel_code(P, variant);
   const Loop lp = jit_no_loops() ? Loop{} : find_loop(code);
   return lp.R >= 2 ? code.size() - (lp.R - 1) * lp.L : code.size();
 }
@@ -427,9 +446,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   // the event stream's thinning draw, a constant of the kernel
   if (variant & 256) return generate_linear(P, variant, out, lds_bytes);
   const char *const thin = P->thin ? "1u" : "0u";
-  std::vector<Instr> code;
-  for (const Instr &in : variant_code(P, variant))
-    if ((in.op & 0xFFu) != I_END) code.push_back(in);  // chunk padding
+  const std::vector<Instr> code = kernel_code(P, variant);
   Loop lp = jit_no_loops() ? Loop{} : find_loop(code);
   if (lp.R < 2) lp = Loop{0, 0, 0, 0, 0};
   // emitted ranges: [0, s), loop body [s, s+L), [s+R*L, n)

@@ -822,10 +822,12 @@ void qcf::mark_dead_gauges(qcf_program *P) {
     if (is_gauge(P->body[i])) P->gauge_dead_body[i] = !live[j++];
   for (uint8_t x : live) nlive += x;
   P->ngauge_live = (uint32_t)nlive;
-  // noiseless: record column b = XOR of the kept draws among those reaching it
+  // no site with a nonzero rate (none at all, or every one at zero, which
+  // native draws never fire): record column b = XOR of the kept draws among
+  // those reaching it
   P->linear_ok = false;
   P->rec_gauge.clear();
-  if (P->sites.empty()) {
+  if (P->classes.empty()) {
     std::vector<uint32_t> pos(k, 0);
     for (size_t i = 0, n = 0; i < k; ++i) pos[i] = live[i] ? (uint32_t)n++ : 0u;
     P->rec_gauge.assign(P->nbits, {});

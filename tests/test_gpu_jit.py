@@ -237,6 +237,35 @@ def test_linear_kernel_on_the_reference_benchmark_circuit(gpu, monkeypatch, shot
 
 
 @pytest.mark.parametrize("shots", [100, 2049])
+def test_zero_rate_noise_runs_as_noiseless(gpu, monkeypatch, shots):
+    """Rotated d=5 x 5 with every noise op at rate 0: the linear kernel, the
+    frame kernel (no event passes compiled in) and the interpreter give the
+    same records; with the frame returned, the frame kernel equals the
+    interpreter on both record and frame."""
+    prog = qcf.Program(circuits.rotated_surface_code(5, 5, 0.0).ops)
+
+    def run():
+        return qcf.pftrajectories(prog, shots, seed=77, keep_frame=False).record_host()
+
+    lin, frame, interp = _three(monkeypatch, run)
+    assert np.array_equal(lin, frame)
+    assert np.array_equal(lin, interp)
+
+    def run_frame():
+        r = qcf.pftrajectories(prog, shots, seed=78)
+        return r.record_host(), r.frame_host()
+
+    monkeypatch.setenv("QCF_REQUIRE_JIT", "1")
+    jr, jf = run_frame()
+    monkeypatch.delenv("QCF_REQUIRE_JIT")
+    monkeypatch.setenv("QCF_NO_JIT", "1")
+    ir, iff = run_frame()
+    monkeypatch.delenv("QCF_NO_JIT")
+    assert np.array_equal(jr, ir)
+    assert np.array_equal(jf, iff)
+
+
+@pytest.mark.parametrize("shots", [100, 2049])
 def test_linear_kernel_on_random_noiseless_circuits(gpu, monkeypatch, shots):
     """Every gate, sMZ/sMRZ/sMX/sMRX, PauliMeasurement and classical XOR
     (noise ops removed), record only."""

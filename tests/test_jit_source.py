@@ -142,3 +142,19 @@ def test_linear_mode_needs_a_noiseless_record_only_launch():
     source(clean, LINEAR)
     with pytest.raises(_lib.MethodError):  # the frame is returned: no linear mode
         source(clean, LINEAR | 2)
+
+
+def test_zero_rate_noise_compiles_like_no_noise():
+    """Noise ops at rate 0 keep their sites (mask replay needs them), but
+    native draws never fire there: the specialised kernels carry no event
+    passes or slot exchanges for them, and the record-only launch takes the
+    linear kernel with the same XORs as the circuit without noise ops."""
+    zero = qcf.Program(circuits.rotated_surface_code(5, 25, 0.0).ops)
+    clean = qcf.Program(circuits.rotated_surface_code(5, 25, 0.0, noise=False).ops)
+    assert zero.info.nsites > 0 and zero.info.nclasses == 0
+    for v in (1 | 8, 2 | 8, 3 | 8):  # fresh record-only, in place + frame, fresh + frame
+        kernel = source(zero, v).split("qcf_jit_kernel")[-1]
+        assert "group_events" not in kernel and "atomicExch" not in kernel
+    lz, lc = source(zero, LINEAR), source(clean, LINEAR)
+    assert lz.splitlines()[1] == lc.splitlines()[1] == \
+        "// linear: 24 gauge words, 565 XORs, 1226 record columns"
