@@ -111,7 +111,9 @@ def time_graph(fn, per_graph, replays, warmup=1):
     torch.cuda.current_stream().wait_stream(s)
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
+    # captured on the warm-up stream: per-stream workspaces (global-memory
+    # frames) were sized by the warm-up calls and cannot grow under capture
+    with torch.cuda.graph(g, stream=s):
         for _ in range(per_graph):
             fn()
     g.replay()
@@ -237,8 +239,9 @@ def bench_reference_circuits(steps, warmup):
     """The reference's own frame-benchmark circuits: the noiseless 441-qubit
     unrotated surface code of benchmark/surface-code-circuit.jl (d=11, 12
     rounds, 2751 record bits; benchmark-surface-code-gpu.jl times 2^3..2^19
-    trajectories) at 2^19 and 10^7 shots, and x_diag_circuit(1000) at 1000
-    trajectories (benchmarks.jl q1001_r10000)."""
+    trajectories) at 2^19 and 10^7 shots, x_diag_circuit(1000) at 1000
+    trajectories (benchmarks.jl q1001_r10000) and the compactification case,
+    x_diag_circuit_noisy_measurement(1000) at 1000 trajectories."""
     import ctypes as C
     import torch
     import qcf
@@ -246,7 +249,9 @@ def bench_reference_circuits(steps, warmup):
     out = {}
     cases = (("surface_code_d11_2pow19", circuits.unrotated_surface_code_benchmark(11, 12), 2**19),
              ("surface_code_d11_1e7", circuits.unrotated_surface_code_benchmark(11, 12), 10**7),
-             ("x_diag_1000_1000shots", circuits.x_diag_circuit(1000), 1000))
+             ("x_diag_1000_1000shots", circuits.x_diag_circuit(1000), 1000),
+             ("x_diag_noisy_measurement_1000_1000shots",
+              circuits.x_diag_circuit_noisy_measurement(1000), 1000))
     for name, ops, shots in cases:
         prog = qcf.Program(ops)
         ld = qcf.frames.plane_words(shots)

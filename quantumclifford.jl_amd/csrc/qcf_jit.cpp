@@ -275,42 +275,6 @@ size_t emitted_size(const qcf_program *P, int variant) {
 // exchange-with-zero per plane (so the buffer is zero again for the next
 // group).  Longer programs keep the planes in LDS (L[p*64+l]) as the
 // interpreter does and the events XOR straight into them.
-// Planes instruction code[i] reads or writes (a noise group: its sites').
-void instr_planes(const std::vector<Instr> &code, size_t i, std::vector<uint32_t> *out) {
-  const Instr &in = code[i];
-  auto use = [&](uint32_t p) {
-    if (p != kNone) out->push_back(p);
-  };
-  switch (in.op & 0xFFu) {
-    case I_XORN: {
-      const uint32_t *w = &in.a;
-      for (uint32_t k = 0; k < (in.op >> 8); ++k) use(w[k] >> 16), use(w[k] & 0xFFFFu);
-      break;
-    }
-    case I_NOISEG:
-      for (uint32_t k = 1; k <= (in.op >> 8) && i + k < code.size(); ++k) {
-        const Instr &s = code[i + k];
-        use(s.a & 0xFFFFu), use(s.a >> 16);
-        if (s.b) use(s.b & 0xFFFFu), use(s.b >> 16);
-      }
-      break;
-    case I_MR: use(in.a), use(in.b), use(in.e); break;
-    case I_MRX: use(in.a), use(in.b), use(in.e), use(in.f); break;
-    case I_XSTORE2: use(in.a), use(in.b), use(in.d); break;
-    case I_MEAS: use(in.a), use(in.c); break;
-    case I_RESETZ:
-    case I_INIT:
-    case I_LIN1: use(in.a), use(in.b); break;
-    case I_ACC_LDS:
-    case I_ACCX_LDS:
-    case I_LOAD:
-    case I_STORE:
-    case I_ZERO: use(in.a); break;
-    case I_ACC_STORE: use(in.b); break;
-    default: break;
-  }
-}
-
 // Split mode: LDS of one workgroup (gfx950: 160 KB) and the most planes it
 // moves to registers (the rest of the 512 a wave may hold is left to the
 // kernel's own values).

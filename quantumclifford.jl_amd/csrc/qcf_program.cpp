@@ -381,6 +381,61 @@ struct Compiler {
     return QCF_OK;
   }
 
+  // Move each noise site back to the run of noise sites before it when no
+  // operation in between reads or writes the planes of its qubits: a Pauli
+  // flip commutes with every operation that leaves those planes alone, so for
+  // any drawn flips the record and the frame come out the same, and fewer,
+  // larger runs mean fewer event passes (x_diag_circuit_noisy_measurement:
+  // each PauliError sits between gates on other qubits, and its 1000 sites
+  // become one run instead of 1000).  A site only ever joins the run of the
+  // noise sites just before it in program order, so the ordinals inside a
+  // run stay consecutive.
+  void hoist_noise() {
+    const std::vector<Instr> &B = P->body;
+    std::vector<Instr> out, run, tail;
+    out.reserve(B.size());
+    std::vector<uint32_t> stamp;  // plane -> id of the tail that touched it
+    uint32_t tail_id = 1;
+    std::vector<uint32_t> pl;
+    auto touch = [&](uint32_t p) {
+      if (stamp.size() <= p) stamp.resize(p + 1, 0);
+      stamp[p] = tail_id;
+    };
+    auto touched = [&](uint32_t p) { return p < stamp.size() && stamp[p] == tail_id; };
+    for (size_t i = 0; i < B.size(); ++i) {
+      const Instr &in = B[i];
+      if (in.op == I_NOISE1 || in.op == I_NOISE2) {
+        const uint32_t q[4] = {in.op == I_NOISE2 ? in.a & 0xFFFFu : in.a,
+                               in.op == I_NOISE2 ? in.a >> 16 : in.b,
+                               in.op == I_NOISE2 ? in.b & 0xFFFFu : kNone,
+                               in.op == I_NOISE2 ? in.b >> 16 : kNone};
+        bool free = !run.empty();
+        for (uint32_t p : q)
+          if (p != kNone && touched(p)) free = false;
+        if (!free) {
+          out.insert(out.end(), run.begin(), run.end());
+          out.insert(out.end(), tail.begin(), tail.end());
+          run.clear();
+          tail.clear();
+          ++tail_id;
+        }
+        run.push_back(in);
+        continue;
+      }
+      if (run.empty()) {
+        out.push_back(in);
+        continue;
+      }
+      tail.push_back(in);
+      pl.clear();
+      instr_planes(B, i, &pl);
+      for (uint32_t p : pl) touch(p);
+    }
+    out.insert(out.end(), run.begin(), run.end());
+    out.insert(out.end(), tail.begin(), tail.end());
+    P->body.swap(out);
+  }
+
   // Turn every run of consecutive noise instructions into groups: a header
   // I_NOISEG followed by one I_SITE data entry per site, never straddling a
   // 64-instruction chunk (NOPs pad to the next chunk when needed).  Pauli
@@ -645,6 +700,42 @@ std::vector<qcf::Instr> qcf::variant_code(const qcf_program *P, int variant) {
   if (variant & 2) code.insert(code.end(), P->epilogue_store.begin(), P->epilogue_store.end());
   if (code.empty()) code.push_back(Instr{I_END, 0, 0, 0, 0, 0, 0, 0});
   return code;
+}
+
+// Planes instruction code[i] reads or writes (a noise group: its sites').
+void qcf::instr_planes(const std::vector<Instr> &code, size_t i, std::vector<uint32_t> *out) {
+  const Instr &in = code[i];
+  auto use = [&](uint32_t p) {
+    if (p != kNone) out->push_back(p);
+  };
+  switch (in.op & 0xFFu) {
+    case I_XORN: {
+      const uint32_t *w = &in.a;
+      for (uint32_t k = 0; k < (in.op >> 8); ++k) use(w[k] >> 16), use(w[k] & 0xFFFFu);
+      break;
+    }
+    case I_NOISEG:
+      for (uint32_t k = 1; k <= (in.op >> 8) && i + k < code.size(); ++k) {
+        const Instr &s = code[i + k];
+        use(s.a & 0xFFFFu), use(s.a >> 16);
+        if (s.b) use(s.b & 0xFFFFu), use(s.b >> 16);
+      }
+      break;
+    case I_MR: use(in.a), use(in.b), use(in.e); break;
+    case I_MRX: use(in.a), use(in.b), use(in.e), use(in.f); break;
+    case I_XSTORE2: use(in.a), use(in.b), use(in.d); break;
+    case I_MEAS: use(in.a), use(in.c); break;
+    case I_RESETZ:
+    case I_INIT:
+    case I_LIN1: use(in.a), use(in.b); break;
+    case I_ACC_LDS:
+    case I_ACCX_LDS:
+    case I_LOAD:
+    case I_STORE:
+    case I_ZERO: use(in.a); break;
+    case I_ACC_STORE: use(in.b); break;
+    default: break;
+  }
 }
 
 // Which gauge draws a fresh, record-only run needs.
@@ -1064,6 +1155,7 @@ extern "C" int qcf_program_build(const qcf_op *ops, size_t nops,
   }
   for (const Site &s : P->sites)
     if (s.accept != 0u && s.accept != (1u << 31)) P->thin = true;
+  C.hoist_noise();
   C.peephole();
   P->nplanes = renumber_slots(P->body, 2 * nq);
   C.group_sites();

@@ -102,6 +102,9 @@ std::vector<Instr> variant_code(const qcf_program *P, int variant);
 // Marks the gauge draws of the fresh prologue and the body that a fresh,
 // record-only run can leave out (gauge_dead_pro / gauge_dead_body).
 void mark_dead_gauges(qcf_program *P);
+// Appends the planes instruction code[i] reads or writes (a noise group: its
+// sites'; a compiler-internal noise site: none -- callers handle those).
+void instr_planes(const std::vector<Instr> &code, size_t i, std::vector<uint32_t> *out);
 }  // namespace qcf
 
 // Host-side compiled program (opaque to C callers).

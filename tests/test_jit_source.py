@@ -107,13 +107,28 @@ def test_quarter_tile_source_stays_in_lds():
 
 
 def test_large_frames_with_many_noise_groups_stay_on_the_interpreter():
-    """x_diag_circuit_noisy_measurement(1000) (benchmark/benchmarks.jl:161-171):
-    1,000 separate noise groups would compile for minutes; split and part-tile
-    kernels decline it."""
-    prog = qcf.Program(circuits.x_diag_circuit_noisy_measurement(1000))
+    """1,000 noise sites on one qubit, each between two gates on it, stay
+    1,000 separate noise groups, which would compile for minutes: split and
+    part-tile kernels decline the program."""
+    n = 1000
+    ops = []
+    for i in range(1, n + 1):
+        ops += [qcf.sHadamard(i), qcf.sCNOT(i, n + 1), qcf.ops.PauliError(n + 1, 0.1)]
+    ops.append(qcf.sMZ(n + 1, 1))
+    prog = qcf.Program(ops)
     for variant in (1 | 4 | 8 | 16, 1 | 4 | 8 | 64, 1 | 4 | 8 | 128):
         with pytest.raises(_lib.MethodError):
             source(prog, variant)
+
+
+def test_noise_moves_back_past_operations_on_other_qubits():
+    """x_diag_circuit_noisy_measurement(1000) (benchmark/benchmarks.jl:161-171):
+    each PauliError(i) sits between operations on other qubits only (H(i) is
+    a relabelling of planes), so its 1000 sites join one run and the
+    quarter-tile kernel makes 16 event passes of at most 63 sites."""
+    prog = qcf.Program(circuits.x_diag_circuit_noisy_measurement(1000))
+    kernel = source(prog, 1 | 8 | 128).split("qcf_jit_kernel")[-1]
+    assert kernel.count("qcfd::group_events") == 16
 
 
 LINEAR = 1 | 8 | 256  # fresh, record only, 32-bit record offsets, linear mode
