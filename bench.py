@@ -83,6 +83,39 @@ def bench_frames(steps, warmup, shots=10**7):
                 nbits=prog.nbits, wall=wall, shots=shots)
 
 
+def bench_frame_forms(steps, shots=10**7):
+    """The headline workload (d=5 x 25 rounds, p=1e-3, 10^7 shots) in the
+    three launch forms of the sampler: fresh with the record only (the
+    headline), fresh returning the final frame too (PauliFrame + initZ! +
+    pftrajectories, src/pauli_frames.jl:193-227), and in place on a frame the
+    caller holds (pftrajectories(frame, circuit), src/pauli_frames.jl:234-239);
+    device time per launch inside a captured graph of 10."""
+    import ctypes as C
+    import torch
+    import qcf
+    from qcf import circuits
+    prog = qcf.Program(circuits.rotated_surface_code(5, 25, 1e-3).ops)
+    ld = qcf.frames.plane_words(shots)
+    rec = torch.empty(prog.nbits * ld, dtype=torch.int64, device="cuda")
+    frame = torch.zeros(2 * prog.nq * ld, dtype=torch.int64, device="cuda")
+    out = {}
+    seed = [7]
+    for name, flags, fr in (("fresh_record_only", 1, None), ("fresh_with_frame", 1, frame),
+                            ("in_place_with_frame", 0, frame)):
+        def step():
+            seed[0] += 1
+            qcf._lib.call("qcf_pf_sample", prog.handle, shots, 0, seed[0],
+                          C.c_void_p(fr.data_ptr()) if fr is not None else None,
+                          C.c_void_p(rec.data_ptr()), None, ld, flags,
+                          C.c_void_p(torch.cuda.current_stream().cuda_stream))
+        ms = time_graph(step, 10, max(2, steps))
+        # record written; with a frame, its 2 nq planes written (and read in place)
+        nbytes = prog.nbits * shots / 8 + (0 if fr is None else
+                                           2 * prog.nq * shots / 8 * (2 if flags == 0 else 1))
+        out[name] = dict(ms=ms, shots_s=shots / (ms * 1e-3), gbs=nbytes / (ms * 1e-3) / 1e9)
+    return out
+
+
 def bench_mul(steps, warmup, n=10**9):
     import numpy as np
     import torch
@@ -518,7 +551,8 @@ def main():
     ap.add_argument("--mlp-steps", type=int, default=24414)  # 2e8 shots of 8192
     ap.add_argument("--mlp-eval-shots", type=int, default=10**7)
     ap.add_argument("--only", choices=("reference_circuits", "canonicalize", "mul_left_1gq",
-                                       "mul_config", "batched", "mlp_train", "gemm"),
+                                       "mul_config", "batched", "mlp_train", "gemm",
+                                       "frame_forms"),
                     help="run one extra alone and print its JSON (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -535,7 +569,8 @@ def main():
               "mul_config": lambda: bench_mul_config(10, 2),
               "batched": lambda: bench_batched(dmax=a.batched_dmax),
               "mlp_train": lambda: bench_mlp_train(min(a.mlp_steps, 2000)),
-              "gemm": lambda: bench_gemm(reps=a.steps if a.steps < 200 else 50)}[a.only]
+              "gemm": lambda: bench_gemm(reps=a.steps if a.steps < 200 else 50),
+              "frame_forms": lambda: bench_frame_forms(5)}[a.only]
         print(json.dumps({a.only: fn()}), flush=True)
         return
     log("frames: d=5 x 25 rounds, p=1e-3 ...")
@@ -543,6 +578,12 @@ def main():
     log(f"  {fr['ms']:.3f} ms/step, {fr['shots_s']:.3e} shots/s")
     extras = {}
     if not a.no_extras:
+        log("launch forms of the d=5 x 25 workload ...")
+        try:
+            extras["frame_forms"] = bench_frame_forms(5, a.shots)
+            log(f"  {extras['frame_forms']}")
+        except Exception as e:  # reported, not hidden
+            extras["frame_forms"] = {"error": repr(e)}
         log("mul_left! 1 Gqubit ...")
         try:
             extras["mul_left_1gq"] = bench_mul(10, 2)

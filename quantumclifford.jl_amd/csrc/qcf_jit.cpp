@@ -969,13 +969,35 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
           body << "    " << WR(a, "0u") << " "
                << WR(b, (in.op & kNoGauge) ? "0u" : "g.next()") << "\n";
           break;
-        case I_LOAD:
-          body << "    "
-               << WR(a, tail ? "(live ? " + at("P.frame", b, false) + " : 0u)"
-                             : at("P.frame", b, false))
-               << "\n";
+        case I_LOAD: {
+          // a run of frame loads (the in-place prologue): up to 32 requested
+          // before any of them is stored, so that their latencies overlap
+          // (left to itself the scheduler waited after every few, 25 round
+          // trips per tile at d=5)
+          size_t j = i;
+          while (j < rg.e && (code[j].op & 0xFFu) == I_LOAD && j - i < 32 &&
+                 (!run_open || j < run_end))
+            ++j;
+          body << "    {\n";
+          for (size_t k = i; k < j; ++k) {
+            const std::string m = at("P.frame", code[k].b, false);
+            body << "    const uint32_t f" << k - i << " = "
+                 << (tail ? "live ? " + m + " : 0u" : m) << ";\n";
+          }
+          body << "    __builtin_amdgcn_sched_barrier(0);\n";
+          for (size_t k = i; k < j; ++k)
+            body << "    " << WR(code[k].a, "f" + std::to_string(k - i)) << "\n";
+          body << "    }\n";
+          i = j - 1;
           break;
+        }
         case I_STORE:
+          // the first store of the epilogue makes the plane stride opaque
+          // again: otherwise the compiler keeps the frame addresses of the
+          // in-place loads live through the whole tile for these stores
+          // (d=5 x 25 in place: 302 VGPRs and four waves per CU, not six)
+          if (i == rg.b || (code[i - 1].op & 0xFFu) != I_STORE)
+            body << "    asm volatile(\"\" : \"+s\"(ld));\n";
           body << "    " << (tail ? "if (live) " : "") << at("P.frame", b, false) << " = " << RD(a)
                << (tail ? " & lm;\n" : ";\n");
           break;

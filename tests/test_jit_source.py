@@ -173,3 +173,18 @@ def test_zero_rate_noise_compiles_like_no_noise():
     lz, lc = source(zero, LINEAR), source(clean, LINEAR)
     assert lz.splitlines()[1] == lc.splitlines()[1] == \
         "// linear: 24 gauge words, 565 XORs, 1226 record columns"
+
+
+def test_in_place_frame_loads_batched_and_addresses_not_kept():
+    """The in-place prologue requests its frame loads in batches before
+    storing any (one latency per batch, not per few loads), and the epilogue
+    makes the plane stride opaque before its stores, so the compiler does not
+    keep every frame address of the loads live through the tile."""
+    prog = qcf.Program(circuits.rotated_surface_code(5, 25, 1e-3).ops)
+    kernel = source(prog, 2 | 8).split("qcf_jit_kernel")[-1]
+    loads = kernel.split("__builtin_amdgcn_sched_barrier(0);")
+    assert len(loads) >= 4  # 100 planes in batches of at most 32
+    assert loads[0].count("P.frame[") == 32
+    last_load = kernel.rindex("= P.frame[")
+    barrier = kernel.index('asm volatile("" : "+s"(ld));', last_load)
+    assert barrier < kernel.index("P.frame[", barrier + 1)
