@@ -500,6 +500,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     const uint32_t kmin = P->nplanes > budget ? P->nplanes - budget : 0u;
     const uint32_t k = std::min(P->nplanes, std::max(kmin, jit_split_regs()));
     if (k > kSplitMaxRegs) return false;
+    if ((variant & 32) && k > jit_two_wave_regs()) return false;
     std::vector<uint32_t> order(P->nplanes);
     for (uint32_t p = 0; p < P->nplanes; ++p) order[p] = p;
     std::stable_sort(order.begin(), order.end(),
@@ -1188,6 +1189,18 @@ bool jit_split_touched() {
   return !(e && e[0] == '0');
 }
 
+// Two-wave split mode (half of a CU's LDS per wave) only with at most this
+// many register planes (QCF_JIT_TWO_WAVE_REGS, default 256): beyond the
+// architectural VGPRs the planes spill into the accumulation registers and
+// the second wave costs more than it hides.  10^7 shots, frame returned:
+// rotated d=9 (125 / 205 register planes at p = 0 / 1e-3) 0.80 / 2.18 ms
+// two-wave against 1.43 / 3.30 one-wave; d=11 at p = 0 (345 planes) 5.89 ms
+// two-wave against 3.70 one-wave.
+uint32_t jit_two_wave_regs() {
+  const char *e = getenv("QCF_JIT_TWO_WAVE_REGS");
+  return e ? (uint32_t)strtoul(e, nullptr, 10) : 256u;
+}
+
 // QCF_JIT_SPLIT=0 leaves frames larger than LDS to the global-memory paths.
 bool jit_split_enabled() {
   const char *e = getenv("QCF_JIT_SPLIT");
@@ -1260,7 +1273,7 @@ std::string jit_knob_signature() {
   static const char *const kKnobs[] = {
       "QCF_JIT_REG_EVENTS", "QCF_JIT_SLOTS", "QCF_JIT_REG_PLANES", "QCF_JIT_MAX_INSTR",
       "QCF_JIT_SPLIT_REGS", "QCF_JIT_SPLIT_BARRIER", "QCF_JIT_SPLIT_TOUCHED", "QCF_JIT_NT",
-      "QCF_JIT_NO_LOOPS", "QCF_JIT_HOIST", "QCF_JIT_PREFETCH"};
+      "QCF_JIT_NO_LOOPS", "QCF_JIT_HOIST", "QCF_JIT_PREFETCH", "QCF_JIT_TWO_WAVE_REGS"};
   std::string s;
   for (const char *k : kKnobs) {
     const char *e = getenv(k);

@@ -46,6 +46,7 @@ bool jit_no_loops();
 bool jit_global_enabled();
 bool jit_split_enabled();
 uint32_t jit_split_regs();
+uint32_t jit_two_wave_regs();
 size_t jit_split_barrier();
 bool jit_split_touched();
 bool jit_nt_stores();
