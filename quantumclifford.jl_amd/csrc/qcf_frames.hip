@@ -597,15 +597,18 @@ int jit_plan(qcf_program *P, const KParams &kp, uint64_t shots, int variant, boo
     js = jit_get(P, v | 16 | 32, &jk);
     if (js != QCF_OK) clear_error();
   }
+  uint32_t parts = 1;  // passes per tile of the kernel taken
   if (js != QCF_OK && try_split) {
     js = jit_get(P, v | 16, &jk);
     if (js != QCF_OK) {
       clear_error();
       js = jit_get(P, v | 64, &jk);
+      if (js == QCF_OK) parts = 2;
     }
     if (js != QCF_OK) {
       clear_error();
       js = jit_get(P, v | 128, &jk);  // quarter tiles: up to 2560 planes
+      if (js == QCF_OK) parts = 4;
     }
   }
   if (js != QCF_OK && !try_split) js = jit_get(P, v, &jk);
@@ -630,7 +633,18 @@ int jit_plan(qcf_program *P, const KParams &kp, uint64_t shots, int variant, boo
                                  : std::min<uint32_t>(jwpc, kGlobalJitWavesPerCU);
   uint64_t g1 = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * jwpc);
   const uint32_t nx = jit_xcds();
-  if (!gjit && jit_full_grid()) {
+  if (parts > 1 && kp.nblocks < (uint64_t)st->cu_count * jwpc) {
+    // fewer tiles than resident workgroups: the parts of a tile run on
+    // different workgroups at once (pset of them per tile); the tile loop
+    // then sees gridDim / pset workgroups, a whole number of XCD rounds
+    uint64_t per = std::min<uint64_t>(kp.nblocks, (uint64_t)st->cu_count * jwpc / parts);
+    if (nx > 1 && per >= nx) {
+      per -= per % nx;
+      jp.xcds = nx;
+    }
+    jp.pset = parts;
+    g1 = std::max<uint64_t>(per, 1) * parts;
+  } else if (!gjit && jit_full_grid()) {
     // one workgroup per tile, rounded up to whole XCD rounds (the extra
     // workgroups find no tile and return)
     g1 = kp.nblocks;
@@ -1168,6 +1182,7 @@ int plan_jit(qcf_batch *B) {
     gp.jp.multi = (const JitParams *)(dbase + p0 * sizeof(JitParams));
     gp.jp.first = (const uint32_t *)(dbase + pbytes + f0 * sizeof(uint32_t));
     gp.jp.ncirc = (uint32_t)m.size();
+    gp.jp.pset = 0;  // every workgroup runs all parts of its tiles
     gp.jp.ntiles = tiles;
     uint64_t g1 = std::min<uint64_t>(tiles, (uint64_t)st->cu_count * std::max(1u, gp.blocks_per_cu));
     const uint32_t nx = jit_xcds();

@@ -55,6 +55,7 @@ struct KJ {
   uint32_t *steal;    // shared tiles: the next tile index (null: dealt statically)
   uint32_t *sync;     // shared tiles: [1] workgroups done, cleared by the last
   uint32_t sync_total, sync_words;
+  uint32_t pset;      // part-tile kernels: workgroups i, i+pset, ... take part i % pset of each tile
 };
 struct FlipLDS {
   uint32_t *L;
@@ -290,7 +291,7 @@ constexpr size_t kMaxLargeFrameGroups = 256;
 const char *const kTileLoopHead = R"QCFJIT(
   const uint32_t ntiles = P0.ncirc ? P0.ntiles : P0.nblocks;
   const uint32_t nx = P0.xcds ? P0.xcds : 1u;
-  const uint32_t per = gridDim.x / nx, xi = blockIdx.x % nx, wi = blockIdx.x / nx;
+  const uint32_t per = gx / nx, xi = bx % nx, wi = bx / nx;
   const uint32_t span = (ntiles + nx - 1u) / nx;
   for (uint32_t t = wi;; t += per) {
     uint32_t tile;
@@ -391,7 +392,7 @@ bool generate_linear(const qcf_program *P, int variant, std::string *out, uint32
       << " XORs, " << P->nbits << " record columns\n";
   src << kDeviceSrc << kKernelPrelude;
   src << "extern \"C\" __global__ void __launch_bounds__(64) qcf_jit_kernel(KJ P0) {\n"
-      << "  const uint32_t l = threadIdx.x;\n";
+      << "  const uint32_t l = threadIdx.x, bx = blockIdx.x, gx = gridDim.x;\n";
   src << kTileLoopHead;
   src << "    const uint32_t gw = blk * 64u + l;\n";
   src << kTileLocals;
@@ -1038,6 +1039,15 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   src << "extern \"C\" __global__ void __launch_bounds__(64) qcf_jit_kernel(KJ P0) {\n"
       << "  extern __shared__ uint32_t L[];\n"
       << "  const uint32_t l = threadIdx.x;\n";
+  // part-tile kernels of a single launch may spread the parts of a tile over
+  // pset workgroups (workgroup i takes part i % pset), so that launches of a
+  // few tiles use more than one CU per tile
+  if (halfm)
+    src << "  const uint32_t pset = P0.pset > 1u && !P0.ncirc && !P0.steal ? P0.pset : 1u;\n"
+           "  const uint32_t hsel = blockIdx.x % pset;\n"
+           "  const uint32_t bx = blockIdx.x / pset, gx = gridDim.x / pset;\n";
+  else
+    src << "  const uint32_t bx = blockIdx.x, gx = gridDim.x;\n";
   {
     const uint32_t nl = glob ? 0u : split ? nlds + max_slots : regs ? max_slots : hyb ? qpl : P->nplanes;
     for (uint32_t w = 1; w * per_window < nl; ++w)
@@ -1071,7 +1081,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   // one part in lanes 0..pw-1 (the event stream is drawn for the whole tile
   // in every pass); passes past the plane stride hold no shots and end it
   if (halfm)
-    src << "    for (uint32_t hf = 0; hf < " << 64u / pw << "u; ++hf) {\n"
+    src << "    for (uint32_t hf = hsel; hf < " << 64u / pw << "u; hf += pset) {\n"
         << "    if (blk * 64u + hf * " << pw << "u >= P.nwbuf) break;\n"
         << "    const bool lh = l < " << pw << "u;\n"
         << "    const uint32_t gw = blk * 64u + hf * " << pw << "u + (l & " << pw - 1 << "u);\n"

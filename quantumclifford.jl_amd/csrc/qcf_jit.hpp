@@ -33,6 +33,9 @@ struct JitParams {
   uint32_t *steal;
   uint32_t *sync;
   uint32_t sync_total, sync_words;
+  // part-tile kernels of a single launch: workgroups i, i + pset, ... take
+  // part i % pset of every tile they visit (0 or 1: every part)
+  uint32_t pset;
 };
 
 // QCF_NO_JIT=1 keeps every launch on the interpreter.

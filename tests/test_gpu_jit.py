@@ -115,7 +115,7 @@ def test_global_frames_specialised_equals_interpreter(gpu, monkeypatch, d, round
 
 
 @pytest.mark.parametrize("case", ["unrotated_11x3", "rotated_13x2", "rotated_11x3", "rotated_9x3",
-                                  "rotated_15x2", "x_diag_1000"])
+                                  "rotated_15x2", "x_diag_1000", "x_diag_noisy_1000"])
 def test_split_frames_specialised_equals_interpreter(gpu, monkeypatch, case):
     """Frames too large for LDS run the split kernel (most-used planes in
     registers, the rest in LDS, noise on register planes through slots), with
@@ -123,9 +123,14 @@ def test_split_frames_specialised_equals_interpreter(gpu, monkeypatch, case):
     d=9, whose frame fits but would run one wave per CU): bit for bit the
     interpreter's result, partial tile and kept frame included.  Rotated d=15
     (1237 planes) is beyond split mode and runs half-tile passes;
-    x_diag_circuit(1000) of benchmark/benchmarks.jl (1,001 qubits) quarter tiles."""
+    x_diag_circuit(1000) of benchmark/benchmarks.jl (1,001 qubits) and its
+    noisy-measurement variant (1000 noise sites in one run) quarter tiles.
+    Launches of fewer tiles than CUs spread the parts of each tile over
+    workgroups."""
     if case == "x_diag_1000":
         ops, shots = circuits.x_diag_circuit(1000), 1000
+    elif case == "x_diag_noisy_1000":
+        ops, shots = circuits.x_diag_circuit_noisy_measurement(1000), 1000
     elif case == "rotated_15x2":
         ops, shots = circuits.rotated_surface_code(15, 2, 5e-3).ops, 3 * 2048 + 1000
     elif case == "unrotated_11x3":
