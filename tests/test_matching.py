@@ -15,6 +15,7 @@ decodes over.
 * the AbstractSyndromeDecoder contract (src/ecc/decoder_pipeline.jl:12-24,
   :172-189): guesses times the faults matrix give the decoded observables.
 """
+import ctypes as C
 import itertools
 import json
 import os
@@ -149,6 +150,52 @@ def oracle_dem(circ, det_bits, obs_bits):
                                   lambda: circuits.rotated_surface_code(3, 3, 2e-3)])
 def test_error_model_equals_oracle_single_faults(make):
     c = make()
+    ours = decoders.DetectorErrorModel(c.ops, c.det_bits, c.obs_bits)
+    got = {(tuple(int(x) for x in d), o): p for d, o, p in ours.mechanisms()}
+    want = oracle_dem(c.ops, c.det_bits, c.obs_bits)
+    assert set(got) == set(want)
+    for k in want:
+        assert got[k] == pytest.approx(want[k], rel=1e-9, abs=1e-15), k
+
+
+def _interleaved_noise_repetition_code():
+    """The d=5, 2-round repetition code with an extra PauliError after every
+    CNOT on a data qubit that CNOT leaves alone: the program compiler moves
+    those sites back across the operations on other qubits into the noise
+    run before them."""
+    c = circuits.repetition_code(5, 2, 3e-3)
+    ops, extra = [], 0
+    k = 0
+    for op in c.ops:
+        ops.append(op)
+        if isinstance(op, qcf.sCNOT):
+            q = 1 + (k % 5)
+            k += 1
+            if q not in (op.q1, op.q2):
+                ops.append(qcf.ops.PauliError(q, 2e-3))
+                extra += 1
+    c.ops = ops
+    return c, extra
+
+
+def test_error_model_with_moved_noise_equals_oracle_single_faults():
+    """Noise moved back across operations on other qubits leaves every
+    single fault's detectors and observable as the oracle's replay of the
+    original operation order gives them."""
+    c, extra = _interleaved_noise_repetition_code()
+
+    def passes(ops):
+        prog = qcf.Program(ops)
+        n = C.c_size_t()
+        _lib.call("qcf_debug_jit_source", prog.handle, 1 | 8, None, 0, C.byref(n))
+        buf = C.create_string_buffer(n.value + 1)
+        _lib.call("qcf_debug_jit_source", prog.handle, 1 | 8, buf, n.value + 1, C.byref(n))
+        return buf.value.decode().split("qcf_jit_kernel")[-1].count("qcfd::group_events")
+
+    # every inserted site joined the noise run before it: no event pass more
+    # than the unmodified circuit's (without the moves, up to one per site)
+    assert extra == 12
+    assert passes(c.ops) == passes(circuits.repetition_code(5, 2, 3e-3).ops)
     ours = decoders.DetectorErrorModel(c.ops, c.det_bits, c.obs_bits)
     got = {(tuple(int(x) for x in d), o): p for d, o, p in ours.mechanisms()}
     want = oracle_dem(c.ops, c.det_bits, c.obs_bits)
