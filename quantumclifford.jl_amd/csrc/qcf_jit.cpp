@@ -453,6 +453,15 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
                     (P->classes.empty() || ev_per_tile >= jit_reg_events() ||
                      2u * P->nq * 256u > 40u * 1024u);
   const bool nt = jit_nt_stores();
+  // LDS mode runs several waves per SIMD, and a wave in an event pass (a
+  // chain of dependent draws, a wave-wide scan and permutes) issues ahead of
+  // the others while it lasts: d=5 x 25, 10^7 shots, p = 1e-4 / 1e-3 / 3e-3:
+  // 0.320 / 0.406 / 0.763 ms at one priority, 0.311 / 0.393 / 0.760 with
+  // passes raised.  Register, split and part-tile kernels stay at one
+  // priority (one wave per SIMD, or no measured gain).
+  const int prio_env = jit_pass_prio();
+  const bool lds_mode = !glob && !split && !halfm && !regs;
+  const int pass_prio = prio_env >= 0 ? prio_env : (lds_mode ? 1 : 0);
   // LDS mode keeps the classical slots (planes 2*nq and up, which no noise
   // site names) in registers: only the qubit planes take LDS, so more waves
   // fit on a CU
@@ -868,6 +877,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
           for (uint32_t done = 0; done < G; ++gi) {
             const Group &g = groups[gi];
             body << "    {\n";
+            if (pass_prio) body << "    __builtin_amdgcn_s_setprio(" << pass_prio << ");\n";
             if (preload) {
               if (!prefetched) emit_prefetch(gi, in_loop, body);
               body << "    if (P.has_class) qcfd::group_events_r" << pass_tmpl(gi) << "("
@@ -907,6 +917,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
                 body << ");\n";
               }
             }
+            if (pass_prio) body << "    __builtin_amdgcn_s_setprio(0);\n";
             body << "    }\n";
             done += g.count;
             // the next pass's first permute goes here, ahead of the gates
@@ -1211,6 +1222,14 @@ uint32_t jit_two_wave_regs() {
   return e ? (uint32_t)strtoul(e, nullptr, 10) : 256u;
 }
 
+// Issue priority of a wave during its event passes (s_setprio), -1 for the
+// default: QCF_JIT_PASS_PRIO=0 keeps every wave at priority 0, 1..3 raises
+// passes to that level in every storage mode.
+int jit_pass_prio() {
+  const char *e = getenv("QCF_JIT_PASS_PRIO");
+  return e && e[0] >= '0' && e[0] <= '3' ? e[0] - '0' : -1;
+}
+
 // QCF_JIT_SPLIT=0 leaves frames larger than LDS to the global-memory paths.
 bool jit_split_enabled() {
   const char *e = getenv("QCF_JIT_SPLIT");
@@ -1283,7 +1302,7 @@ std::string jit_knob_signature() {
   static const char *const kKnobs[] = {
       "QCF_JIT_REG_EVENTS", "QCF_JIT_SLOTS", "QCF_JIT_REG_PLANES", "QCF_JIT_MAX_INSTR",
       "QCF_JIT_SPLIT_REGS", "QCF_JIT_SPLIT_BARRIER", "QCF_JIT_SPLIT_TOUCHED", "QCF_JIT_NT",
-      "QCF_JIT_NO_LOOPS", "QCF_JIT_HOIST", "QCF_JIT_PREFETCH", "QCF_JIT_TWO_WAVE_REGS"};
+      "QCF_JIT_NO_LOOPS", "QCF_JIT_HOIST", "QCF_JIT_PREFETCH", "QCF_JIT_TWO_WAVE_REGS", "QCF_JIT_PASS_PRIO"};
   std::string s;
   for (const char *k : kKnobs) {
     const char *e = getenv(k);

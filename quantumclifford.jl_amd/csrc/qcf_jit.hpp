@@ -50,6 +50,7 @@ bool jit_global_enabled();
 bool jit_split_enabled();
 uint32_t jit_split_regs();
 uint32_t jit_two_wave_regs();
+int jit_pass_prio();
 size_t jit_split_barrier();
 bool jit_split_touched();
 bool jit_nt_stores();

@@ -188,3 +188,19 @@ def test_in_place_frame_loads_batched_and_addresses_not_kept():
     last_load = kernel.rindex("= P.frame[")
     barrier = kernel.index('asm volatile("" : "+s"(ld));', last_load)
     assert barrier < kernel.index("P.frame[", barrier + 1)
+
+
+def test_event_passes_raise_priority_in_lds_mode(monkeypatch):
+    """LDS-mode kernels (several waves per SIMD) raise a wave's issue priority
+    for the length of each event pass and drop it after; register and split
+    kernels keep one priority; QCF_JIT_PASS_PRIO=0 turns it off."""
+    lds = qcf.Program(circuits.rotated_surface_code(5, 25, 1e-3).ops)
+    kernel = source(lds, 1 | 8).split("qcf_jit_kernel")[-1]
+    n_up = kernel.count("__builtin_amdgcn_s_setprio(1);")
+    assert n_up > 0 and n_up == kernel.count("__builtin_amdgcn_s_setprio(0);")
+    assert kernel.count("qcfd::group_events") == n_up
+    for other, v in ((circuits.rotated_surface_code(5, 25, 1e-2).ops, 1 | 8),      # registers
+                     (circuits.rotated_surface_code(13, 2, 1e-3).ops, 1 | 8 | 16)):  # split
+        assert "s_setprio" not in source(qcf.Program(other), v).split("qcf_jit_kernel")[-1]
+    monkeypatch.setenv("QCF_JIT_PASS_PRIO", "0")
+    assert "s_setprio" not in source(lds, 1 | 8).split("qcf_jit_kernel")[-1]
