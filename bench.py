@@ -27,9 +27,9 @@ for _p in (ROOT, PKG):
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per shot of the d=5 x 25 launch from rocprofv3 --pmc (profiles/
 # pf_jit_kernel_d5x25_traffic.md): WRITE_SIZE 1,496,639.5 KiB and FETCH_SIZE
-# 217.8 KiB per 10^7-shot dispatch, scaled by what the counters read for a
+# 217.2 KiB per 10^7-shot dispatch, scaled by what the counters read for a
 # 1 GiB copy (WRITE_SIZE 1.0x, FETCH_SIZE 0.5x of the bytes moved).
-TRAFFIC_BYTES_PER_SHOT = (1496639.5 * 1.0 + 217.8 * 2.0) * 1024 / 1e7
+TRAFFIC_BYTES_PER_SHOT = (1496639.5 * 1.0 + 217.2 * 2.0) * 1024 / 1e7
 INT_VALU_PEAK = 256 * 64 * 2.4e9  # int32 lane-ops/s (256 CUs x 64 lanes x 2.4 GHz)
 METRIC = "batched vs per-circuit shots/s, d=5 surface 25 rds; NN vs MWPM LER"
 
