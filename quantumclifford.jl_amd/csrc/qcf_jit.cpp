@@ -1042,6 +1042,13 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   if ((!halfm && !emit(false, full)) || !emit(true, part)) return false;
   std::ostringstream src;
   src << "#define QCF_RTC 1\n";
+  // LDS-mode kernels are scheduled for instruction-level parallelism (the
+  // machine scheduler's max-ilp strategy): d=5 x 25, p = 1e-3, 10^7 shots
+  // 0.393 -> 0.383 ms.  Other modes keep the default, which measured faster
+  // there (two-wave split d=9: 1.88 against 2.22 ms; register mode p = 1e-2:
+  // 1.34 against 1.40 ms).
+  if (lds_mode && jit_sched_ilp() != 0)
+    src << kCompileOptTag << "-mllvm -amdgpu-sched-strategy=max-ilp\n";
   src << "// folded: prologue " << lp.s << ", loop " << lp.R << " x " << lp.L << ", of "
       << code.size() << " instructions\n";
   src << kDeviceSrc << kKernelPrelude;
@@ -1132,9 +1139,23 @@ int compile(const std::string &src, std::vector<char> *code, std::string *log) {
   if (hiprtcCreateProgram(&prog, src.c_str(), "qcf_jit.hip", 0, nullptr, nullptr) !=
       HIPRTC_SUCCESS)
     return set_error(QCF_ERR_HIP, "hiprtcCreateProgram failed");
-  const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang",
-                        "-target-feature", "-Xclang", "-packed-fp32-ops"};
-  const hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof *opts), opts);
+  std::vector<const char *> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang",
+                                    "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  // options the generator asked for (lines of the source starting with the
+  // tag), then QCF_JIT_OPTS: extra compiler options separated by spaces
+  std::vector<std::string> extra;
+  for (size_t at = src.find(kCompileOptTag); at != std::string::npos;
+       at = src.find(kCompileOptTag, at + 1)) {
+    const size_t b = at + strlen(kCompileOptTag), e = src.find('\n', b);
+    std::istringstream is(src.substr(b, e == std::string::npos ? std::string::npos : e - b));
+    for (std::string w; is >> w;) extra.push_back(w);
+  }
+  if (const char *e = getenv("QCF_JIT_OPTS")) {
+    std::istringstream is(e);
+    for (std::string w; is >> w;) extra.push_back(w);
+  }
+  for (const std::string &w : extra) opts.push_back(w.c_str());
+  const hiprtcResult r = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
   size_t n = 0;
   hiprtcGetProgramLogSize(prog, &n);
   if (n > 1) {
@@ -1222,6 +1243,13 @@ uint32_t jit_two_wave_regs() {
   return e ? (uint32_t)strtoul(e, nullptr, 10) : 256u;
 }
 
+// QCF_JIT_SCHED_ILP=0 compiles LDS-mode kernels with the default machine
+// scheduler instead of max-ilp.
+int jit_sched_ilp() {
+  const char *e = getenv("QCF_JIT_SCHED_ILP");
+  return e && e[0] == '0' ? 0 : 1;
+}
+
 // Issue priority of a wave during its event passes (s_setprio), -1 for the
 // default: QCF_JIT_PASS_PRIO=0 keeps every wave at priority 0, 1..3 raises
 // passes to that level in every storage mode.
@@ -1302,7 +1330,7 @@ std::string jit_knob_signature() {
   static const char *const kKnobs[] = {
       "QCF_JIT_REG_EVENTS", "QCF_JIT_SLOTS", "QCF_JIT_REG_PLANES", "QCF_JIT_MAX_INSTR",
       "QCF_JIT_SPLIT_REGS", "QCF_JIT_SPLIT_BARRIER", "QCF_JIT_SPLIT_TOUCHED", "QCF_JIT_NT",
-      "QCF_JIT_NO_LOOPS", "QCF_JIT_HOIST", "QCF_JIT_PREFETCH", "QCF_JIT_TWO_WAVE_REGS", "QCF_JIT_PASS_PRIO"};
+      "QCF_JIT_NO_LOOPS", "QCF_JIT_HOIST", "QCF_JIT_PREFETCH", "QCF_JIT_TWO_WAVE_REGS", "QCF_JIT_PASS_PRIO", "QCF_JIT_OPTS", "QCF_JIT_SCHED_ILP"};
   std::string s;
   for (const char *k : kKnobs) {
     const char *e = getenv(k);
@@ -1342,14 +1370,17 @@ int jit_get(qcf_program *P, int variant, JitKernelInfo *info) {
     jk.key = std::hash<std::string>{}(src);
     // programs with the same source (the same circuit built twice) share one
     // compilation
-    auto it = g_code_cache.find(src);
+    // (keyed with the extra compiler options, which change the code too)
+    const char *xo = getenv("QCF_JIT_OPTS");
+    const std::string ckey = src + "\n//" + (xo ? xo : "");
+    auto it = g_code_cache.find(ckey);
     if (it != g_code_cache.end()) {
       bin = it->second;
     } else {
       st = compile(src, &bin, &log);
       if (st == QCF_OK) {
         if (g_code_cache.size() >= kCodeCacheMax) g_code_cache.clear();
-        g_code_cache.emplace(src, bin);
+        g_code_cache.emplace(ckey, bin);
       }
     }
   }

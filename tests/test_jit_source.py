@@ -204,3 +204,21 @@ def test_event_passes_raise_priority_in_lds_mode(monkeypatch):
         assert "s_setprio" not in source(qcf.Program(other), v).split("qcf_jit_kernel")[-1]
     monkeypatch.setenv("QCF_JIT_PASS_PRIO", "0")
     assert "s_setprio" not in source(lds, 1 | 8).split("qcf_jit_kernel")[-1]
+
+
+def test_lds_mode_kernels_ask_for_ilp_scheduling(monkeypatch):
+    """LDS-mode kernels carry a compiler-option line for the max-ilp machine
+    scheduler; register and split kernels and the linear kernel do not;
+    QCF_JIT_SCHED_ILP=0 drops it; the tagged kernel compiles."""
+    tag = "// qcf-compile-options: -mllvm -amdgpu-sched-strategy=max-ilp"
+    lds = qcf.Program(circuits.rotated_surface_code(5, 25, 1e-3).ops)
+    assert tag in source(lds, 1 | 8)
+    n = C.c_size_t()
+    _lib.call("qcf_debug_jit_compile", lds.handle, 1 | 8, C.byref(n))
+    assert n.value > 0
+    assert tag not in source(qcf.Program(circuits.rotated_surface_code(5, 25, 1e-2).ops), 1 | 8)
+    assert tag not in source(qcf.Program(circuits.rotated_surface_code(13, 2, 1e-3).ops), 1 | 8 | 16)
+    clean = qcf.Program(circuits.rotated_surface_code(3, 3, 0.0, noise=False).ops)
+    assert tag not in source(clean, LINEAR)
+    monkeypatch.setenv("QCF_JIT_SCHED_ILP", "0")
+    assert tag not in source(lds, 1 | 8)
