@@ -1042,13 +1042,17 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   if ((!halfm && !emit(false, full)) || !emit(true, part)) return false;
   std::ostringstream src;
   src << "#define QCF_RTC 1\n";
-  // LDS-mode kernels are scheduled for instruction-level parallelism (the
-  // machine scheduler's max-ilp strategy): d=5 x 25, p = 1e-3, 10^7 shots
-  // 0.393 -> 0.383 ms.  Other modes keep the default, which measured faster
-  // there (two-wave split d=9: 1.88 against 2.22 ms; register mode p = 1e-2:
-  // 1.34 against 1.40 ms).
-  if (lds_mode && jit_sched_ilp() != 0)
+  // Machine-scheduler strategy by storage mode: LDS-mode kernels are
+  // scheduled for instruction-level parallelism (max-ilp; d=5 x 25, p =
+  // 1e-3, 10^7 shots: 0.393 -> 0.383 ms), two-wave split kernels for low
+  // register pressure (iterative-minreg; rotated d=9, p = 1e-3: 1.88 -> 1.74
+  // ms).  The other modes keep the default, which measured as fast or faster
+  // there than max-ilp, max-memory-clause, iterative-minreg and
+  // iterative-maxocc.
+  if (jit_sched_tags() && lds_mode)
     src << kCompileOptTag << "-mllvm -amdgpu-sched-strategy=max-ilp\n";
+  if (jit_sched_tags() && split && (variant & 32))
+    src << kCompileOptTag << "-mllvm -amdgpu-sched-strategy=iterative-minreg\n";
   src << "// folded: prologue " << lp.s << ", loop " << lp.R << " x " << lp.L << ", of "
       << code.size() << " instructions\n";
   src << kDeviceSrc << kKernelPrelude;
@@ -1243,11 +1247,11 @@ uint32_t jit_two_wave_regs() {
   return e ? (uint32_t)strtoul(e, nullptr, 10) : 256u;
 }
 
-// QCF_JIT_SCHED_ILP=0 compiles LDS-mode kernels with the default machine
-// scheduler instead of max-ilp.
-int jit_sched_ilp() {
-  const char *e = getenv("QCF_JIT_SCHED_ILP");
-  return e && e[0] == '0' ? 0 : 1;
+// QCF_JIT_SCHED=0 compiles every kernel with the default machine scheduler
+// (instead of the strategy chosen per storage mode).
+bool jit_sched_tags() {
+  const char *e = getenv("QCF_JIT_SCHED");
+  return !(e && e[0] == '0');
 }
 
 // Issue priority of a wave during its event passes (s_setprio), -1 for the
@@ -1330,7 +1334,7 @@ std::string jit_knob_signature() {
   static const char *const kKnobs[] = {
       "QCF_JIT_REG_EVENTS", "QCF_JIT_SLOTS", "QCF_JIT_REG_PLANES", "QCF_JIT_MAX_INSTR",
       "QCF_JIT_SPLIT_REGS", "QCF_JIT_SPLIT_BARRIER", "QCF_JIT_SPLIT_TOUCHED", "QCF_JIT_NT",
-      "QCF_JIT_NO_LOOPS", "QCF_JIT_HOIST", "QCF_JIT_PREFETCH", "QCF_JIT_TWO_WAVE_REGS", "QCF_JIT_PASS_PRIO", "QCF_JIT_OPTS", "QCF_JIT_SCHED_ILP"};
+      "QCF_JIT_NO_LOOPS", "QCF_JIT_HOIST", "QCF_JIT_PREFETCH", "QCF_JIT_TWO_WAVE_REGS", "QCF_JIT_PASS_PRIO", "QCF_JIT_OPTS", "QCF_JIT_SCHED"};
   std::string s;
   for (const char *k : kKnobs) {
     const char *e = getenv(k);

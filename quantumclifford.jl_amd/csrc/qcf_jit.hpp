@@ -51,7 +51,7 @@ bool jit_split_enabled();
 uint32_t jit_split_regs();
 uint32_t jit_two_wave_regs();
 int jit_pass_prio();
-int jit_sched_ilp();
+bool jit_sched_tags();
 // Source lines starting with this tag carry compiler options for the kernel.
 constexpr const char *kCompileOptTag = "// qcf-compile-options: ";
 size_t jit_split_barrier();

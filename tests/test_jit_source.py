@@ -209,7 +209,7 @@ def test_event_passes_raise_priority_in_lds_mode(monkeypatch):
 def test_lds_mode_kernels_ask_for_ilp_scheduling(monkeypatch):
     """LDS-mode kernels carry a compiler-option line for the max-ilp machine
     scheduler; register and split kernels and the linear kernel do not;
-    QCF_JIT_SCHED_ILP=0 drops it; the tagged kernel compiles."""
+    QCF_JIT_SCHED=0 drops it; the tagged kernel compiles."""
     tag = "// qcf-compile-options: -mllvm -amdgpu-sched-strategy=max-ilp"
     lds = qcf.Program(circuits.rotated_surface_code(5, 25, 1e-3).ops)
     assert tag in source(lds, 1 | 8)
@@ -220,5 +220,9 @@ def test_lds_mode_kernels_ask_for_ilp_scheduling(monkeypatch):
     assert tag not in source(qcf.Program(circuits.rotated_surface_code(13, 2, 1e-3).ops), 1 | 8 | 16)
     clean = qcf.Program(circuits.rotated_surface_code(3, 3, 0.0, noise=False).ops)
     assert tag not in source(clean, LINEAR)
-    monkeypatch.setenv("QCF_JIT_SCHED_ILP", "0")
+    # two-wave split kernels ask for the low-register-pressure strategy
+    minreg = "// qcf-compile-options: -mllvm -amdgpu-sched-strategy=iterative-minreg"
+    two = source(qcf.Program(circuits.rotated_surface_code(9, 3, 1e-3).ops), 1 | 8 | 16 | 32)
+    assert minreg in two and tag not in two
+    monkeypatch.setenv("QCF_JIT_SCHED", "0")
     assert tag not in source(lds, 1 | 8)
