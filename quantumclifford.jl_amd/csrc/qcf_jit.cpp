@@ -478,7 +478,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     }
     i += G;
   }
-  const uint32_t kSlots = jit_slots();  // noise slots per event pass in register mode
+  uint32_t kSlots = jit_slots();  // noise slots per event pass (register and split modes)
   // Split mode: sreg[p] marks the planes kept in registers, lidx[p] gives the
   // others their LDS plane (packed), and the noise slots of register planes
   // follow the last LDS plane.  A plane's score is the operations that read
@@ -505,6 +505,16 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     }
     // variant bit 5: half of a CU's LDS per wave, so that two waves share it
     const uint32_t lds_planes = kSplitLdsBytes / 256u / ((variant & 32) ? 2u : 1u);
+    // Frames that would need more than 256 register planes beside the
+    // default slots take 24 slots instead: the LDS they free holds planes
+    // that would otherwise sit in registers (the accumulation registers, past
+    // 256), at the cost of event passes split to fit fewer slots.  Rotated
+    // d=13, p = 1e-4 / 1e-3 / 1e-2, 4e6 shots: 5.08 / 6.28 / 18.4 ms with 80
+    // slots, 3.61 / 4.81 / 17.0 with 24; d=11 (about 110 register planes)
+    // and two-wave d=9 measured best at the default.
+    if (noisy && !getenv("QCF_JIT_SLOTS") && lds_planes > kSlots &&
+        P->nplanes > lds_planes - kSlots && P->nplanes - (lds_planes - kSlots) > 256u)
+      kSlots = 24;
     if (noisy && lds_planes <= kSlots) return false;
     const uint32_t budget = lds_planes - (noisy ? kSlots : 0u);
     const uint32_t kmin = P->nplanes > budget ? P->nplanes - budget : 0u;

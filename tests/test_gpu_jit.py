@@ -114,8 +114,8 @@ def test_global_frames_specialised_equals_interpreter(gpu, monkeypatch, d, round
     assert np.array_equal(fa, fb)
 
 
-@pytest.mark.parametrize("case", ["unrotated_11x3", "rotated_13x2", "rotated_11x3", "rotated_9x3",
-                                  "rotated_15x2", "x_diag_1000", "x_diag_noisy_1000"])
+@pytest.mark.parametrize("case", ["unrotated_11x3", "rotated_13x2", "rotated_13x13", "rotated_11x3",
+                                  "rotated_9x3", "rotated_15x2", "x_diag_1000", "x_diag_noisy_1000"])
 def test_split_frames_specialised_equals_interpreter(gpu, monkeypatch, case):
     """Frames too large for LDS run the split kernel (most-used planes in
     registers, the rest in LDS, noise on register planes through slots), with
@@ -137,6 +137,8 @@ def test_split_frames_specialised_equals_interpreter(gpu, monkeypatch, case):
         ops, shots = circuits.unrotated_surface_code_benchmark(11, 3), 3 * 2048 + 5
     elif case == "rotated_13x2":
         ops, shots = circuits.rotated_surface_code(13, 2, 5e-3).ops, 2 * 2048 + 77
+    elif case == "rotated_13x13":  # 929 planes: the split kernel with 24 noise slots
+        ops, shots = circuits.rotated_surface_code(13, 13, 2e-3).ops, 2048 + 77
     elif case == "rotated_11x3":
         ops, shots = circuits.rotated_surface_code(11, 3, 5e-3).ops, 3 * 2048 + 100
     else:
