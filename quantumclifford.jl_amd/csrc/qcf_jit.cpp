@@ -439,7 +439,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   // variant bit 3: the record spans less than 4 GB, so a record address is
   // one 64-bit base plus a 32-bit byte offset (one scalar multiply per
   // store instead of a 64-bit multiply-add)
-  const bool rec32 = (variant & 8) != 0;
+  bool rec32 = (variant & 8) != 0;
   // registers pay an LDS exchange per noisy plane per noise pass; LDS planes
   // pay LDS traffic per gate but take events for free, and measured faster
   // until events get dense (d=5 x 25: 0.62 vs 0.76 ms at p=1e-3, 2.1 vs
@@ -521,6 +521,11 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     const uint32_t k = std::min(P->nplanes, std::max(kmin, jit_split_regs()));
     if (k > kSplitMaxRegs) return false;
     if ((variant & 32) && k > jit_two_wave_regs()) return false;
+    // With more than 256 register planes the 64-bit record addressing
+    // measured faster than the 32-bit offsets (rotated d=13, 4e6 shots,
+    // p = 1e-3: 4.07 against 4.69 ms; every smaller frame the other way
+    // round, e.g. d=11 split 2.26 against 2.18 ms, d=5 x 25 0.45 against 0.39)
+    if (k > 256u) rec32 = false;
     std::vector<uint32_t> order(P->nplanes);
     for (uint32_t p = 0; p < P->nplanes; ++p) order[p] = p;
     std::stable_sort(order.begin(), order.end(),

@@ -226,3 +226,19 @@ def test_lds_mode_kernels_ask_for_ilp_scheduling(monkeypatch):
     assert minreg in two and tag not in two
     monkeypatch.setenv("QCF_JIT_SCHED", "0")
     assert tag not in source(lds, 1 | 8)
+
+
+def test_split_frames_with_many_register_planes_take_fewer_slots_and_64_bit_records(monkeypatch):
+    """Rotated d=13 x 13 (929 planes) would need more than 256 register planes
+    beside 80 noise slots: its split kernel takes 24 slots and 64-bit record
+    addresses; d=11 x 11 keeps 80 slots and 32-bit record offsets."""
+    big = source(qcf.Program(circuits.rotated_surface_code(13, 13, 1e-3).ops), 1 | 8 | 16)
+    small = source(qcf.Program(circuits.rotated_surface_code(11, 11, 1e-3).ops), 1 | 8 | 16)
+    kb, ks = big.split("qcf_jit_kernel")[-1], small.split("qcf_jit_kernel")[-1]
+    assert "(recb +" not in kb and "P.rec[" in kb
+    assert "(recb +" in ks and "P.rec[" not in ks
+    # its noise groups are cut to fit 24 slots: more event passes than with 80
+    monkeypatch.setenv("QCF_JIT_SLOTS", "80")
+    k80 = source(qcf.Program(circuits.rotated_surface_code(13, 13, 1e-3).ops),
+                 1 | 8 | 16).split("qcf_jit_kernel")[-1]
+    assert kb.count("qcfd::group_events") > k80.count("qcfd::group_events")
