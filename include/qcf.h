@@ -340,7 +340,7 @@ typedef struct qcf_tune {
                               fit in LDS and have more planes than this share
                               their tiles with the persistent kernel, which then
                               runs beside them: both take tiles from one counter
-                              per launch (0 = those with more than 1024 planes,
+                              per launch (0 = those with more than 768 planes,
                               0xFFFFFFFF = none); the records do not depend on
                               the split                                         */
 } qcf_tune;
@@ -357,8 +357,9 @@ typedef struct qcf_tune {
  * ONE persistent kernel for the circuits without one (frames beyond what
  * registers plus LDS hold); its workgroups pull (circuit, tile range) items
  * from a device queue.  By default the specialised launches whose frames
- * exceed LDS and have more than 1024 planes (their part-tile kernels hold
- * 32 shot words per CU) share their tiles with it: the
+ * exceed LDS and have more than 768 planes (split kernels at one wave per
+ * CU, part-tile kernels holding 32 shot words per CU) share their tiles
+ * with it: the
  * persistent kernel then starts beside them and, once its own items are
  * done, takes tiles from the same per-launch counters they take them from
  * (qcf_tune.steal_planes).  The last workgroup to leave resets the

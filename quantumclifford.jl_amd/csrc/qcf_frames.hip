@@ -1018,8 +1018,12 @@ int upload_items(qcf_batch *B) {
 // 512-circuit memory batch (d = 3..15, 1.6e7 to 2e7 shots per distance),
 // sharing took d = 15 (1237 planes) from 108 to 55 ms at 8 persistent waves
 // per CU, but d = 13 (929 planes) from 46 to 48-53 ms and d = 11 (665) from
-// 14 to 26-30 ms.
-constexpr uint32_t kStealPlanes = 1024;
+// 14 to 26-30 ms with the kernels of that time.  Since the d = 13 split
+// kernels took 24 noise slots and 64-bit record addresses, sharing d = 13 as
+// well measured best for the whole batch: 78.0-80.9 ms against 89.2-95.6
+// with d = 15 alone and 78.7-79.1 with d = 11 too (8 waves per CU, one or
+// two helper streams).
+constexpr uint32_t kStealPlanes = 768;
 constexpr uint32_t kStealWavesPerCU = 8;  // persistent grid when it shares tiles
 int share_tiles(qcf_batch *B, uint32_t planes) {
   if (planes == 0) planes = kStealPlanes;

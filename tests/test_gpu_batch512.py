@@ -42,7 +42,7 @@ def test_512_circuit_batch_equals_single_launches(gpu):
             bad.append((k, p.nq, s))
     assert not bad, bad[:10]
     # other launch shapes give the same bits: the persistent kernel sharing
-    # the tiles of the launches of more than 1024 planes, of all whose frames
+    # the tiles of the launches of more than 768 planes (the default), of all whose frames
     # exceed LDS, or of none, and every circuit of the largest frames on the persistent
     # kernel, started beside the specialised launches on one helper stream
     big = max(p.info.nplanes for p in progs)
