@@ -1199,10 +1199,15 @@ std::unordered_map<std::string, std::vector<char>> g_code_cache;  // source -> c
 }  // namespace
 
 // Planes go to registers only with at least this many expected noise events
-// per 2048-shot tile (QCF_JIT_REG_EVENTS).
+// per 2048-shot tile (QCF_JIT_REG_EVENTS, default 60,000).  d=5 x 25 (3874
+// sites), 10^7 shots, LDS against register mode: p = 2e-3 (15.9 k events per
+// tile) 0.50 / 0.68 ms, 3e-3 (23.8 k) 0.61 / 0.77, 5e-3 (39.7 k) 0.84 /
+// 0.93, 1e-2 (79.3 k) 1.43 / 1.35 -- since the LDS kernels raise the
+// priority of event passes and are scheduled for ILP (the threshold was
+// 20,000 before).
 double jit_reg_events() {
   const char *e = getenv("QCF_JIT_REG_EVENTS");
-  return e ? strtod(e, nullptr) : 20000.0;
+  return e ? strtod(e, nullptr) : 60000.0;
 }
 
 // Noise slots (256 bytes of LDS each) one event pass may use in register
