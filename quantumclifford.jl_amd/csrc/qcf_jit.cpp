@@ -1057,14 +1057,15 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   if ((!halfm && !emit(false, full)) || !emit(true, part)) return false;
   std::ostringstream src;
   src << "#define QCF_RTC 1\n";
-  // Machine-scheduler strategy by storage mode: LDS-mode kernels are
-  // scheduled for instruction-level parallelism (max-ilp; d=5 x 25, p =
-  // 1e-3, 10^7 shots: 0.393 -> 0.383 ms), two-wave split kernels for low
-  // register pressure (iterative-minreg; rotated d=9, p = 1e-3: 1.88 -> 1.74
-  // ms).  The other modes keep the default, which measured as fast or faster
-  // there than max-ilp, max-memory-clause, iterative-minreg and
-  // iterative-maxocc.
-  if (jit_sched_tags() && lds_mode)
+  // Machine-scheduler strategy by storage mode: LDS-mode and part-tile
+  // kernels are scheduled for instruction-level parallelism (max-ilp; d=5 x
+  // 25, p = 1e-3, 10^7 shots: 0.393 -> 0.383 ms; half tiles, rotated d=15,
+  // 2e6 shots: 5.36 -> 5.25 ms), two-wave split kernels for low register
+  // pressure (iterative-minreg; rotated d=9, p = 1e-3: 1.88 -> 1.74 ms).
+  // Register and one-wave split kernels keep the default, which measured
+  // about as fast or faster there than max-ilp, max-memory-clause,
+  // iterative-minreg and iterative-maxocc.
+  if (jit_sched_tags() && (lds_mode || halfm))
     src << kCompileOptTag << "-mllvm -amdgpu-sched-strategy=max-ilp\n";
   if (jit_sched_tags() && split && (variant & 32))
     src << kCompileOptTag << "-mllvm -amdgpu-sched-strategy=iterative-minreg\n";
