@@ -427,6 +427,26 @@ def test_batched_equals_single_launches(gpu, monkeypatch, batch_jit):
         assert len(diff) == 0, (k, p.nq, p.info.lds_bytes, s, len(diff), diff[:5].tolist())
 
 
+def test_batched_mixes_zero_rate_and_noisy_circuits(gpu):
+    """A batch holding the same code at p = 0 (noise ops kept at rate 0: its
+    kernel drops their event passes, and record-only launches of it take the
+    linear kernel) and at p > 0, noiseless circuits without noise ops, and a
+    d=13 circuit (split kernel, 24 noise slots): every record equals its
+    single launch."""
+    cs = [circuits.rotated_surface_code(5, 5, 0.0), circuits.rotated_surface_code(5, 5, 2e-3),
+          circuits.rotated_surface_code(3, 3, 0.0, noise=False),
+          circuits.rotated_surface_code(5, 5, 0.0), circuits.rotated_surface_code(13, 13, 1e-3),
+          circuits.rotated_surface_code(7, 7, 0.0)]
+    progs = [qcf.Program(c.ops) for c in cs]
+    shots = [3000, 5000, 2049, 777, 4200, 2048]
+    seeds = [11, 12, 13, 14, 15, 16]
+    from qcf import batch
+    outs = batch.sample_batched(progs, shots, seeds, graph=True)
+    for k, (p, s, sd, o) in enumerate(zip(progs, shots, seeds, outs)):
+        single = qcf.pftrajectories(p, s, seed=sd, keep_frame=False).record_host()
+        assert np.array_equal(o, single), k
+
+
 def test_global_frame_storage_matches_lds(gpu, monkeypatch):
     """Frames too large for LDS run from global memory; forcing that path on
     circuits that fit must give the same bits."""
