@@ -534,7 +534,41 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     for (uint32_t p = 0; p < P->nplanes; ++p)
       if (!sreg[p]) lidx[p] = nlds++;
   }
-  auto in_reg = [&](uint32_t p) { return regs || (hyb && p >= qpl) || (split && sreg[p]); };
+  // Part-tile kernels: up to jit_part_regs() planes that no noise site names
+  // sit in registers, those the most operations touch; the other planes keep
+  // their LDS places (part_lds: LDS planes up to the last of them).  Serial
+  // chains through such planes then run without an LDS round trip per step:
+  // x_diag_circuit_noisy_measurement(1000) XORs every qubit into the target
+  // and 7 classical slots into each of its 1000 records.
+  std::vector<uint8_t> preg(P->nplanes, 0);
+  uint32_t part_lds = P->nplanes;
+  if (halfm && jit_part_regs()) {
+    std::vector<int64_t> score(P->nplanes, 0);
+    std::vector<uint8_t> named(P->nplanes, 0);
+    std::vector<uint32_t> pl;
+    for (size_t i = 0; i < code.size(); ++i) {
+      pl.clear();
+      instr_planes(code, i, &pl);
+      const bool noise = (code[i].op & 0xFFu) == I_NOISEG;
+      for (uint32_t p : pl)
+        if (p < P->nplanes) {
+          if (noise) named[p] = 1;
+          else ++score[p];
+        }
+      if (noise) i += code[i].op >> 8;
+    }
+    std::vector<uint32_t> cand;
+    for (uint32_t p = 0; p < P->nplanes; ++p)
+      if (!named[p] && score[p] >= 2) cand.push_back(p);
+    std::stable_sort(cand.begin(), cand.end(),
+                     [&](uint32_t x, uint32_t y) { return score[x] > score[y]; });
+    if (cand.size() > jit_part_regs()) cand.resize(jit_part_regs());
+    for (uint32_t p : cand) preg[p] = 1;
+    while (part_lds > 1u && preg[part_lds - 1u]) --part_lds;
+  }
+  auto in_reg = [&](uint32_t p) {
+    return regs || (hyb && p >= qpl) || (split && sreg[p]) || (halfm && preg[p]);
+  };
   // LDS plane of an LDS-resident plane
   auto lp_of = [&](uint32_t p) { return split ? lidx[p] : p; };
   const uint32_t slot_base = split ? nlds : 0u;  // first noise slot
@@ -790,7 +824,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
     return o.str();
   };
   auto emit = [&](bool tail, std::ostringstream &body) -> bool {
-  if (regs || hyb || split) {
+  if (regs || hyb || split || halfm) {
     const char *sep = "    uint32_t ";
     for (uint32_t p = 0; p < P->nplanes; ++p)
       if (in_reg(p)) {
@@ -1087,7 +1121,12 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   else
     src << "  const uint32_t bx = blockIdx.x, gx = gridDim.x;\n";
   {
-    const uint32_t nl = glob ? 0u : split ? nlds + max_slots : regs ? max_slots : hyb ? qpl : P->nplanes;
+    const uint32_t nl = glob    ? 0u
+                        : split ? nlds + max_slots
+                        : regs  ? max_slots
+                        : hyb   ? qpl
+                        : halfm ? part_lds
+                                : P->nplanes;
     for (uint32_t w = 1; w * per_window < nl; ++w)
       src << "  uint32_t lb" << w << " = " << w * 16384u << "u + l;\n  asm volatile(\"\" : \"+v\"(lb"
           << w << "));\n";
@@ -1146,7 +1185,7 @@ bool generate(const qcf_program *P, int variant, std::string *out, uint32_t *lds
   *out = src.str();
   if (lds_bytes)
     *lds_bytes = glob    ? 0u
-                 : halfm ? P->nplanes * pw * 4u
+                 : halfm ? part_lds * pw * 4u
                  : split ? (nlds + max_slots) * 256u
                  : regs  ? std::max<uint32_t>(max_slots, 1u) * 256u
                  : hyb  ? qpl * 256u
@@ -1283,6 +1322,13 @@ int jit_pass_prio() {
   return e && e[0] >= '0' && e[0] <= '3' ? e[0] - '0' : -1;
 }
 
+// Planes a part-tile kernel may keep in registers (QCF_JIT_PART_REGS,
+// default 32; 0 keeps every plane in LDS).
+uint32_t jit_part_regs() {
+  const char *e = getenv("QCF_JIT_PART_REGS");
+  return e ? (uint32_t)strtoul(e, nullptr, 10) : 32u;
+}
+
 // QCF_JIT_SPLIT=0 leaves frames larger than LDS to the global-memory paths.
 bool jit_split_enabled() {
   const char *e = getenv("QCF_JIT_SPLIT");
@@ -1355,7 +1401,7 @@ std::string jit_knob_signature() {
   static const char *const kKnobs[] = {
       "QCF_JIT_REG_EVENTS", "QCF_JIT_SLOTS", "QCF_JIT_REG_PLANES", "QCF_JIT_MAX_INSTR",
       "QCF_JIT_SPLIT_REGS", "QCF_JIT_SPLIT_BARRIER", "QCF_JIT_SPLIT_TOUCHED", "QCF_JIT_NT",
-      "QCF_JIT_NO_LOOPS", "QCF_JIT_HOIST", "QCF_JIT_PREFETCH", "QCF_JIT_TWO_WAVE_REGS", "QCF_JIT_PASS_PRIO", "QCF_JIT_OPTS", "QCF_JIT_SCHED"};
+      "QCF_JIT_NO_LOOPS", "QCF_JIT_HOIST", "QCF_JIT_PREFETCH", "QCF_JIT_TWO_WAVE_REGS", "QCF_JIT_PASS_PRIO", "QCF_JIT_OPTS", "QCF_JIT_SCHED", "QCF_JIT_PART_REGS"};
   std::string s;
   for (const char *k : kKnobs) {
     const char *e = getenv(k);

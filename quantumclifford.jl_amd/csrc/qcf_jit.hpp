@@ -52,6 +52,7 @@ uint32_t jit_split_regs();
 uint32_t jit_two_wave_regs();
 int jit_pass_prio();
 bool jit_sched_tags();
+uint32_t jit_part_regs();
 // Source lines starting with this tag carry compiler options for the kernel.
 constexpr const char *kCompileOptTag = "// qcf-compile-options: ";
 size_t jit_split_barrier();

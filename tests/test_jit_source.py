@@ -106,6 +106,47 @@ def test_quarter_tile_source_stays_in_lds():
     assert "const bool lh = l < 16u;" in src and "FlipPart<16> flip{L, hf}" in src
 
 
+def _part_lds_top(src, pw):
+    hi = 0
+    for w, off in re.findall(r"L\[lb(\d+) \+ (\d+)u\]", src):
+        hi = max(hi, int(w) * 16384 + int(off) + pw)
+    for off in re.findall(r"L\[(\d+)u \+ l\]", src):
+        hi = max(hi, int(off) + pw)
+    return hi
+
+
+def _reg_planes(src):
+    m = re.search(r"^    uint32_t (r\d+ = 0u(?:, r\d+ = 0u)*);$", src, re.M)
+    return [int(x) for x in re.findall(r"r(\d+) = 0u", m.group(1))] if m else []
+
+
+def test_part_tile_kernels_keep_noise_free_planes_in_registers(monkeypatch):
+    """x_diag_circuit_noisy_measurement(1000): every qubit 0..999 is measured
+    through a noise site and XORed into the target (qubit 1000, planes 2000
+    and 2001), and 7 classical slots (planes 2004..2010) go into each of its
+    records.  The quarter-tile kernel keeps the target and the slots (no
+    noise site names them) in registers and addresses LDS only below plane
+    2004; QCF_JIT_PART_REGS=0 puts them back in LDS.  Rotated d=15 (337
+    slots) takes at most the default cap of 32, none of them a noise site's."""
+    prog = qcf.Program(circuits.x_diag_circuit_noisy_measurement(1000))
+    assert prog.info.nplanes == 2 * prog.info.nqubits + 7
+    qpl = 2 * prog.info.nqubits
+    src = source(prog, 1 | 4 | 8 | 128)
+    assert _reg_planes(src) == [2000, 2001] + list(range(2004, 2011))
+    assert 0 < _part_lds_top(src, 16) <= qpl * 16
+    assert "L[lb1 + 15616u]" not in src  # plane 2000's LDS place
+    monkeypatch.setenv("QCF_JIT_PART_REGS", "0")
+    src0 = source(prog, 1 | 4 | 8 | 128)
+    assert _reg_planes(src0) == []
+    assert qpl * 16 < _part_lds_top(src0, 16) <= prog.info.nplanes * 16
+    monkeypatch.delenv("QCF_JIT_PART_REGS")
+    d15 = qcf.Program(circuits.rotated_surface_code(15, 2, 5e-3).ops)
+    s15 = source(d15, 1 | 4 | 8 | 64)
+    regs = _reg_planes(s15)
+    assert 0 < len(regs) <= 32 and min(regs) >= 2 * d15.info.nqubits
+    assert _part_lds_top(s15, 32) <= d15.info.nplanes * 32
+
+
 def test_large_frames_with_many_noise_groups_stay_on_the_interpreter():
     """1,000 noise sites on one qubit, each between two gates on it, stay
     1,000 separate noise groups, which would compile for minutes: split and
