@@ -1323,10 +1323,13 @@ int jit_pass_prio() {
 }
 
 // Planes a part-tile kernel may keep in registers (QCF_JIT_PART_REGS,
-// default 32; 0 keeps every plane in LDS).
+// default 128; 0 keeps every plane in LDS).  Rotated d=15 (337 noise-free
+// classical slots), half tiles, 2e6 shots at p=1e-3 / 1e6 at p=1e-4:
+// 5.24 / 1.72 ms with none, 5.21 / 1.70 with 32, 5.16 / 1.68 with 128,
+// 5.18-5.21 / 1.70-1.71 with 192, 256 or all.
 uint32_t jit_part_regs() {
   const char *e = getenv("QCF_JIT_PART_REGS");
-  return e ? (uint32_t)strtoul(e, nullptr, 10) : 32u;
+  return e ? (uint32_t)strtoul(e, nullptr, 10) : 128u;
 }
 
 // QCF_JIT_SPLIT=0 leaves frames larger than LDS to the global-memory paths.

@@ -127,7 +127,7 @@ def test_part_tile_kernels_keep_noise_free_planes_in_registers(monkeypatch):
     records.  The quarter-tile kernel keeps the target and the slots (no
     noise site names them) in registers and addresses LDS only below plane
     2004; QCF_JIT_PART_REGS=0 puts them back in LDS.  Rotated d=15 (337
-    slots) takes at most the default cap of 32, none of them a noise site's."""
+    slots) takes at most the default cap of 128, none of them a noise site's."""
     prog = qcf.Program(circuits.x_diag_circuit_noisy_measurement(1000))
     assert prog.info.nplanes == 2 * prog.info.nqubits + 7
     qpl = 2 * prog.info.nqubits
@@ -143,7 +143,7 @@ def test_part_tile_kernels_keep_noise_free_planes_in_registers(monkeypatch):
     d15 = qcf.Program(circuits.rotated_surface_code(15, 2, 5e-3).ops)
     s15 = source(d15, 1 | 4 | 8 | 64)
     regs = _reg_planes(s15)
-    assert 0 < len(regs) <= 32 and min(regs) >= 2 * d15.info.nqubits
+    assert 0 < len(regs) <= 128 and min(regs) >= 2 * d15.info.nqubits
     assert _part_lds_top(s15, 32) <= d15.info.nplanes * 32
 
 
